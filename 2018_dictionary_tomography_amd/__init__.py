@@ -1,0 +1,20 @@
+"""MI355X-native Gaussian-atom tomography: the projector, its derivatives,
+the volume render, the Fourier evaluation and the fitters of
+robtovey/2018_dictionary_tomography, computed by hand-written gfx950 HIP
+kernels behind the C ABI in include/gausstomo.h.
+
+The package name starts with a digit, so import it with
+``importlib.import_module("2018_dictionary_tomography_amd")``.
+"""
+from . import _lib, ops, spaces
+from ._lib import GTError
+from .atomFuncs import GaussTomo
+from .ops import L2derivs_RadProj, RadProj, VolProj, derivs_RadProj
+from .spaces import (AtomElement, AtomSpace, ProjElement, ProjSpace, VolElement, VolSpace, axis_tilt_geometry,
+                     boundary_nodes, cell_centres, perp, theta_to_vec)
+
+__all__ = [
+    "GTError", "GaussTomo", "RadProj", "derivs_RadProj", "L2derivs_RadProj", "VolProj", "AtomSpace",
+    "AtomElement", "ProjSpace", "ProjElement", "VolSpace", "VolElement", "theta_to_vec", "perp",
+    "axis_tilt_geometry", "cell_centres", "boundary_nodes", "ops", "spaces",
+]

@@ -1,0 +1,89 @@
+"""ctypes binding of libgausstomo.so (the C ABI declared in include/gausstomo.h).
+
+The library is the only compute path: if it is missing this module raises,
+there is no fallback.  torch is imported first so that the library binds to
+the HIP runtime torch already loaded.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch  # noqa: F401  (loads the HIP runtime before the library)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgausstomo.so")
+
+GT_MODE_LINEAR, GT_MODE_L2, GT_MODE_RESID = 0, 1, 2
+
+
+class GTError(RuntimeError):
+    """Error reported by the HIP library (argument, workspace or HIP error)."""
+
+
+class gt_atoms(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("n", ctypes.c_int), ("iso", ctypes.c_int),
+                ("I", ctypes.c_void_p), ("x", ctypes.c_void_p), ("r", ctypes.c_void_p)]
+
+
+class gt_geom(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("lenT", ctypes.c_int), ("nA", ctypes.c_int), ("n0", ctypes.c_int),
+                ("n1", ctypes.c_int), ("sorted", ctypes.c_int), ("T", ctypes.c_void_p), ("w0", ctypes.c_void_p),
+                ("w1", ctypes.c_void_p), ("p0", ctypes.c_void_p), ("p1", ctypes.c_void_p)]
+
+
+class gt_vol(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("m0", ctypes.c_int), ("m1", ctypes.c_int), ("m2", ctypes.c_int),
+                ("sorted", ctypes.c_int), ("y0", ctypes.c_void_p), ("y1", ctypes.c_void_p), ("y2", ctypes.c_void_p)]
+
+
+class gt_kgrid(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("m0", ctypes.c_int), ("m1", ctypes.c_int), ("m2", ctypes.c_int),
+                ("sorted", ctypes.c_int), ("k0", ctypes.c_void_p), ("k1", ctypes.c_void_p), ("k2", ctypes.c_void_p)]
+
+
+_P = ctypes.POINTER
+_vp, _sz, _ci = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
+
+# name -> (restype, argtypes); every symbol declared in include/gausstomo.h
+SIGNATURES = {
+    "gt_last_error": (ctypes.c_char_p, []),
+    "gt_version": (ctypes.c_char_p, []),
+    "gt_forward_workspace": (_sz, [_P(gt_atoms), _P(gt_geom)]),
+    "gt_forward": (_ci, [_P(gt_atoms), _P(gt_geom), _vp, _vp, _sz, _vp]),
+    "gt_derivs_workspace": (_sz, [_P(gt_atoms), _P(gt_geom), _ci]),
+    "gt_derivs": (_ci, [_P(gt_atoms), _P(gt_geom), _vp, _ci, _ci, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "gt_volume_workspace": (_sz, [_P(gt_atoms), _P(gt_vol)]),
+    "gt_volume": (_ci, [_P(gt_atoms), _P(gt_vol), _vp, _vp, _sz, _vp]),
+    "gt_ft_workspace": (_sz, [_P(gt_atoms), _P(gt_kgrid)]),
+    "gt_ft_eval": (_ci, [_P(gt_atoms), _P(gt_kgrid), _vp, _vp, _sz, _vp]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libgausstomo.so and declare every exported entry point."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with __graft_entry__.build() "
+                          "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        msg = load().gt_last_error().decode(errors="replace")
+        raise GTError(f"{what} failed (rc={rc}): {msg}")
+
+
+def stream_handle(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)

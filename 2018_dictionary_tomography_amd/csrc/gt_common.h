@@ -1,0 +1,79 @@
+// gt_common.h -- shared host/device helpers of the HIP library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gausstomo.h"
+#include "gt_pair.h"
+
+namespace gt {
+
+// Thread-local last error message (gt_last_error).
+void set_error(const char *fmt, ...);
+
+#define GT_CHECK_ARG(cond, ...)          \
+    do {                                 \
+        if (!(cond)) {                   \
+            ::gt::set_error(__VA_ARGS__); \
+            return GT_EINVAL;            \
+        }                                \
+    } while (0)
+
+#define GT_HIP(call)                                                          \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) {                                               \
+            ::gt::set_error("%s failed: %s", #call, hipGetErrorString(e_));  \
+            return GT_EHIP;                                                   \
+        }                                                                     \
+    } while (0)
+
+constexpr size_t ALIGN = 256;
+inline size_t align_up(size_t v) { return (v + ALIGN - 1) / ALIGN * ALIGN; }
+
+int check_atoms(const gt_atoms *a);
+int check_geom(const gt_geom *g, int dim);
+
+// Atom j into a PairGeom (iso: U = rho * Id).
+template <int D> __device__ __host__ inline void load_atom(const gt_atoms &A, int j, PairGeom<D> &g)
+{
+    constexpr int NR = Dims<D>::NR;
+    for (int i = 0; i < 3; i++)
+        for (int k = 0; k < 3; k++) g.U[i][k] = 0.0;
+    if (A.iso) {
+        double rho = A.r[j];
+        for (int i = 0; i < D; i++) g.U[i][i] = rho;
+    } else {
+        for (int k = 0; k < NR; k++) g.U[slot_row(D, k)][slot_col(D, k)] = A.r[(size_t)j * NR + k];
+    }
+    for (int i = 0; i < 3; i++) g.x[i] = (i < D) ? (double)A.x[(size_t)j * D + i] : 0.0;
+    g.I = A.I[j];
+}
+
+// Orientation a into a PairGeom: T and the detector columns G.
+template <int D> __device__ __host__ inline void load_angle(const gt_geom &G, int a, PairGeom<D> &g)
+{
+    for (int i = 0; i < 3; i++) {
+        g.T[i] = 0.0;
+        g.G[0][i] = 0.0;
+        g.G[1][i] = 0.0;
+    }
+    const int L = G.lenT;
+    for (int i = 0; i < L; i++) {
+        g.T[i] = G.T[(size_t)a * L + i];
+        g.G[0][i] = G.w0[(size_t)a * L + i];
+    }
+    if (D == 3) {
+        if (L == 3) {
+            for (int i = 0; i < 3; i++) g.G[1][i] = G.w1[(size_t)a * 3 + i];
+        } else {
+            g.G[1][2] = 1.0;  // 2.5D: y = (p0 w0[0], p0 w0[1], p1), NewtonBases.py:684-688
+        }
+    }
+}
+
+// Wave64 helpers.
+__device__ inline int lane_id() { return threadIdx.x & 63; }
+
+}  // namespace gt

@@ -1,0 +1,477 @@
+// gt_pair.h -- per-(atom, angle) FP64 algebra of the Gaussian-atom projector.
+//
+// The reference evaluates, for every ray, the line integral of one atom
+// (GaussDictCode/bin/NewtonBases.py:260-310) and its first and second
+// derivatives with respect to theta = [I, x, r] (NewtonBases.py:313-527).
+// For a fixed atom and a fixed projection angle that line integral is a 2D
+// (1D for 2D problems) Gaussian on the detector:
+//
+//   R(p) = alpha * exp(-30 m(p)) * [m(p) <= 1],   alpha = I / |U T|,
+//   m(p) = d^T B d,   d = (p - p*, 1),
+//
+// where U is the upper-triangular atom matrix (r slots, NewtonBases.py:249-251,
+// 319-325), T the ray direction, y = G p the detector-to-volume map
+// (NewtonBases.py:676-688, NewtonBases2D.py:246-249) and
+//
+//   B_ab = h_a^T Q h_b,  Q = U^T (Id - t t^T / s) U,  t = U T,  s = |t|^2,
+//   h_k = G e_k (k < K),  h_K = G p* - x.
+//
+// This identity is exact (the reference's m = |(Id - t^ t^T) U (y - x)|^2
+// expands to it), so the projector is: one FP64 setup per (atom, angle) here,
+// then one quadratic, one compare and one exp2 per pixel in FP32.
+//
+// Derivatives use the same structure.  R depends on theta only through the
+// intermediate vector gamma = (alpha, B_ab [a <= b]), and R is a polynomial in
+// d times exp(-30 m), so every pixel sum the reference accumulates
+// (NewtonBases.py:833-839, 1034-1039) is a contraction of the pixel moments
+//
+//   S1[mono] = sum_p w1(p) mono(p - p*),   S2[mono] = sum_p E(p)^2 mono(p - p*)
+//
+// over the monomials of degree <= 4 (15 in 2D detectors, 5 in 1D), with
+// w1 = (lambda alpha E - sigma C) E chosen per mode.  pair_finish() turns those
+// moments into this angle's contribution to (f, grad, Hessian) with the exact
+// chain rule gamma -> theta, in FP64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace gt {
+
+// -30 * log2(e): m -> exponent of exp2
+constexpr double KAPPA = -43.28085122666891;
+constexpr int MAXD = 3;
+
+// r-slot -> (row, col) of U.  NewtonBases.py:319-325, NewtonBases2D.py:84-87.
+__host__ __device__ inline int slot_row(int D, int k)
+{
+    return D == 3 ? (k < 3 ? k : (k == 3 ? 0 : (k == 4 ? 1 : 0))) : (k < 2 ? k : 0);
+}
+__host__ __device__ inline int slot_col(int D, int k)
+{
+    return D == 3 ? (k < 3 ? k : (k == 3 ? 1 : 2)) : (k < 2 ? k : 1);
+}
+
+template <int D> struct Dims {
+    static constexpr int NR = D * (D + 1) / 2;  // r slots
+    static constexpr int P = 1 + D + NR;        // parameters [I, x, r]
+    static constexpr int K = D - 1;             // detector dims
+    static constexpr int NB = (K + 1) * (K + 2) / 2;  // B entries a <= b over {0..K}
+    static constexpr int NG = 1 + NB;           // gamma = (alpha, B)
+    static constexpr int NM = (K == 2) ? 15 : 5;  // monomials of degree <= 4
+};
+
+// Monomial index of dp0^i dp1^j (K = 2) or dp^i (K = 1).
+__host__ __device__ inline int mono2(int i, int j)
+{
+    int deg = i + j;
+    return deg * (deg + 1) / 2 + j;
+}
+
+// B-pair index p -> (a, b), a <= b, over {0..K}.
+template <int K> __host__ __device__ inline void bpair(int p, int &a, int &b)
+{
+    if (K == 2) {
+        const int A[6] = {0, 0, 0, 1, 1, 2}, Bv[6] = {0, 1, 2, 1, 2, 2};
+        a = A[p];
+        b = Bv[p];
+    } else {
+        const int A[3] = {0, 0, 1}, Bv[3] = {0, 1, 1};
+        a = A[p];
+        b = Bv[p];
+    }
+}
+
+// Exponents (of dp0, dp1) of d_a: d = (dp0, dp1, 1) for K = 2, (dp, 1) for K = 1.
+template <int K> __host__ __device__ inline void dexp(int a, int &e0, int &e1)
+{
+    e0 = (a == 0 && K >= 1) ? 1 : 0;
+    e1 = (K == 2 && a == 1) ? 1 : 0;
+}
+
+template <int K> __host__ __device__ inline int mono_index(int e0, int e1)
+{
+    return K == 2 ? mono2(e0, e1) : e0;
+}
+
+// Geometry of one (atom, angle) pair in FP64.
+template <int D> struct PairGeom {
+    double U[3][3];
+    double T[3];
+    double G[2][3];  // detector basis columns g_k: y = sum_k p_k g_k
+    double x[3];
+    double I;
+};
+
+// Quantities shared by setup and finish.
+template <int D> struct PairAlg {
+    double t[3], s, alpha;
+    double h[3][3];   // h_0..h_K
+    double u[3][3];   // U h_a
+    double mu[3];     // (u_a . t) / s
+    double ut[3][3];  // M u_a
+    double ht[3][3];  // h_a - mu_a T
+    double B[3][3];   // B_ab
+};
+
+template <int D> __host__ __device__ inline void matvec_upper(const double U[3][3], const double *v, double *out)
+{
+    for (int i = 0; i < D; i++) {
+        double acc = 0.0;
+        for (int j = i; j < D; j++) acc += U[i][j] * v[j];
+        out[i] = acc;
+    }
+}
+
+// Fill PairAlg at expansion point pstar.
+template <int D>
+__host__ __device__ inline void pair_alg(const PairGeom<D> &g, const double *pstar, PairAlg<D> &a)
+{
+    constexpr int K = Dims<D>::K;
+    matvec_upper<D>(g.U, g.T, a.t);
+    a.s = 0.0;
+    for (int i = 0; i < D; i++) a.s += a.t[i] * a.t[i];
+    a.alpha = g.I / sqrt(a.s);
+    for (int k = 0; k < K; k++)
+        for (int i = 0; i < D; i++) a.h[k][i] = g.G[k][i];
+    for (int i = 0; i < D; i++) {
+        double y = 0.0;
+        for (int k = 0; k < K; k++) y += pstar[k] * g.G[k][i];
+        a.h[K][i] = y - g.x[i];
+    }
+    for (int b = 0; b <= K; b++) {
+        matvec_upper<D>(g.U, a.h[b], a.u[b]);
+        double ip = 0.0;
+        for (int i = 0; i < D; i++) ip += a.u[b][i] * a.t[i];
+        a.mu[b] = ip / a.s;
+        for (int i = 0; i < D; i++) {
+            a.ut[b][i] = a.u[b][i] - a.mu[b] * a.t[i];
+            a.ht[b][i] = a.h[b][i] - a.mu[b] * g.T[i];
+        }
+    }
+    for (int b0 = 0; b0 <= K; b0++)
+        for (int b1 = 0; b1 <= K; b1++) {
+            double v = 0.0;
+            for (int i = 0; i < D; i++) v += a.u[b0][i] * a.ut[b1][i];
+            a.B[b0][b1] = v;
+        }
+}
+
+// FP32 record of one (atom, angle) pair, consumed by the pixel kernels.
+struct PairRec {
+    float alpha;
+    float c_hi[2], c_lo[2];  // expansion point p* = c_hi + c_lo
+    float q[6];              // kappa * m coefficients (see pair_q)
+    float pad;
+    int lo[2], hi[2];        // inclusive detector index box, lo > hi if empty
+};
+static_assert(sizeof(PairRec) == 64, "PairRec must stay 64 bytes");
+
+// First index i with p[i] >= v (p ascending, length n).
+__host__ __device__ inline int lower_idx(const float *p, int n, double v)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if ((double)p[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+// Last index i with p[i] <= v (p ascending), -1 if none.
+__host__ __device__ inline int upper_idx(const float *p, int n, double v)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if ((double)p[mid] <= v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo - 1;
+}
+
+// Setup of one pair: expansion point, FP32 quadratic and the detector box.
+// det[k], nd[k]: detector coordinate vectors; cull == 0 disables the box.
+template <int D>
+__host__ __device__ inline void pair_setup(const PairGeom<D> &g, const float *const *det, const int *nd,
+                                           int cull, PairRec &rec)
+{
+    constexpr int K = Dims<D>::K;
+    // Quadratic in p around the origin: m(p) = p^T A p + 2 b^T p + c.
+    double zero[2] = {0.0, 0.0};
+    PairAlg<D> a0;
+    pair_alg<D>(g, zero, a0);
+    double pstar[2] = {0.0, 0.0};
+    bool degenerate = false;
+    if (K == 2) {
+        double A00 = a0.B[0][0], A01 = a0.B[0][1], A11 = a0.B[1][1];
+        double b0 = a0.B[0][2], b1 = a0.B[1][2];
+        double det2 = A00 * A11 - A01 * A01;
+        if (det2 > 1e-12 * A00 * A11 && A00 > 0.0 && A11 > 0.0) {
+            pstar[0] = -(A11 * b0 - A01 * b1) / det2;
+            pstar[1] = -(A00 * b1 - A01 * b0) / det2;
+        } else {
+            degenerate = true;
+        }
+    } else {
+        double A = a0.B[0][0], b = a0.B[0][1];
+        if (A > 0.0) pstar[0] = -b / A;
+        else degenerate = true;
+    }
+    PairAlg<D> a;
+    pair_alg<D>(g, pstar, a);
+    rec.alpha = (float)a.alpha;
+    for (int k = 0; k < 2; k++) {
+        float hi = (float)pstar[k];
+        rec.c_hi[k] = hi;
+        rec.c_lo[k] = (float)(pstar[k] - (double)hi);
+    }
+    const double kp = KAPPA;
+    for (int i = 0; i < 6; i++) rec.q[i] = 0.0f;
+    if (K == 2) {
+        rec.q[0] = (float)(kp * a.B[0][0]);
+        rec.q[1] = (float)(2.0 * kp * a.B[0][1]);
+        rec.q[2] = (float)(kp * a.B[1][1]);
+        rec.q[3] = (float)(2.0 * kp * a.B[0][2]);
+        rec.q[4] = (float)(2.0 * kp * a.B[1][2]);
+        rec.q[5] = (float)(kp * a.B[2][2]);
+    } else {
+        rec.q[0] = (float)(kp * a.B[0][0]);
+        rec.q[3] = (float)(2.0 * kp * a.B[0][1]);
+        rec.q[5] = (float)(kp * a.B[1][1]);
+    }
+    rec.pad = 0.0f;
+    // Detector box of {m <= 1}.
+    for (int k = 0; k < 2; k++) {
+        rec.lo[k] = 0;
+        rec.hi[k] = (k < K) ? nd[k] - 1 : 0;
+    }
+    if (!cull || degenerate) return;
+    double ctr[2], hw[2];
+    bool empty = false;
+    if (K == 2) {
+        double A00 = a.B[0][0], A01 = a.B[0][1], A11 = a.B[1][1];
+        double det2 = A00 * A11 - A01 * A01;
+        double i00 = A11 / det2, i01 = -A01 / det2, i11 = A00 / det2;
+        double b0 = a.B[0][2], b1 = a.B[1][2];
+        double c0 = -(i00 * b0 + i01 * b1), c1 = -(i01 * b0 + i11 * b1);
+        double rho2 = 1.0 - a.B[2][2] + (b0 * (i00 * b0 + i01 * b1) + b1 * (i01 * b0 + i11 * b1));
+        if (!(rho2 >= 0.0)) empty = true;
+        ctr[0] = pstar[0] + c0;
+        ctr[1] = pstar[1] + c1;
+        hw[0] = sqrt(fmax(rho2, 0.0) * i00);
+        hw[1] = sqrt(fmax(rho2, 0.0) * i11);
+    } else {
+        double A = a.B[0][0], b = a.B[0][1];
+        double rho2 = 1.0 - a.B[1][1] + b * b / A;
+        if (!(rho2 >= 0.0)) empty = true;
+        ctr[0] = pstar[0] - b / A;
+        hw[0] = sqrt(fmax(rho2, 0.0) / A);
+    }
+    for (int k = 0; k < K; k++) {
+        double pad = 1e-5 * hw[k] + 1e-7;
+        if (empty || !(hw[k] == hw[k])) {
+            rec.lo[k] = 1;
+            rec.hi[k] = 0;
+            continue;
+        }
+        rec.lo[k] = lower_idx(det[k], nd[k], ctr[k] - hw[k] - pad);
+        rec.hi[k] = upper_idx(det[k], nd[k], ctr[k] + hw[k] + pad);
+    }
+}
+
+// FP32 exponent q = kappa * m at offset (dp0, dp1).
+__host__ __device__ inline float pair_q(const float *q, float dp0, float dp1)
+{
+    float row = fmaf(q[0], dp0, fmaf(q[1], dp1, q[3]));
+    float col = fmaf(fmaf(q[2], dp1, q[4]), dp1, q[5]);
+    return fmaf(row, dp0, col);
+}
+
+// Moment-to-derivative contraction for one pair, in FP64.
+//   S1[NM]: sum w1 * mono,  S2[NM]: sum E^2 * mono (used when gn != 0)
+//   The pixel weight is w1 = (lambda alpha E - sigma C) E; the finishing
+//   only needs to know that v = sum res dR/dgamma uses w1 with "res" the
+//   residual (L2, residual modes) or C itself (linear mode).
+// Accumulates into grad[P] and hess[P][P] (full, symmetric).
+template <int D>
+__host__ __device__ inline void pair_finish(const PairGeom<D> &g, const double *pstar, const double *S1,
+                                            const double *S2, int gn, int want_hess, double *grad, double *hess)
+{
+    using Dm = Dims<D>;
+    constexpr int K = Dm::K, NB = Dm::NB, NG = Dm::NG, P = Dm::P, NR = Dm::NR;
+    PairAlg<D> a;
+    pair_alg<D>(g, pstar, a);
+    const double al = a.alpha, s = a.s;
+
+    // psi_p = (2 - delta_ab) d_a d_b: factor and exponents
+    double fac[NB];
+    int pe0[NB], pe1[NB];
+    for (int p = 0; p < NB; p++) {
+        int x0, x1, ea0, ea1, eb0, eb1;
+        bpair<K>(p, x0, x1);
+        dexp<K>(x0, ea0, ea1);
+        dexp<K>(x1, eb0, eb1);
+        fac[p] = (x0 == x1) ? 1.0 : 2.0;
+        pe0[p] = ea0 + eb0;
+        pe1[p] = ea1 + eb1;
+    }
+    // v = sum res dR/dgamma ; Kg = sum dR dR^T (gn) + sum res d2R
+    double v[NG], Kg[NG][NG];
+    v[0] = S1[0];
+    for (int p = 0; p < NB; p++) v[1 + p] = -30.0 * al * fac[p] * S1[mono_index<K>(pe0[p], pe1[p])];
+    Kg[0][0] = gn ? S2[0] : 0.0;
+    for (int p = 0; p < NB; p++) {
+        int m = mono_index<K>(pe0[p], pe1[p]);
+        double val = -30.0 * fac[p] * (S1[m] + (gn ? al * S2[m] : 0.0));
+        Kg[0][1 + p] = val;
+        Kg[1 + p][0] = val;
+    }
+    for (int p = 0; p < NB; p++)
+        for (int q = p; q < NB; q++) {
+            int m = mono_index<K>(pe0[p] + pe0[q], pe1[p] + pe1[q]);
+            double val = 900.0 * fac[p] * fac[q] * (al * S1[m] + (gn ? al * al * S2[m] : 0.0));
+            Kg[1 + p][1 + q] = val;
+            Kg[1 + q][1 + p] = val;
+        }
+
+    // Jacobian J[NG][P] of gamma w.r.t. theta = [I, x, r].
+    double J[NG][P];
+    for (int i = 0; i < NG; i++)
+        for (int j = 0; j < P; j++) J[i][j] = 0.0;
+    double rs = 1.0 / sqrt(s), rs3 = rs / s;
+    J[0][0] = rs;
+    for (int k = 0; k < NR; k++) {
+        int i = slot_row(D, k), j = slot_col(D, k);
+        J[0][1 + D + k] = -g.I * rs3 * a.t[i] * g.T[j];
+    }
+    // Q h_a = U^T (M u_a)
+    double Qh[3][3];
+    for (int b = 0; b <= K; b++)
+        for (int c = 0; c < D; c++) {
+            double acc = 0.0;
+            for (int i = 0; i <= c; i++) acc += g.U[i][c] * a.ut[b][i];
+            Qh[b][c] = acc;
+        }
+    for (int p = 0; p < NB; p++) {
+        int x0, x1;
+        bpair<K>(p, x0, x1);
+        for (int c = 0; c < D; c++) {
+            double val = 0.0;
+            if (x0 == K) val -= Qh[x1][c];
+            if (x1 == K) val -= Qh[x0][c];
+            J[1 + p][1 + c] = val;
+        }
+        for (int k = 0; k < NR; k++) {
+            int i = slot_row(D, k), j = slot_col(D, k);
+            J[1 + p][1 + D + k] = a.ut[x0][i] * a.ht[x1][j] + a.ut[x1][i] * a.ht[x0][j];
+        }
+    }
+
+    // grad += J^T v
+    for (int j = 0; j < P; j++) {
+        double acc = 0.0;
+        for (int i = 0; i < NG; i++) acc += J[i][j] * v[i];
+        grad[j] += acc;
+    }
+    if (!want_hess) return;
+    // hess += J^T Kg J
+    double KJ[NG][P];
+    for (int i = 0; i < NG; i++)
+        for (int j = 0; j < P; j++) {
+            double acc = 0.0;
+            for (int l = 0; l < NG; l++) acc += Kg[i][l] * J[l][j];
+            KJ[i][j] = acc;
+        }
+    for (int i = 0; i < P; i++)
+        for (int j = i; j < P; j++) {
+            double acc = 0.0;
+            for (int l = 0; l < NG; l++) acc += J[l][i] * KJ[l][j];
+            hess[i * P + j] += acc;
+            if (j != i) hess[j * P + i] += acc;
+        }
+
+    // hess += Hessian of Phi(theta) = v_alpha alpha + sum_ab V_ab B_ab  (v fixed)
+    double V[3][3];
+    for (int p = 0; p < NB; p++) {
+        int x0, x1;
+        bpair<K>(p, x0, x1);
+        double w = (x0 == x1) ? v[1 + p] : 0.5 * v[1 + p];
+        V[x0][x1] = w;
+        V[x1][x0] = w;
+    }
+    double Hh[3][3], Uh[3][3], Uu[3][3];
+    for (int i = 0; i < D; i++)
+        for (int j = 0; j < D; j++) {
+            double hh = 0.0, uh = 0.0, uu = 0.0;
+            for (int b0 = 0; b0 <= K; b0++)
+                for (int b1 = 0; b1 <= K; b1++) {
+                    hh += V[b0][b1] * a.ht[b0][i] * a.ht[b1][j];
+                    uh += V[b0][b1] * a.ut[b0][i] * a.ht[b1][j];
+                    uu += V[b0][b1] * a.ut[b0][i] * a.ut[b1][j];
+                }
+            Hh[i][j] = hh;
+            Uh[i][j] = uh;
+            Uu[i][j] = uu;
+        }
+    const double va = v[0];
+    const double rs5 = rs3 / s;
+    // I - r
+    for (int k = 0; k < NR; k++) {
+        int i = slot_row(D, k), j = slot_col(D, k);
+        double val = -va * rs3 * a.t[i] * g.T[j];
+        hess[0 * P + 1 + D + k] += val;
+        hess[(1 + D + k) * P + 0] += val;
+    }
+    // x - x : 2 V_KK Q_cd, Q_cd = (U e_c) . M (U e_d)
+    double z[3];  // U^T t
+    for (int c = 0; c < D; c++) {
+        double acc = 0.0;
+        for (int i = 0; i <= c; i++) acc += g.U[i][c] * a.t[i];
+        z[c] = acc;
+    }
+    double MUc[3][3];  // column c of M U
+    for (int c = 0; c < D; c++)
+        for (int i = 0; i < D; i++) MUc[c][i] = (i <= c ? g.U[i][c] : 0.0) - z[c] / s * a.t[i];
+    for (int c = 0; c < D; c++)
+        for (int d = 0; d < D; d++) {
+            double Qcd = 0.0;
+            for (int i = 0; i <= c; i++) Qcd += g.U[i][c] * MUc[d][i];
+            hess[(1 + c) * P + 1 + d] += 2.0 * V[K][K] * Qcd;
+        }
+    // x - r
+    double qt[3], uq[3];
+    for (int i = 0; i < D; i++) {
+        double a1 = 0.0, a2 = 0.0;
+        for (int b = 0; b <= K; b++) {
+            a1 += V[b][K] * a.ht[b][i];
+            a2 += V[b][K] * a.ut[b][i];
+        }
+        qt[i] = a1;
+        uq[i] = a2;
+    }
+    for (int c = 0; c < D; c++)
+        for (int k = 0; k < NR; k++) {
+            int i = slot_row(D, k), j = slot_col(D, k);
+            double ec_j = (j == c ? 1.0 : 0.0) - z[c] / s * g.T[j];
+            double val = -2.0 * (MUc[c][i] * qt[j] + uq[i] * ec_j);
+            hess[(1 + c) * P + 1 + D + k] += val;
+            hess[(1 + D + k) * P + 1 + c] += val;
+        }
+    // r - r
+    for (int k = 0; k < NR; k++) {
+        int i1 = slot_row(D, k), j1 = slot_col(D, k);
+        for (int l = k; l < NR; l++) {
+            int i2 = slot_row(D, l), j2 = slot_col(D, l);
+            double dik = (i1 == i2) ? 1.0 : 0.0;
+            double val = 2.0 * (dik * Hh[j2][j1] -
+                                (a.t[i1] * a.t[i2] * Hh[j2][j1] + a.t[i1] * g.T[j2] * Uh[i2][j1] +
+                                 g.T[j1] * a.t[i2] * Uh[i1][j2] + g.T[j1] * g.T[j2] * Uu[i1][i2]) / s);
+            val += va * g.I *
+                   (3.0 * rs5 * a.t[i1] * g.T[j1] * a.t[i2] * g.T[j2] - rs3 * dik * g.T[j1] * g.T[j2]);
+            hess[(1 + D + k) * P + 1 + D + l] += val;
+            if (l != k) hess[(1 + D + l) * P + 1 + D + k] += val;
+        }
+    }
+}
+
+}  // namespace gt

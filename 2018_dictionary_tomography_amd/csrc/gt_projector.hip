@@ -1,0 +1,542 @@
+// gt_projector.hip -- forward projection and per-atom derivatives on CDNA4.
+//
+// Pipeline (all stream-ordered, no host synchronisation):
+//   setup    one thread per (atom, angle): FP64 detector Gaussian -> PairRec
+//   forward  one block per (angle, 16x64 pixel tile): stages the angle's
+//            records that touch the tile in LDS, 4 pixels per lane, Kahan sums
+//   moments  one wave per (atom, angle): FP32 pixel moments over the
+//            footprint box, LDS-transposed wave reduction, no atomics
+//   finish   one wave per atom, one lane per angle: FP64 moment -> (f, g, H)
+//            contraction (gt_pair.h), FP64 wave reduction over angles
+//
+// Reference counterparts: __RadProj_3D/_2p5D (NewtonBases.py:729-756),
+// __RadProj (NewtonBases2D.py:284-296), __L2derivs_RadProj_aniso_3D and
+// __derivs_RadProj_aniso_3D (NewtonBases.py:792-870, 993-1069) and their 2D
+// versions (NewtonBases2D.py:329-401, 444-515).
+#include "gt_common.h"
+
+namespace gt {
+
+constexpr int TILE_ROWS = 16, TILE_COLS = 64, ROWS_PER_LANE = 4;
+constexpr int CHUNK = 256;
+constexpr int MOM_STRIDE = 32;  // floats per (atom, angle) moment record
+constexpr int MODE_LINEAR = GT_MODE_LINEAR, MODE_L2 = GT_MODE_L2, MODE_RESID = GT_MODE_RESID;
+
+__device__ inline void kahan_add(float &s, float &c, float v)
+{
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// ---------------------------------------------------------------------------
+// setup
+// ---------------------------------------------------------------------------
+template <int D> __global__ __launch_bounds__(256) void setup_kernel(gt_atoms A, gt_geom G, PairRec *recs)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)A.n * G.nA;
+    if (idx >= total) return;
+    const int a = (int)(idx / A.n), j = (int)(idx - (long)a * A.n);
+    PairGeom<D> g;
+    load_atom<D>(A, j, g);
+    load_angle<D>(G, a, g);
+    const float *det[2] = {G.p0, G.p1};
+    const int nd[2] = {G.n0, G.n1};
+    PairRec rec;
+    pair_setup<D>(g, det, nd, G.sorted, rec);
+    recs[idx] = rec;
+}
+
+// Deterministic block-wide compaction of the records that touch a box
+// [r_lo, r_hi] x [c_lo, c_hi]; returns the list length.  The list keeps atom
+// order, so the per-pixel summation order does not depend on timing.
+__device__ inline int stage_chunk(const PairRec *__restrict__ ra, int base, int N, int r_lo, int r_hi, int c_lo,
+                                  int c_hi, PairRec *list, int *wcnt)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j = base + (int)threadIdx.x;
+    bool hit = false;
+    int4 v0 = make_int4(0, 0, 0, 0), v1 = v0, v2 = v0, v3 = v0;
+    if (j < N) {
+        const int4 *src = reinterpret_cast<const int4 *>(ra + j);
+        v0 = src[0];
+        v1 = src[1];
+        v2 = src[2];
+        v3 = src[3];
+        // v3 = (lo[0], lo[1], hi[0], hi[1])
+        hit = v3.x <= r_hi && v3.z >= r_lo && v3.y <= c_hi && v3.w >= c_lo;
+    }
+    const unsigned long long m = __ballot(hit);
+    if (lane == 0) wcnt[wave] = __popcll(m);
+    __syncthreads();
+    int off = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        if (w < wave) off += wcnt[w];
+        total += wcnt[w];
+    }
+    if (hit) {
+        int4 *dst = reinterpret_cast<int4 *>(list + off + __popcll(m & ((1ull << lane) - 1ull)));
+        dst[0] = v0;
+        dst[1] = v1;
+        dst[2] = v2;
+        dst[3] = v3;
+    }
+    __syncthreads();
+    return total;
+}
+
+// ---------------------------------------------------------------------------
+// forward, 2D detectors (3D and 2.5D geometries)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void forward3_kernel(const PairRec *__restrict__ recs, int N, int n0, int n1,
+                                                       const float *__restrict__ p0, const float *__restrict__ p1,
+                                                       float *__restrict__ R)
+{
+    __shared__ PairRec list[CHUNK];
+    __shared__ int wcnt[4];
+    const int a = blockIdx.y;
+    const int tilesC = (n1 + TILE_COLS - 1) / TILE_COLS;
+    const int tr = blockIdx.x / tilesC, tc = blockIdx.x - tr * tilesC;
+    const int row0 = tr * TILE_ROWS, col0 = tc * TILE_COLS;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i1 = col0 + lane;
+    const int r0 = row0 + wave * ROWS_PER_LANE;
+    const int thi0 = min(row0 + TILE_ROWS - 1, n0 - 1), thi1 = min(col0 + TILE_COLS - 1, n1 - 1);
+    const float pc = p1[min(i1, n1 - 1)];
+    float pr[ROWS_PER_LANE], s[ROWS_PER_LANE], c[ROWS_PER_LANE];
+#pragma unroll
+    for (int k = 0; k < ROWS_PER_LANE; k++) {
+        pr[k] = p0[min(r0 + k, n0 - 1)];
+        s[k] = 0.0f;
+        c[k] = 0.0f;
+    }
+    const PairRec *ra = recs + (size_t)a * N;
+    const float kap = (float)KAPPA;
+    for (int base = 0; base < N; base += CHUNK) {
+        const int total = stage_chunk(ra, base, N, row0, thi0, col0, thi1, list, wcnt);
+        for (int e = 0; e < total; e++) {
+            const PairRec &q = list[e];
+            if (q.lo[0] > r0 + ROWS_PER_LANE - 1 || q.hi[0] < r0) continue;
+            const float dp1 = (pc - q.c_hi[1]) - q.c_lo[1];
+            const float rowlin = fmaf(q.q[1], dp1, q.q[3]);
+            const float col = fmaf(fmaf(q.q[2], dp1, q.q[4]), dp1, q.q[5]);
+#pragma unroll
+            for (int k = 0; k < ROWS_PER_LANE; k++) {
+                const float dp0 = (pr[k] - q.c_hi[0]) - q.c_lo[0];
+                const float qq = fmaf(fmaf(q.q[0], dp0, rowlin), dp0, col);
+                if (qq >= kap) kahan_add(s[k], c[k], q.alpha * __builtin_amdgcn_exp2f(qq));
+            }
+        }
+        __syncthreads();
+    }
+    if (i1 < n1)
+#pragma unroll
+        for (int k = 0; k < ROWS_PER_LANE; k++)
+            if (r0 + k < n0) R[((size_t)a * n0 + r0 + k) * n1 + i1] = s[k];
+}
+
+// ---------------------------------------------------------------------------
+// forward, 1D detectors (2D geometry)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void forward2_kernel(const PairRec *__restrict__ recs, int N, int n0,
+                                                       const float *__restrict__ p0, float *__restrict__ R)
+{
+    __shared__ PairRec list[CHUNK];
+    __shared__ int wcnt[4];
+    const int a = blockIdx.y;
+    const int row0 = blockIdx.x * 256;
+    const int i0 = row0 + (int)threadIdx.x;
+    const int wave = threadIdx.x >> 6;
+    const int w0 = row0 + wave * 64, w1 = w0 + 63;
+    const int thi = min(row0 + 255, n0 - 1);
+    const float pr = p0[min(i0, n0 - 1)];
+    float s = 0.0f, c = 0.0f;
+    const PairRec *ra = recs + (size_t)a * N;
+    const float kap = (float)KAPPA;
+    for (int base = 0; base < N; base += CHUNK) {
+        const int total = stage_chunk(ra, base, N, row0, thi, 0, 0, list, wcnt);
+        for (int e = 0; e < total; e++) {
+            const PairRec &q = list[e];
+            if (q.lo[0] > w1 || q.hi[0] < w0) continue;
+            const float dp0 = (pr - q.c_hi[0]) - q.c_lo[0];
+            const float qq = fmaf(fmaf(q.q[0], dp0, q.q[3]), dp0, q.q[5]);
+            if (qq >= kap) kahan_add(s, c, q.alpha * __builtin_amdgcn_exp2f(qq));
+        }
+        __syncthreads();
+    }
+    if (i0 < n0) R[(size_t)a * n0 + i0] = s;
+}
+
+// ---------------------------------------------------------------------------
+// moments
+// ---------------------------------------------------------------------------
+// Layout of one moment record (32 floats): [0, NM) sum w1 mono, [15, 15+NM)
+// sum E^2 mono, [30] sum_fp (alpha E - C)^2, [31] sum_fp C^2 (L2 mode).
+template <int K, int MODE>
+__global__ __launch_bounds__(256) void moments_kernel(const PairRec *__restrict__ recs, int N, int nA, int n0,
+                                                      int n1, const float *__restrict__ p0,
+                                                      const float *__restrict__ p1, const float *__restrict__ C,
+                                                      float *__restrict__ mom)
+{
+    __shared__ float red[4][64][MOM_STRIDE + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long pair = (long)blockIdx.x * 4 + wave;
+    const bool valid = pair < (long)N * nA;
+    float acc[MOM_STRIDE];
+#pragma unroll
+    for (int v = 0; v < MOM_STRIDE; v++) acc[v] = 0.0f;
+    if (valid) {
+        const PairRec rec = recs[pair];
+        const int a = (int)(pair / N);
+        const int W = (K == 2) ? rec.hi[1] - rec.lo[1] + 1 : 1;
+        const int H = rec.hi[0] - rec.lo[0] + 1;
+        const long total = (W > 0 && H > 0) ? (long)W * H : 0;
+        int r = (W > 0) ? lane / W : 0, cidx = (W > 0) ? lane - r * W : 0;
+        const int dr = (W > 0) ? 64 / W : 0, dc = (W > 0) ? 64 - dr * W : 0;
+        const float *Ca = C + (size_t)a * n0 * n1;
+        const float kap = (float)KAPPA;
+        for (long k = lane; k < total; k += 64) {
+            const int i0 = rec.lo[0] + r;
+            const int i1 = (K == 2) ? rec.lo[1] + cidx : 0;
+            const float dp0 = (p0[i0] - rec.c_hi[0]) - rec.c_lo[0];
+            const float dp1 = (K == 2) ? (p1[i1] - rec.c_hi[1]) - rec.c_lo[1] : 0.0f;
+            const float q = pair_q(rec.q, dp0, dp1);
+            if (q >= kap) {
+                const float E = __builtin_amdgcn_exp2f(q);
+                const float cv = Ca[(size_t)i0 * n1 + i1];
+                float w1;
+                if (MODE == MODE_L2) {
+                    const float res = fmaf(rec.alpha, E, -cv);
+                    w1 = res * E;
+                    acc[30] = fmaf(res, res, acc[30]);
+                    acc[31] = fmaf(cv, cv, acc[31]);
+                } else {
+                    w1 = cv * E;
+                }
+                const float w2 = E * E;
+                if (K == 2) {
+                    const float x1 = dp0, y1 = dp1;
+                    const float x2 = x1 * x1, xy = x1 * y1, y2 = y1 * y1;
+                    const float mono[15] = {1.0f,    x1,      y1,      x2,      xy,
+                                            y2,      x2 * x1, x2 * y1, x1 * y2, y2 * y1,
+                                            x2 * x2, x2 * xy, x2 * y2, xy * y2, y2 * y2};
+#pragma unroll
+                    for (int m = 0; m < 15; m++) acc[m] = fmaf(w1, mono[m], acc[m]);
+                    if (MODE != MODE_LINEAR) {
+#pragma unroll
+                        for (int m = 0; m < 15; m++) acc[15 + m] = fmaf(w2, mono[m], acc[15 + m]);
+                    }
+                } else {
+                    const float x1 = dp0, x2 = x1 * x1;
+                    const float mono[5] = {1.0f, x1, x2, x2 * x1, x2 * x2};
+#pragma unroll
+                    for (int m = 0; m < 5; m++) acc[m] = fmaf(w1, mono[m], acc[m]);
+                    if (MODE != MODE_LINEAR) {
+#pragma unroll
+                        for (int m = 0; m < 5; m++) acc[15 + m] = fmaf(w2, mono[m], acc[15 + m]);
+                    }
+                }
+            }
+            cidx += dc;
+            r += dr;
+            if (cidx >= W) {
+                cidx -= W;
+                r++;
+            }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < MOM_STRIDE; v++) red[wave][lane][v] = acc[v];
+    __syncthreads();
+    if (valid) {
+        const int col = lane & 31, half = lane >> 5;
+        float s = 0.0f;
+        for (int i = 0; i < 32; i++) s += red[wave][half * 32 + i][col];
+        s += __shfl_xor(s, 32);
+        if (lane < 32) mom[pair * MOM_STRIDE + lane] = s;
+    }
+}
+
+// FP64 sum of C^2 per angle (the L2 value of the pixels outside every footprint).
+__global__ __launch_bounds__(256) void sumsq_kernel(const float *__restrict__ C, long per, double *__restrict__ out)
+{
+    __shared__ double red[256];
+    const int a = blockIdx.x;
+    double s = 0.0;
+    for (long i = threadIdx.x; i < per; i += 256) {
+        const double v = C[(size_t)a * per + i];
+        s += v * v;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[a] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// finish
+// ---------------------------------------------------------------------------
+template <int D, int MODE>
+__global__ __launch_bounds__(64) void finish_kernel(gt_atoms A, gt_geom G, const PairRec *__restrict__ recs,
+                                                    const float *__restrict__ mom, const double *__restrict__ sumC2,
+                                                    int want_hess, double *__restrict__ fo, double *__restrict__ go,
+                                                    double *__restrict__ Ho)
+{
+    using Dm = Dims<D>;
+    constexpr int P = Dm::P, NM = Dm::NM, NH = P * (P + 1) / 2, NV = 1 + P + NH;
+    __shared__ double red[64][NV + 1];
+    const int j = blockIdx.x, lane = threadIdx.x;
+    const int N = A.n;
+    double grad[P], hess[P * P], F = 0.0;
+    for (int i = 0; i < P; i++) grad[i] = 0.0;
+    for (int i = 0; i < P * P; i++) hess[i] = 0.0;
+    PairGeom<D> g;
+    load_atom<D>(A, j, g);
+    for (int a = lane; a < G.nA; a += 64) {
+        const size_t pi = (size_t)a * N + j;
+        const PairRec rec = recs[pi];
+        const float *m = mom + pi * MOM_STRIDE;
+        if (MODE == MODE_L2) F += 0.5 * (sumC2[a] - (double)m[31] + (double)m[30]);
+        const bool empty = rec.lo[0] > rec.hi[0] || (D == 3 && rec.lo[1] > rec.hi[1]);
+        if (empty) continue;
+        load_angle<D>(G, a, g);
+        double S1[NM], S2[NM];
+        for (int k = 0; k < NM; k++) {
+            S1[k] = m[k];
+            S2[k] = m[15 + k];
+        }
+        if (MODE == MODE_LINEAR) {
+            double t[3], s = 0.0;
+            matvec_upper<D>(g.U, g.T, t);
+            for (int i = 0; i < D; i++) s += t[i] * t[i];
+            F += g.I / sqrt(s) * S1[0];
+        }
+        const double pstar[2] = {(double)rec.c_hi[0] + (double)rec.c_lo[0],
+                                 (double)rec.c_hi[1] + (double)rec.c_lo[1]};
+        pair_finish<D>(g, pstar, S1, S2, MODE != MODE_LINEAR, want_hess, grad, hess);
+    }
+    red[lane][0] = F;
+    for (int i = 0; i < P; i++) red[lane][1 + i] = grad[i];
+    {
+        int k = 1 + P;
+        for (int i = 0; i < P; i++)
+            for (int l = i; l < P; l++) red[lane][k++] = hess[i * P + l];
+    }
+    __syncthreads();
+    for (int col = lane; col < NV; col += 64) {
+        double s = 0.0;
+        for (int i = 0; i < 64; i++) s += red[i][col];
+        red[0][col] = s;  // rows 1..63 of this column are no longer read
+    }
+    __syncthreads();
+    if (lane != 0) return;
+    const double *v = red[0];
+    if (fo) fo[j] = v[0];
+    if (!A.iso) {
+        for (int i = 0; i < P; i++) go[(size_t)j * P + i] = v[1 + i];
+        if (Ho && want_hess) {
+            int k = 1 + P;
+            for (int i = 0; i < P; i++)
+                for (int l = i; l < P; l++) {
+                    Ho[((size_t)j * P + i) * P + l] = v[k];
+                    Ho[((size_t)j * P + l) * P + i] = v[k];
+                    k++;
+                }
+        }
+    } else {
+        // U = rho * Id: d r_k / d rho = 1 for the D diagonal slots.
+        constexpr int PI = 2 + D;
+        double Hf[P][P];
+        {
+            int k = 1 + P;
+            for (int i = 0; i < P; i++)
+                for (int l = i; l < P; l++) {
+                    Hf[i][l] = v[k];
+                    Hf[l][i] = v[k];
+                    k++;
+                }
+        }
+        double gi[PI], Hi[PI][PI];
+        double E[P][PI];
+        for (int i = 0; i < P; i++)
+            for (int l = 0; l < PI; l++) E[i][l] = 0.0;
+        for (int i = 0; i <= D; i++) E[i][i] = 1.0;
+        for (int k = 0; k < D; k++) E[1 + D + k][1 + D] = 1.0;
+        for (int l = 0; l < PI; l++) {
+            double s = 0.0;
+            for (int i = 0; i < P; i++) s += E[i][l] * v[1 + i];
+            gi[l] = s;
+        }
+        for (int l = 0; l < PI; l++) go[(size_t)j * PI + l] = gi[l];
+        if (Ho && want_hess) {
+            for (int l = 0; l < PI; l++)
+                for (int m2 = 0; m2 < PI; m2++) {
+                    double s = 0.0;
+                    for (int i = 0; i < P; i++)
+                        for (int k = 0; k < P; k++) s += E[i][l] * Hf[i][k] * E[k][m2];
+                    Hi[l][m2] = s;
+                }
+            for (int l = 0; l < PI; l++)
+                for (int m2 = 0; m2 < PI; m2++) Ho[((size_t)j * PI + l) * PI + m2] = Hi[l][m2];
+        }
+    }
+}
+
+__global__ void fill_kernel(float *__restrict__ p, size_t n, float v)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static inline size_t rec_bytes(const gt_atoms *A, const gt_geom *G)
+{
+    return align_up((size_t)A->n * G->nA * sizeof(PairRec));
+}
+static inline size_t mom_bytes(const gt_atoms *A, const gt_geom *G)
+{
+    return align_up((size_t)A->n * G->nA * MOM_STRIDE * sizeof(float));
+}
+
+int launch_setup(const gt_atoms *A, const gt_geom *G, PairRec *recs, hipStream_t st)
+{
+    const long total = (long)A->n * G->nA;
+    if (total == 0) return 0;
+    const unsigned blocks = (unsigned)((total + 255) / 256);
+    if (G->dim == 3)
+        hipLaunchKernelGGL(setup_kernel<3>, dim3(blocks), dim3(256), 0, st, *A, *G, recs);
+    else
+        hipLaunchKernelGGL(setup_kernel<2>, dim3(blocks), dim3(256), 0, st, *A, *G, recs);
+    GT_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_fill(float *p, size_t n, float v, hipStream_t st)
+{
+    if (n == 0) return 0;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65535);
+    hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, st, p, n, v);
+    GT_HIP(hipGetLastError());
+    return 0;
+}
+
+// Forward projection from precomputed records (used by the fitters too).
+int launch_forward(const gt_atoms *A, const gt_geom *G, const PairRec *recs, float *R, hipStream_t st)
+{
+    const size_t nout = (size_t)G->nA * G->n0 * G->n1;
+    if (A->n == 0) return launch_fill(R, nout, 0.0f, st);
+    if (G->dim == 3) {
+        const int tiles = ((G->n0 + TILE_ROWS - 1) / TILE_ROWS) * ((G->n1 + TILE_COLS - 1) / TILE_COLS);
+        hipLaunchKernelGGL(forward3_kernel, dim3(tiles, G->nA), dim3(256), 0, st, recs, A->n, G->n0, G->n1, G->p0,
+                           G->p1, R);
+    } else {
+        const int tiles = (G->n0 + 255) / 256;
+        hipLaunchKernelGGL(forward2_kernel, dim3(tiles, G->nA), dim3(256), 0, st, recs, A->n, G->n0, G->p0, R);
+    }
+    GT_HIP(hipGetLastError());
+    return 0;
+}
+
+template <int D, int MODE>
+static void launch_deriv_kernels(const gt_atoms *A, const gt_geom *G, const PairRec *recs, float *mom,
+                                 const double *sumC2, const float *C, int want_hess, double *f, double *g, double *H,
+                                 hipStream_t st)
+{
+    constexpr int K = D - 1;
+    const long pairs = (long)A->n * G->nA;
+    hipLaunchKernelGGL((moments_kernel<K, MODE>), dim3((unsigned)((pairs + 3) / 4)), dim3(256), 0, st, recs, A->n,
+                       G->nA, G->n0, G->n1, G->p0, G->p1, C, mom);
+    hipLaunchKernelGGL((finish_kernel<D, MODE>), dim3(A->n), dim3(64), 0, st, *A, *G, recs, mom, sumC2, want_hess,
+                       f, g, H);
+}
+
+int launch_derivs(const gt_atoms *A, const gt_geom *G, const PairRec *recs, float *mom, double *sumC2,
+                  const float *C, int mode, int want_hess, double *f, double *g, double *H, hipStream_t st)
+{
+    if (A->n == 0) return 0;
+    if (mode == MODE_L2 && f) {
+        hipLaunchKernelGGL(sumsq_kernel, dim3(G->nA), dim3(256), 0, st, C, (long)G->n0 * G->n1, sumC2);
+    }
+    const int D = G->dim;
+#define GT_DISPATCH(DD, MM)                                                                             \
+    if (D == DD && mode == MM)                                                                          \
+        launch_deriv_kernels<DD, MM>(A, G, recs, mom, sumC2, C, want_hess, (mode == MODE_RESID) ? nullptr : f, g, \
+                                     H, st);
+    GT_DISPATCH(3, MODE_LINEAR)
+    GT_DISPATCH(3, MODE_L2)
+    GT_DISPATCH(3, MODE_RESID)
+    GT_DISPATCH(2, MODE_LINEAR)
+    GT_DISPATCH(2, MODE_L2)
+    GT_DISPATCH(2, MODE_RESID)
+#undef GT_DISPATCH
+    GT_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace gt
+
+using namespace gt;
+
+extern "C" size_t gt_forward_workspace(const gt_atoms *atoms, const gt_geom *geom)
+{
+    if (!atoms || !geom) return 0;
+    return rec_bytes(atoms, geom);
+}
+
+extern "C" int gt_forward(const gt_atoms *atoms, const gt_geom *geom, float *R, void *ws, size_t ws_bytes,
+                          void *stream)
+{
+    int rc;
+    if ((rc = check_atoms(atoms)) || (rc = check_geom(geom, atoms->dim))) return rc;
+    GT_CHECK_ARG(R != nullptr, "gt_forward: R is NULL");
+    const size_t need = gt_forward_workspace(atoms, geom);
+    if (need > ws_bytes || (need > 0 && ws == nullptr)) {
+        set_error("gt_forward: workspace of %zu bytes given, %zu needed", ws_bytes, need);
+        return GT_EWORKSPACE;
+    }
+    GT_CHECK_ARG(((uintptr_t)ws & 63) == 0, "gt_forward: workspace must be 64-byte aligned");
+    hipStream_t st = (hipStream_t)stream;
+    PairRec *recs = (PairRec *)ws;
+    if ((rc = launch_setup(atoms, geom, recs, st))) return rc;
+    return launch_forward(atoms, geom, recs, R, st);
+}
+
+extern "C" size_t gt_derivs_workspace(const gt_atoms *atoms, const gt_geom *geom, int mode)
+{
+    (void)mode;
+    if (!atoms || !geom) return 0;
+    return rec_bytes(atoms, geom) + mom_bytes(atoms, geom) + align_up((size_t)geom->nA * sizeof(double));
+}
+
+extern "C" int gt_derivs(const gt_atoms *atoms, const gt_geom *geom, const float *C, int mode, int order, double *f,
+                         double *g, double *H, void *ws, size_t ws_bytes, void *stream)
+{
+    int rc;
+    if ((rc = check_atoms(atoms)) || (rc = check_geom(geom, atoms->dim))) return rc;
+    GT_CHECK_ARG(mode == GT_MODE_LINEAR || mode == GT_MODE_L2 || mode == GT_MODE_RESID,
+                 "gt_derivs: mode must be 0 (linear), 1 (L2) or 2 (residual), got %d", mode);
+    GT_CHECK_ARG(order == 1 || order == 2, "gt_derivs: order must be 1 or 2, got %d", order);
+    GT_CHECK_ARG(C != nullptr || atoms->n == 0, "gt_derivs: C is NULL");
+    GT_CHECK_ARG(g != nullptr || atoms->n == 0, "gt_derivs: g is NULL");
+    const size_t need = gt_derivs_workspace(atoms, geom, mode);
+    if (need > ws_bytes || (need > 0 && ws == nullptr)) {
+        set_error("gt_derivs: workspace of %zu bytes given, %zu needed", ws_bytes, need);
+        return GT_EWORKSPACE;
+    }
+    GT_CHECK_ARG(((uintptr_t)ws & 63) == 0, "gt_derivs: workspace must be 64-byte aligned");
+    hipStream_t st = (hipStream_t)stream;
+    char *w = (char *)ws;
+    PairRec *recs = (PairRec *)w;
+    float *mom = (float *)(w + rec_bytes(atoms, geom));
+    double *sumC2 = (double *)(w + rec_bytes(atoms, geom) + mom_bytes(atoms, geom));
+    if ((rc = launch_setup(atoms, geom, recs, st))) return rc;
+    return launch_derivs(atoms, geom, recs, mom, sumC2, C, mode, order > 1, f, g, H, st);
+}

@@ -1,0 +1,118 @@
+/*
+ * gausstomo.h -- C ABI of the MI355X Gaussian-atom tomography library.
+ *
+ * Every pointer in the structs and arguments below is a DEVICE pointer unless
+ * stated otherwise; the library never allocates on these calls.  Work is
+ * enqueued on `stream` (a hipStream_t passed as void*, NULL = default stream)
+ * and the calls return as soon as it is enqueued.  Return value: 0 on
+ * success, GT_EINVAL (-1) for a bad argument, GT_EHIP (-2) for a HIP error,
+ * GT_EWORKSPACE (-3) for a workspace smaller than the matching *_workspace()
+ * size; gt_last_error() gives the message of the calling thread's last error.
+ *
+ * The entry points replace the module-level "myRad" functions that
+ * GaussTomo._getOps binds (GaussDictCode/atomFuncs.py:106-113):
+ *
+ *   gt_forward   <- RadProj          GaussDictCode/bin/NewtonBases.py:714-726
+ *                                     GaussDictCode/bin/NewtonBases2D.py:274-281
+ *   gt_derivs    <- L2derivs_RadProj GaussDictCode/bin/NewtonBases.py:759-789
+ *                                     GaussDictCode/bin/NewtonBases2D.py:299-326
+ *                <- derivs_RadProj   GaussDictCode/bin/NewtonBases.py:954-990
+ *                                     GaussDictCode/bin/NewtonBases2D.py:415-441
+ *                   (batched over atoms: GaussTomo.derivs / L2_derivs loop,
+ *                    GaussDictCode/atomFuncs.py:130-158)
+ *   gt_volume    <- VolProj          GaussDictCode/bin/NewtonBases.py:691-695
+ *                                     GaussDictCode/bin/NewtonBases2D.py:252-256
+ *   gt_ft_eval   <- evaluate         GaussDictCode/bin/FTGaussian3D.py:12-52
+ */
+#ifndef GAUSSTOMO_H
+#define GAUSSTOMO_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GT_EINVAL (-1)
+#define GT_EHIP (-2)
+#define GT_EWORKSPACE (-3)
+
+/* Derivative modes of gt_derivs. */
+#define GT_MODE_LINEAR 0 /* f = sum R C, Df = sum C dR, DDf = sum C ddR       (derivs_RadProj)   */
+#define GT_MODE_L2 1     /* f = 1/2 sum (R-C)^2, Df, DDf of that              (L2derivs_RadProj) */
+#define GT_MODE_RESID 2  /* C is the full residual R_all - data: Df, DDf of    */
+                         /* 1/2 ||R_all - data||^2 w.r.t. each atom (= L2 with */
+                         /* C_j = data - R_all + R_j, NewtonGaussian.py:59-60) */
+
+/* Atoms in the reference layout (GaussDictCode/dictionary_def.py:147-214). */
+typedef struct gt_atoms {
+    int dim;        /* 2 or 3 */
+    int n;          /* number of atoms (>= 0) */
+    int iso;        /* 1: r holds one radius per atom, 0: dim*(dim+1)/2 slots */
+    const float *I; /* [n] */
+    const float *x; /* [n][dim] */
+    const float *r; /* [n][iso ? 1 : dim*(dim+1)/2], slots [r00,r11,(r22,)r01,(r12,r02)] */
+} gt_atoms;
+
+/* Projection geometry (ProjSpace, GaussDictCode/dictionary_def.py:263-283). */
+typedef struct gt_geom {
+    int dim;         /* 2 or 3 */
+    int lenT;        /* 3: 3D (w0, w1 given); 2: 2D, or 3D single-axis ("2.5D") */
+    int nA;          /* number of orientations */
+    int n0, n1;      /* detector sizes; n1 = 1 when dim == 2 */
+    int sorted;      /* 1 if p0 (and p1) are ascending: enables footprint culling */
+    const float *T;  /* [nA][lenT] ray directions (orientations) */
+    const float *w0; /* [nA][lenT] first detector axis (ortho[0]) */
+    const float *w1; /* [nA][3] second detector axis (ortho[1]); 3D only, else NULL */
+    const float *p0; /* [n0] detector coordinates (detector[0]) */
+    const float *p1; /* [n1] detector coordinates (detector[1]); dim 3 only */
+} gt_geom;
+
+/* Volume grid for gt_volume (VolSpace coordinate vectors). */
+typedef struct gt_vol {
+    int dim;
+    int m0, m1, m2;  /* m2 = 1 when dim == 2 */
+    int sorted;      /* 1 if the coordinate vectors are ascending */
+    const float *y0, *y1, *y2;
+} gt_vol;
+
+/* Separable Fourier grid for gt_ft_eval: k = (k0[i0], k1[i1], k2[i2]). */
+typedef struct gt_kgrid {
+    int dim; /* 3 */
+    int m0, m1, m2;
+    int sorted;
+    const float *k0, *k1, *k2;
+} gt_kgrid;
+
+const char *gt_last_error(void);
+const char *gt_version(void);
+
+/* RadProj: R[a][i0][i1] = sum over atoms of the line integral (written, not accumulated). */
+size_t gt_forward_workspace(const gt_atoms *atoms, const gt_geom *geom);
+int gt_forward(const gt_atoms *atoms, const gt_geom *geom, float *R, void *ws, size_t ws_bytes,
+               void *stream);
+
+/* Batched per-atom derivatives over all rays.  C: [nA][n0][n1] FP32.
+ * Outputs (FP64, any may be NULL except g): f[n], g[n][P], H[n][P][P] with
+ * P = 1 + dim + (iso ? 1 : dim*(dim+1)/2), parameter order [I, x, r].
+ * order 1 skips H. */
+size_t gt_derivs_workspace(const gt_atoms *atoms, const gt_geom *geom, int mode);
+int gt_derivs(const gt_atoms *atoms, const gt_geom *geom, const float *C, int mode, int order, double *f,
+              double *g, double *H, void *ws, size_t ws_bytes, void *stream);
+
+/* VolProj: u[j0][j1][j2] = sum over atoms of I sqrt(30/pi) exp(-30|U(y-x)|^2), cut at 1. */
+size_t gt_volume_workspace(const gt_atoms *atoms, const gt_vol *vol);
+int gt_volume(const gt_atoms *atoms, const gt_vol *vol, float *u, void *ws, size_t ws_bytes, void *stream);
+
+/* Fourier evaluation on a separable 3D grid (FTGaussian3D.__grid_eval_*,
+ * GaussDictCode/bin/FTGaussian3D.py:95-110,151-164).  atoms hold FT-space
+ * parameters (Fourier_Transform.py:293-333).  out: [m0][m1][m2] complex64
+ * as interleaved (re, im) floats. */
+size_t gt_ft_workspace(const gt_atoms *atoms, const gt_kgrid *grid);
+int gt_ft_eval(const gt_atoms *atoms, const gt_kgrid *grid, float *out, void *ws, size_t ws_bytes,
+               void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

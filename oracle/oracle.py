@@ -1,0 +1,443 @@
+"""FP64 oracle for the Gaussian-atom tomography hot path.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product package imports this
+module; only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline``
+leg of ``bench.py`` use it, and only as the checker.
+
+The heavy loops live in ``gt_oracle.c`` (built into ``libgt_oracle.so`` by
+``build()`` below); this module restates the small host-side pieces of the
+reference in NumPy:
+
+* ``theta_to_vec`` / ``perp``            GaussDictCode/atomFuncs.py:13-74
+* isotropic -> anisotropic slots and the chain-rule collapse of derivatives
+  (the intended mathematics of GaussDictCode/bin/NewtonBases.py:530-672,
+  873-951, 1072-1152 and GaussDictCode/bin/NewtonBases2D.py:316-326,
+  404-441, whose iso paths are broken in the reference)
+* ``analytic_gauss_fourier`` / ``analytic_gauss_ifourier``
+                                         Fourier_Transform.py:293-376
+* ``l2_squared_loss``                    GaussDictCode/transport_loss.py:567-585
+* ``newton_linesearch``                  NewtonGaussian.py:18-109 with
+                                         GD_lib.py:492-536 (``_step``)
+* ``gd_linesearch_block``                GD_lib.py:82-178
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libgt_oracle.so")
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile gt_oracle.c with gcc (OpenMP) into libgt_oracle.so."""
+    src = os.path.join(_HERE, "gt_oracle.c")
+    if force or not os.path.exists(_LIB_PATH) or os.path.getmtime(_LIB_PATH) < os.path.getmtime(src):
+        cmd = ["gcc", "-O2", "-std=c11", "-fopenmp", "-fPIC", "-shared", "-o", _LIB_PATH + ".tmp", src, "-lm"]
+        subprocess.check_call(cmd)
+        os.replace(_LIB_PATH + ".tmp", _LIB_PATH)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ci = ctypes.c_int
+        _lib.or_radproj.argtypes = [ci, ci, dp, dp, dp, ci, ci, dp, dp, dp, ci, dp, ci, dp, dp]
+        _lib.or_derivs.argtypes = [ci, ci, ci, ci, dp, dp, dp, ci, ci, dp, dp, dp, ci, dp, ci, dp, dp, dp, dp, dp]
+        _lib.or_volproj.argtypes = [ci, ci, dp, dp, dp, ci, dp, ci, dp, ci, dp, dp]
+        _lib.or_ft_eval.argtypes = [ci, ci, ci, dp, dp, dp, ctypes.c_long, dp, dp]
+        _lib.or_num_threads.restype = ci
+    return _lib
+
+
+def num_threads() -> int:
+    return int(lib().or_num_threads())
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _c64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ---------------------------------------------------------------------------
+# Geometry: GaussDictCode/atomFuncs.py:13-74
+# ---------------------------------------------------------------------------
+def theta_to_vec(theta):
+    """Ray directions for a tuple of angle vectors (atomFuncs.py:13-47)."""
+    theta = [np.asarray(t, dtype=np.float64) for t in theta]
+    if len(theta) == 1:
+        t = theta[0]
+        return np.stack([-np.sin(t), np.cos(t)], axis=-1)
+    if len(theta) == 2:
+        t0, t1 = np.meshgrid(theta[0], theta[1], indexing="ij")
+        return np.stack([-np.sin(t0) * np.cos(t1), np.cos(t0) * np.cos(t1), np.sin(t1)], axis=-1)
+    raise ValueError("only 2D and 3D geometries are restated")
+
+
+def perp(theta):
+    """Detector bases for a tuple of angle vectors (atomFuncs.py:50-74)."""
+    theta = [np.asarray(t, dtype=np.float64) for t in theta]
+    if len(theta) == 1:
+        t = theta[0]
+        return (np.stack([np.cos(t), np.sin(t)], axis=-1),)
+    if len(theta) == 2:
+        t0, t1 = np.meshgrid(theta[0], theta[1], indexing="ij")
+        w0 = np.stack([np.cos(t0), np.sin(t0), np.zeros_like(t0)], axis=-1)
+        w1 = np.stack([np.sin(t0) * np.sin(t1), -np.cos(t0) * np.sin(t1), np.cos(t1)], axis=-1)
+        return w0, w1
+    raise ValueError("only 2D and 3D geometries are restated")
+
+
+# ---------------------------------------------------------------------------
+# Atom slot conventions
+# ---------------------------------------------------------------------------
+def n_slots(D):
+    return D * (D + 1) // 2
+
+
+def to_aniso(r, D):
+    """Isotropic radius -> anisotropic slots U = rho * Id (NewtonBases.py:282-291,
+    NewtonBases2D.py:404-412 intent)."""
+    r = np.asarray(r, dtype=np.float64).reshape(len(r), -1)
+    if r.shape[1] == n_slots(D):
+        return r
+    out = np.zeros((r.shape[0], n_slots(D)))
+    out[:, :D] = r[:, :1]
+    return out
+
+
+def iso_collapse(D):
+    """d theta_aniso / d theta_iso for theta = [I, x, r]."""
+    Pa, Pi = 1 + D + n_slots(D), 1 + D + 1
+    E = np.zeros((Pa, Pi))
+    for i in range(1 + D):
+        E[i, i] = 1.0
+    E[1 + D:1 + D + D, 1 + D] = 1.0
+    return E
+
+
+class Geometry:
+    """Explicit projection geometry mirroring ProjSpace(orientations=, ortho=, detector=)
+    (GaussDictCode/dictionary_def.py:263-283)."""
+
+    def __init__(self, orientations, ortho, detector):
+        self.orientations = np.asarray(orientations, dtype=np.float64)
+        self.ortho = tuple(np.asarray(o, dtype=np.float64) for o in ortho)
+        self.detector = [np.asarray(p, dtype=np.float64).reshape(-1) for p in detector]
+        self.shape = [self.orientations.shape[0]] + [p.size for p in self.detector]
+
+    @property
+    def dim(self):
+        return 2 if len(self.detector) == 1 else 3
+
+
+def _geom_args(g: Geometry):
+    D = g.dim
+    T = _c64(g.orientations)
+    lenT = T.shape[1]
+    w0 = _c64(g.ortho[0])
+    w1 = _c64(g.ortho[1]) if (D == 3 and lenT == 3) else None
+    p0 = _c64(g.detector[0])
+    p1 = _c64(g.detector[1]) if D == 3 else np.zeros(1)
+    return D, T, lenT, w0, w1, p0, p1
+
+
+def radproj(I, x, r, geom: Geometry):
+    """Forward projection in FP64 (NewtonBases.py:714-756, NewtonBases2D.py:274-296)."""
+    D, T, lenT, w0, w1, p0, p1 = _geom_args(geom)
+    I, x = _c64(I).reshape(-1), _c64(x).reshape(-1, D)
+    ra = _c64(to_aniso(r, D))
+    N = I.size
+    R = np.zeros(geom.shape if D == 3 else geom.shape + [1])
+    lib().or_radproj(D, N, _p(I), _p(x), _p(ra), T.shape[0], lenT, _p(T), _p(w0),
+                     _p(w1) if w1 is not None else None, p0.size, _p(p0), p1.size, _p(p1), _p(R))
+    return R.reshape(geom.shape)
+
+
+def derivs(I, x, r, geom: Geometry, C, mode="L2", order=2):
+    """Per-atom (f, Df, DDf) over all rays, each atom independent.
+
+    mode "L2":     L2derivs_RadProj (NewtonBases.py:759-870, NewtonBases2D.py:299-401)
+    mode "linear": derivs_RadProj   (NewtonBases.py:954-1069, NewtonBases2D.py:415-515)
+    Isotropic atoms are differentiated as U = rho * Id and collapsed by the chain rule.
+    """
+    D, T, lenT, w0, w1, p0, p1 = _geom_args(geom)
+    I, x = _c64(I).reshape(-1), _c64(x).reshape(-1, D)
+    r = np.asarray(r, dtype=np.float64).reshape(I.size, -1)
+    iso = r.shape[1] == 1
+    ra = _c64(to_aniso(r, D))
+    N = I.size
+    P = 1 + D + n_slots(D)
+    C = _c64(C)
+    f = np.zeros(N)
+    Df = np.zeros((N, P))
+    DDf = np.zeros((N, P, P))
+    lib().or_derivs(D, 1 if mode == "L2" else 0, order, N, _p(I), _p(x), _p(ra), T.shape[0], lenT,
+                    _p(T), _p(w0), _p(w1) if w1 is not None else None, p0.size, _p(p0), p1.size,
+                    _p(p1), _p(C), _p(f), _p(Df), _p(DDf))
+    if iso:
+        E = iso_collapse(D)
+        Df = Df @ E
+        DDf = np.einsum("ai,nab,bj->nij", E, DDf, E)
+    return f, Df, DDf
+
+
+def volproj(I, x, r, coords):
+    """Volume render (NewtonBases.py:691-711, NewtonBases2D.py:252-271)."""
+    D = len(coords)
+    I, x = _c64(I).reshape(-1), _c64(x).reshape(-1, D)
+    ra = _c64(to_aniso(r, D))
+    y = [_c64(c).reshape(-1) for c in coords]
+    if D == 2:
+        y.append(np.zeros(1))
+    u = np.zeros(tuple(len(c) for c in y))
+    lib().or_volproj(D, I.size, _p(I), _p(x), _p(ra), y[0].size, _p(y[0]), y[1].size, _p(y[1]),
+                     y[2].size, _p(y[2]), _p(u))
+    return u.reshape(tuple(len(c) for c in coords))
+
+
+def ft_points(I, x, r, K, iso=False):
+    """Fourier evaluation at explicit points K (npts x D) (FTGaussian3D.py:95-196,
+    FTGaussian2D.py:42-100).  I, x, r are FT-space parameters."""
+    K = _c64(K)
+    D = K.shape[1]
+    I, x = _c64(I).reshape(-1), _c64(x).reshape(-1, D)
+    r = _c64(r).reshape(I.size, -1)
+    out = np.zeros((K.shape[0], 2))
+    lib().or_ft_eval(D, 1 if iso else 0, I.size, _p(I), _p(x), _p(r), K.shape[0], _p(K), _p(out))
+    return out[:, 0] + 1j * out[:, 1]
+
+
+def ft_grid(I, x, r, k, iso=False):
+    """Evaluation on a separable grid k = (k0, k1[, k2]) (FTGaussian3D.py:40-50)."""
+    mesh = np.meshgrid(*[np.asarray(v, dtype=np.float64) for v in k], indexing="ij")
+    K = np.stack([m.reshape(-1) for m in mesh], axis=1)
+    return ft_points(I, x, r, K, iso).reshape(mesh[0].shape)
+
+
+def ft_hyperplane(I, x, r, k0, k1, w0, w1=None, iso=False):
+    """Fourier-slice samples k = k0[i1] w0[i0] + k1[i2] w1[i0] (FTGaussian3D.py:113-148);
+    w1=None selects the 2.5D form k = (k0 w0[i0,0], k0 w0[i0,1], k1) (:132-148)."""
+    k0, k1 = np.asarray(k0, float), np.asarray(k1, float)
+    w0 = np.asarray(w0, float)
+    nA = w0.shape[0]
+    a, i, j = np.meshgrid(np.arange(nA), np.arange(k0.size), np.arange(k1.size), indexing="ij")
+    if w1 is None:
+        K = np.stack([k0[i] * w0[a, 0], k0[i] * w0[a, 1], k1[j]], axis=-1)
+    else:
+        w1 = np.asarray(w1, float)
+        K = k0[i][..., None] * w0[a] + k1[j][..., None] * w1[a]
+    return ft_points(I, x, r, K.reshape(-1, 3), iso).reshape(nA, k0.size, k1.size)
+
+
+# ---------------------------------------------------------------------------
+# Fourier parameter maps: Fourier_Transform.py:293-376
+# ---------------------------------------------------------------------------
+def analytic_gauss_fourier(I, x, r):
+    """(I, x, r) of f(y) = I exp(-|U(y-x)|^2/2) -> FT parameters (Fourier_Transform.py:293-333)."""
+    I, x, r = np.asarray(I, float), np.asarray(x, float), np.asarray(r, float)
+    N, D = x.shape
+    tau = 2 * np.pi
+    nI, nx, nr = np.empty(N), tau * x, np.empty_like(r)
+    if r.shape[1] == 1:
+        nI[:] = I * np.sqrt(tau) ** D * r[:, 0] ** D
+        nr[:, 0] = tau / r[:, 0]
+        return nI, nx, nr
+    det = np.prod(r[:, :D], axis=1)
+    nI[:] = I * np.sqrt(tau) ** D / det
+    if D == 2:
+        nr[:, 0] = tau / r[:, 0]
+        nr[:, 1] = tau / r[:, 1]
+        nr[:, 2] = -(tau * r[:, 2]) / (r[:, 0] * r[:, 1])
+    else:
+        nr[:, 0] = tau / r[:, 0]
+        nr[:, 1] = tau / r[:, 1]
+        nr[:, 2] = tau / r[:, 2]
+        nr[:, 3] = -(tau * r[:, 3]) / (r[:, 0] * r[:, 1])
+        nr[:, 4] = -(tau * r[:, 4]) / (r[:, 1] * r[:, 2])
+        nr[:, 5] = tau * (r[:, 3] * r[:, 4] / r[:, 1] - r[:, 5]) / (r[:, 0] * r[:, 2])
+    return nI, nx, nr
+
+
+def analytic_gauss_ifourier(I, x, r):
+    """Inverse parameter map (Fourier_Transform.py:336-376)."""
+    I, x, r = np.asarray(I, float), np.asarray(x, float), np.asarray(r, float)
+    N, D = x.shape
+    tau = 2 * np.pi
+    nx = x / tau
+    nr = np.empty_like(r)
+    if r.shape[1] == 1:
+        nr[:, 0] = tau / r[:, 0]
+        nI = I / (nr[:, 0] ** D * np.sqrt(tau) ** D)
+        return nI, nx, nr
+    if D == 2:
+        nr[:, 0] = tau / r[:, 0]
+        nr[:, 1] = tau / r[:, 1]
+        nr[:, 2] = -(tau * r[:, 2]) / (r[:, 0] * r[:, 1])
+    else:
+        nr[:, 0] = tau / r[:, 0]
+        nr[:, 1] = tau / r[:, 1]
+        nr[:, 2] = tau / r[:, 2]
+        nr[:, 3] = -(tau * r[:, 3]) / (r[:, 0] * r[:, 1])
+        nr[:, 4] = -(tau * r[:, 4]) / (r[:, 1] * r[:, 2])
+        nr[:, 5] = tau * (r[:, 3] * r[:, 4] / r[:, 1] - r[:, 5]) / (r[:, 0] * r[:, 2])
+    nI = I * np.prod(nr[:, :D], axis=1) / np.sqrt(tau) ** D
+    return nI, nx, nr
+
+
+# ---------------------------------------------------------------------------
+# Fitters
+# ---------------------------------------------------------------------------
+def l2_squared_loss(R, data):
+    """1/2 ||R - data||^2 (transport_loss.py:567-585), here in FP64."""
+    d = np.asarray(R, float) - np.asarray(data, float)
+    return 0.5 * float(np.sum(d * d))
+
+
+def _project(D, iso, I, x, r, d):
+    """Apply one update with the projections of GD_lib._step (GD_lib.py:502-522)."""
+    I2 = max(0.0, I + d[0])
+    x2 = x + d[1:1 + D]
+    if iso:
+        r2 = np.abs(r + d[1 + D])
+    else:
+        r2 = r + d[1 + D:]
+        if D == 2:
+            if r2[0] < 0:
+                r2[0], r2[2] = -r2[0], -r2[2]
+            if r2[1] < 0:
+                r2[1] = -r2[1]
+        else:
+            if r2[0] < 0:
+                r2[0], r2[3], r2[5] = -r2[0], -r2[3], -r2[5]
+            if r2[1] < 0:
+                r2[1], r2[4] = -r2[1], -r2[4]
+            if r2[2] < 0:
+                r2[2] = -r2[2]
+    return I2, x2, r2
+
+
+def newton_linesearch(I, x, r, geom, data, max_iter, tol=1e-4, min_iter=1, seed=1):
+    """Per-atom damped Newton with accept/reject (NewtonGaussian.py:18-109 and
+    GD_lib.py:492-536), null regulariser, no initial guess.  FP64 throughout.
+
+    Returns (I, x, r, E) with E the energy history."""
+    D = geom.dim
+    I, x = np.array(I, float).reshape(-1), np.array(x, float).reshape(-1, D)
+    r = np.array(r, float).reshape(I.size, -1)
+    iso = r.shape[1] == 1
+    n = I.size
+    if not hasattr(max_iter, "__iter__"):
+        max_iter = [max_iter, 1]
+    eps = 1e-4
+    h = [1.0] * n
+    data = np.asarray(data, float)
+    R = radproj(I, x, r, geom)
+    E = [l2_squared_loss(R, data)]
+    rs = np.random.RandomState(seed)
+    for it in range(max_iter[0]):
+        brk = [True, True]
+        for j in rs.permutation(np.arange(n)):
+            for _ in range(max_iter[1]):
+                brk[1] = True
+                Rj = radproj(I[j:j + 1], x[j:j + 1], r[j:j + 1], geom)
+                C = data + Rj - R
+                _, df, ddf = derivs(I[j:j + 1], x[j:j + 1], r[j:j + 1], geom, C, "L2", 2)
+                df, ddf = df[0].copy(), ddf[0].copy()
+                ddf[np.diag_indices_from(ddf)] += 1.0 / h[j]
+                dx = np.linalg.solve(ddf, -df)
+                # _step
+                old = (I[j], x[j].copy(), r[j].copy())
+                I[j], x[j], r[j] = _project(D, iso, I[j], x[j], r[j], dx)
+                Rnew = R + radproj(I[j:j + 1], x[j:j + 1], r[j:j + 1], geom) - Rj
+                En = l2_squared_loss(Rnew, data)
+                if En > E[-1]:
+                    I[j], x[j], r[j] = old
+                    E.append(E[-1])
+                    if h[j] > 2 * eps:
+                        brk = [False, False]
+                    h[j] /= 10
+                else:
+                    R = Rnew
+                    E.append(En)
+                    if np.sqrt(np.sum(dx * dx)) > tol:
+                        brk = [False, False]
+                    h[j] *= 2
+                h[j] = min(max(h[j], eps), 1 / eps)
+                if brk[1]:
+                    break
+        if brk[0] and it > min_iter:
+            break
+    return I, x, r, np.array(E)
+
+
+def gd_linesearch_block(I, x, r, geom, data, max_iter, dim="xrI", tol=1e-4, min_iter=1, seed=1):
+    """Block gradient descent (GD_lib.py:82-178), null regulariser, FP64.
+
+    Divergence from the reference, kept deliberately: on reject the sinogram is
+    recomputed from the reverted atoms (GD_lib.py:157-160 leaves the rejected
+    sinogram in place)."""
+    D = geom.dim
+    I, x = np.array(I, float).reshape(-1), np.array(x, float).reshape(-1, D)
+    r = np.array(r, float).reshape(I.size, -1)
+    n = I.size
+    dim = "".join(c for c in "xrI" if c.lower() in dim.lower())
+    if not hasattr(max_iter, "__iter__"):
+        max_iter = [max_iter, 1]
+    eps = 1e-8
+    h = [[1e-2] * n for _ in dim]
+    data = np.asarray(data, float)
+    R = radproj(I, x, r, geom)
+    E = [l2_squared_loss(R, data)]
+    rs = np.random.RandomState(seed)
+    arrs = {"I": I, "x": x, "r": r}
+    for it in range(max_iter[0]):
+        brk = [True, True]
+        for j in rs.permutation(np.arange(n)):
+            for _ in range(max_iter[1]):
+                brk[1] = True
+                for t, name in enumerate(dim):
+                    Rj = radproj(I[j:j + 1], x[j:j + 1], r[j:j + 1], geom)
+                    _, df, _ = derivs(I[j:j + 1], x[j:j + 1], r[j:j + 1], geom, data + Rj - R, "L2", 1)
+                    df = df[0]
+                    if name == "I":
+                        grad = df[0:1]
+                    elif name == "x":
+                        grad = df[1:1 + D]
+                    else:
+                        grad = df[1 + D:]
+                    A = arrs[name]
+                    old = np.array(A[j], dtype=float, copy=True)
+                    h[t][j] = min(h[t][j], 1.0 / max(np.sqrt(np.sum(grad * grad)), eps))
+                    new = old - h[t][j] * grad if name != "I" else old - h[t][j] * grad[0]
+                    A[j] = new
+                    Rn = radproj(I, x, r, geom)
+                    En = l2_squared_loss(Rn, data)
+                    if En > E[-1]:
+                        A[j] = old
+                        E.append(E[-1])
+                        h[t][j] /= 2
+                    else:
+                        R = Rn
+                        E.append(En)
+                        h[t][j] *= 1.1
+                        if np.sqrt(np.sum((old - new) ** 2)) > tol * np.sqrt(np.sum(old * old)):
+                            brk = [False, False]
+                    h[t][j] = max(h[t][j], eps)
+                if brk[1]:
+                    break
+        if brk[0] and it > min_iter:
+            break
+    return I, x, r, np.array(E)
