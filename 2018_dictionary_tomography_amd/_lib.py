@@ -57,7 +57,22 @@ SIGNATURES = {
     "gt_volume": (_ci, [_P(gt_atoms), _P(gt_vol), _vp, _vp, _sz, _vp]),
     "gt_ft_workspace": (_sz, [_P(gt_atoms), _P(gt_kgrid)]),
     "gt_ft_eval": (_ci, [_P(gt_atoms), _P(gt_kgrid), _vp, _vp, _sz, _vp]),
+    "gt_profile_enable": (_ci, [_ci]),
+    "gt_profile_read": (_ci, [_ci, _P(ctypes.c_double), _P(ctypes.c_long)]),
 }
+
+GT_K_SETUP, GT_K_FORWARD, GT_K_MOMENTS, GT_K_FINISH, GT_K_VOLUME, GT_K_FT, GT_K_FIT = range(7)
+
+
+def profile_enable(on=True):
+    load().gt_profile_enable(1 if on else 0)
+
+
+def profile_read(kernel_id):
+    """(total_ms, count) of the launches of one kernel id since the last read."""
+    tot, cnt = ctypes.c_double(0), ctypes.c_long(0)
+    check(load().gt_profile_read(kernel_id, ctypes.byref(tot), ctypes.byref(cnt)), "gt_profile_read")
+    return tot.value, cnt.value
 
 _lib = None
 

@@ -12,6 +12,10 @@ namespace gt {
 // Thread-local last error message (gt_last_error).
 void set_error(const char *fmt, ...);
 
+// Optional per-launch event timing (gt_profile_enable / gt_profile_read).
+void prof_begin(hipStream_t st, int id);
+void prof_end(hipStream_t st, int id);
+
 #define GT_CHECK_ARG(cond, ...)          \
     do {                                 \
         if (!(cond)) {                   \

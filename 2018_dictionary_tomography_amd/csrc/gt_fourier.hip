@@ -255,7 +255,9 @@ extern "C" int gt_ft_eval(const gt_atoms *atoms, const gt_kgrid *grid, float *ou
     hipLaunchKernelGGL(ft_setup_kernel, dim3((atoms->n + 255) / 256), dim3(256), 0, st, *atoms, *grid, recs);
     const long blocks = (long)((grid->m0 + 3) / 4) * ((grid->m1 + 3) / 4) * ((grid->m2 + 63) / 64);
     GT_CHECK_ARG(blocks < (1L << 31), "gt_ft_eval: grid too large");
+    prof_begin(st, GT_K_FT);
     hipLaunchKernelGGL(ft_grid_kernel, dim3((unsigned)blocks), dim3(256), 0, st, recs, atoms->n, *grid, (float2 *)out);
+    prof_end(st, GT_K_FT);
     GT_HIP(hipGetLastError());
     return 0;
 }

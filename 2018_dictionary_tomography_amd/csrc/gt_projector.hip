@@ -411,10 +411,12 @@ int launch_setup(const gt_atoms *A, const gt_geom *G, PairRec *recs, hipStream_t
     const long total = (long)A->n * G->nA;
     if (total == 0) return 0;
     const unsigned blocks = (unsigned)((total + 255) / 256);
+    prof_begin(st, GT_K_SETUP);
     if (G->dim == 3)
         hipLaunchKernelGGL(setup_kernel<3>, dim3(blocks), dim3(256), 0, st, *A, *G, recs);
     else
         hipLaunchKernelGGL(setup_kernel<2>, dim3(blocks), dim3(256), 0, st, *A, *G, recs);
+    prof_end(st, GT_K_SETUP);
     GT_HIP(hipGetLastError());
     return 0;
 }
@@ -433,6 +435,7 @@ int launch_forward(const gt_atoms *A, const gt_geom *G, const PairRec *recs, flo
 {
     const size_t nout = (size_t)G->nA * G->n0 * G->n1;
     if (A->n == 0) return launch_fill(R, nout, 0.0f, st);
+    prof_begin(st, GT_K_FORWARD);
     if (G->dim == 3) {
         const int tiles = ((G->n0 + TILE_ROWS - 1) / TILE_ROWS) * ((G->n1 + TILE_COLS - 1) / TILE_COLS);
         hipLaunchKernelGGL(forward3_kernel, dim3(tiles, G->nA), dim3(256), 0, st, recs, A->n, G->n0, G->n1, G->p0,
@@ -441,6 +444,7 @@ int launch_forward(const gt_atoms *A, const gt_geom *G, const PairRec *recs, flo
         const int tiles = (G->n0 + 255) / 256;
         hipLaunchKernelGGL(forward2_kernel, dim3(tiles, G->nA), dim3(256), 0, st, recs, A->n, G->n0, G->p0, R);
     }
+    prof_end(st, GT_K_FORWARD);
     GT_HIP(hipGetLastError());
     return 0;
 }
@@ -452,16 +456,21 @@ static void launch_deriv_kernels(const gt_atoms *A, const gt_geom *G, const Pair
 {
     constexpr int K = D - 1;
     const long pairs = (long)A->n * G->nA;
+    prof_begin(st, GT_K_MOMENTS);
     hipLaunchKernelGGL((moments_kernel<K, MODE>), dim3((unsigned)((pairs + 3) / 4)), dim3(256), 0, st, recs, A->n,
                        G->nA, G->n0, G->n1, G->p0, G->p1, C, mom);
+    prof_end(st, GT_K_MOMENTS);
+    prof_begin(st, GT_K_FINISH);
     hipLaunchKernelGGL((finish_kernel<D, MODE>), dim3(A->n), dim3(64), 0, st, *A, *G, recs, mom, sumC2, want_hess,
                        f, g, H);
+    prof_end(st, GT_K_FINISH);
 }
 
 int launch_derivs(const gt_atoms *A, const gt_geom *G, const PairRec *recs, float *mom, double *sumC2,
                   const float *C, int mode, int want_hess, double *f, double *g, double *H, hipStream_t st)
 {
     if (A->n == 0) return 0;
+    GT_CHECK_ARG((long)A->n * G->nA / 4 < (1L << 31), "too many (atom, angle) pairs for one launch");
     if (mode == MODE_L2 && f) {
         hipLaunchKernelGGL(sumsq_kernel, dim3(G->nA), dim3(256), 0, st, C, (long)G->n0 * G->n1, sumC2);
     }

@@ -268,11 +268,15 @@ extern "C" int gt_volume(const gt_atoms *atoms, const gt_vol *vol, float *u, voi
         hipLaunchKernelGGL(vol_setup_kernel<3>, dim3(sb), dim3(256), 0, st, *atoms, V, recs);
         const long blocks = (long)((V.m0 + 3) / 4) * ((V.m1 + 3) / 4) * ((V.m2 + 63) / 64);
         GT_CHECK_ARG(blocks < (1L << 31), "gt_volume: grid too large");
+        prof_begin(st, GT_K_VOLUME);
         hipLaunchKernelGGL(volume3_kernel, dim3((unsigned)blocks), dim3(256), 0, st, recs, atoms->n, V, u);
+        prof_end(st, GT_K_VOLUME);
     } else {
         hipLaunchKernelGGL(vol_setup_kernel<2>, dim3(sb), dim3(256), 0, st, *atoms, V, recs);
         const long blocks = (long)((V.m0 + 15) / 16) * ((V.m1 + 63) / 64);
+        prof_begin(st, GT_K_VOLUME);
         hipLaunchKernelGGL(volume2_kernel, dim3((unsigned)blocks), dim3(256), 0, st, recs, atoms->n, V, u);
+        prof_end(st, GT_K_VOLUME);
     }
     GT_HIP(hipGetLastError());
     return 0;

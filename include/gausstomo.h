@@ -87,6 +87,22 @@ typedef struct gt_kgrid {
 const char *gt_last_error(void);
 const char *gt_version(void);
 
+/* Kernel ids for gt_profile_read. */
+#define GT_K_SETUP 0   /* per-(atom, angle) FP64 setup            */
+#define GT_K_FORWARD 1 /* forward projection                      */
+#define GT_K_MOMENTS 2 /* derivative pixel moments                */
+#define GT_K_FINISH 3  /* FP64 moment -> gradient / Hessian       */
+#define GT_K_VOLUME 4  /* volume render                           */
+#define GT_K_FT 5      /* Fourier evaluation                      */
+#define GT_K_FIT 6     /* fitter kernels                          */
+
+/* Timing support (no counterpart in the reference, which only prints wall
+ * time): when enabled, each kernel launch is bracketed by HIP events on its
+ * stream; gt_profile_read waits for the recorded launches of one kernel id,
+ * returns their summed duration in ms and their count, and forgets them. */
+int gt_profile_enable(int on);
+int gt_profile_read(int kernel_id, double *total_ms, long *count);
+
 /* RadProj: R[a][i0][i1] = sum over atoms of the line integral (written, not accumulated). */
 size_t gt_forward_workspace(const gt_atoms *atoms, const gt_geom *geom);
 int gt_forward(const gt_atoms *atoms, const gt_geom *geom, float *R, void *ws, size_t ws_bytes,
