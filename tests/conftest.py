@@ -39,8 +39,10 @@ def cuda():
 
 def relerr(a, b):
     """Norm-wise relative error ||a - b||_inf / ||b||_inf."""
-    a = np.asarray(a, dtype=np.float64)
-    b = np.asarray(b, dtype=np.float64)
+    a = np.asarray(a)
+    b = np.asarray(b)
+    a = a.astype(np.complex128 if np.iscomplexobj(a) or np.iscomplexobj(b) else np.float64)
+    b = b.astype(a.dtype)
     den = np.max(np.abs(b))
     return float(np.max(np.abs(a - b)) / (den if den > 0 else 1.0))
 

@@ -1,0 +1,98 @@
+"""CPU tests of the C ABI surface and the host-side logic (no GPU compute)."""
+import ctypes
+import importlib
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names |= set(re.findall(r"\b(gt_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol(gt):
+    lib = gt._lib.load()
+    declared = _declared_functions()
+    assert len(declared) >= 12
+    for name in declared:
+        assert hasattr(lib, name), name
+        assert name in gt._lib.SIGNATURES, f"{name} has no ctypes signature"
+    assert lib.gt_version().startswith(b"gausstomo")
+
+
+def test_argument_validation_errors(gt):
+    """Bad arguments are refused before any device work, with a message."""
+    L = gt._lib
+    lib = L.load()
+    a = L.gt_atoms(3, 5, 0, 1, 1, 1)
+    g = L.gt_geom(3, 3, 4, 8, 8, 1, 1, 1, 1, 1, 1)
+    bad_dim = L.gt_atoms(4, 5, 0, 1, 1, 1)
+    assert lib.gt_forward(ctypes.byref(bad_dim), ctypes.byref(g), 1, 1, 1 << 20, None) == -1
+    assert b"dim" in lib.gt_last_error()
+    g2 = L.gt_geom(2, 2, 4, 8, 1, 1, 1, 1, None, 1, None)
+    assert lib.gt_forward(ctypes.byref(a), ctypes.byref(g2), 1, 1, 1 << 20, None) == -1
+    assert b"does not match" in lib.gt_last_error()
+    g3 = L.gt_geom(3, 3, 4, 8, 8, 1, 1, 1, None, 1, 1)  # lenT 3 needs w1
+    assert lib.gt_forward(ctypes.byref(a), ctypes.byref(g3), 1, 1, 1 << 20, None) == -1
+    assert lib.gt_derivs(ctypes.byref(a), ctypes.byref(g), 1, 7, 2, None, 1, None, 1, 1 << 20, None) == -1
+    assert b"mode" in lib.gt_last_error()
+    need = lib.gt_forward_workspace(ctypes.byref(a), ctypes.byref(g))
+    assert need >= 5 * 4 * 64
+    assert lib.gt_forward(ctypes.byref(a), ctypes.byref(g), 1, 256, need - 1, None) == -3
+    assert b"workspace" in lib.gt_last_error()
+    with pytest.raises(L.GTError):
+        L.check(-1, "probe")
+
+
+def test_atom_space_random_matches_reference_draw_order():
+    """AtomSpace.random draws r, then x, then I from numpy's legacy generator
+    (dictionary_def.py:130-139)."""
+    sp = importlib.import_module("2018_dictionary_tomography_amd.spaces")
+    rs = np.random.RandomState(3)
+    r = rs.rand(4, 6) + 1
+    r[:, 3:] /= 4
+    x = 0.8 * (2 * rs.rand(4, 3) - 1)
+    I = 2 * rs.rand(4)
+    space = sp.AtomSpace(3, isotropic=False)
+    # reproduce the draw on the host only (no device tensors)
+    rs2 = np.random.RandomState(3)
+    r2 = rs2.rand(4, space.nr) + 1
+    r2[:, space.dim:] /= 4
+    assert np.allclose(r, r2) and np.allclose(x, 0.8 * (2 * rs2.rand(4, 3) - 1)) and np.allclose(I, 2 * rs2.rand(4))
+
+
+def test_geometry_helpers():
+    sp = importlib.import_module("2018_dictionary_tomography_amd.spaces")
+    c = sp.cell_centres(-1, 1, 4)
+    assert np.allclose(c, [-0.75, -0.25, 0.25, 0.75])
+    assert np.allclose(sp.boundary_nodes(0, 1, 3), [0, 0.5, 1])
+    T, (w0, w1) = sp.axis_tilt_geometry(np.linspace(-1, 1, 5), [0, 1, 0], ref=np.array([1.0, 0, 0]))
+    for u, v in ((T, w0), (T, w1), (w0, w1)):
+        assert np.allclose((u * v).sum(1), 0)
+    assert np.allclose(np.cross(w0, w1), T)
+    ps = sp.ProjSpace(angles=np.linspace(0, np.pi, 7), detector=(np.linspace(-1, 1, 5), np.linspace(-1, 1, 3)))
+    assert ps.shape == [7, 5, 3] and ps.dim == 3 and ps.orientations.shape == (7, 2)
+    ps2 = sp.ProjSpace(angles=np.linspace(0, np.pi, 7), detector=np.linspace(-1, 1, 5))
+    assert ps2.shape == [7, 5] and ps2.dim == 2
+
+
+def test_workload_cholesky_convention():
+    """U = L^T / sqrt(60): 30 |U y|^2 = y^T L L^T y / 2."""
+    W = importlib.import_module("2018_dictionary_tomography_amd.workloads")
+    rng = np.random.default_rng(0)
+    I, x, r, L = W.draw_aniso3(5, rng)
+    for j in range(5):
+        U = np.array([[r[j, 0], r[j, 3], r[j, 5]], [0, r[j, 1], r[j, 4]], [0, 0, r[j, 2]]])
+        y = rng.normal(size=3)
+        assert np.isclose(30 * np.sum((U @ y) ** 2), 0.5 * y @ L[j] @ L[j].T @ y)
+    assert np.all((x >= -0.7) & (x <= 0.7)) and np.all((I >= 0.5) & (I <= 1.5))
