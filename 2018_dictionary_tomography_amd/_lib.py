@@ -57,6 +57,7 @@ SIGNATURES = {
     "gt_volume": (_ci, [_P(gt_atoms), _P(gt_vol), _vp, _vp, _sz, _vp]),
     "gt_ft_workspace": (_sz, [_P(gt_atoms), _P(gt_kgrid)]),
     "gt_ft_eval": (_ci, [_P(gt_atoms), _P(gt_kgrid), _vp, _vp, _sz, _vp]),
+    "gt_fit_run": (_ci, [_vp, _ci, _ci, _ci, _vp]),
     "gt_profile_enable": (_ci, [_ci]),
     "gt_profile_read": (_ci, [_ci, _P(ctypes.c_double), _P(ctypes.c_long)]),
 }

@@ -1,0 +1,115 @@
+// gt_pixel.h -- per-lane pixel loops over one (atom, angle) footprint box.
+//
+// A wave walks the pixels of a PairRec box in row-major order, 64 pixels per
+// step, keeping (row, col) incrementally so no integer division is done per
+// pixel.  Pixels with m > 1 contribute nothing (NewtonBases.py:49-56, 375-380).
+#pragma once
+
+#include "gt_common.h"
+
+namespace gt {
+
+constexpr int MOMW = 32;  // floats per moment record
+
+// Moment accumulation over one pair box.
+//   MODE 0 (linear): w1 = C E          acc[0..NM)
+//   MODE 1 (L2):     w1 = (aE - C) E, w2 = E^2, acc[30] += (aE-C)^2, acc[31] += C^2
+//   MODE 2 (resid):  w1 = C E, w2 = E^2
+// acc[15 + m] holds the w2 moments.  Ca points at this angle's [n0][n1] slab.
+template <int K, int MODE>
+__device__ inline void pair_moments(const PairRec &rec, const float *__restrict__ p0, const float *__restrict__ p1,
+                                    const float *__restrict__ Ca, int n1, int lane, float *acc)
+{
+    const int W = (K == 2) ? rec.hi[1] - rec.lo[1] + 1 : 1;
+    const int H = rec.hi[0] - rec.lo[0] + 1;
+    const long total = (W > 0 && H > 0) ? (long)W * H : 0;
+    if (total == 0) return;
+    int r = lane / W, cidx = lane - (lane / W) * W;
+    const int dr = 64 / W, dc = 64 - dr * W;
+    const float kap = (float)KAPPA;
+    for (long k = lane; k < total; k += 64) {
+        const int i0 = rec.lo[0] + r;
+        const int i1 = (K == 2) ? rec.lo[1] + cidx : 0;
+        const float dp0 = (p0[i0] - rec.c_hi[0]) - rec.c_lo[0];
+        const float dp1 = (K == 2) ? (p1[i1] - rec.c_hi[1]) - rec.c_lo[1] : 0.0f;
+        const float q = pair_q(rec.q, dp0, dp1);
+        if (q >= kap) {
+            const float E = __builtin_amdgcn_exp2f(q);
+            const float cv = Ca[(size_t)i0 * n1 + i1];
+            float w1;
+            if (MODE == GT_MODE_L2) {
+                const float res = fmaf(rec.alpha, E, -cv);
+                w1 = res * E;
+                acc[30] = fmaf(res, res, acc[30]);
+                acc[31] = fmaf(cv, cv, acc[31]);
+            } else {
+                w1 = cv * E;
+            }
+            const float w2 = E * E;
+            if (K == 2) {
+                const float x1 = dp0, y1 = dp1;
+                const float x2 = x1 * x1, xy = x1 * y1, y2 = y1 * y1;
+                const float mono[15] = {1.0f,    x1,      y1,      x2,      xy,      y2,      x2 * x1, x2 * y1,
+                                        x1 * y2, y2 * y1, x2 * x2, x2 * xy, x2 * y2, xy * y2, y2 * y2};
+#pragma unroll
+                for (int m = 0; m < 15; m++) acc[m] = fmaf(w1, mono[m], acc[m]);
+                if (MODE != GT_MODE_LINEAR) {
+#pragma unroll
+                    for (int m = 0; m < 15; m++) acc[15 + m] = fmaf(w2, mono[m], acc[15 + m]);
+                }
+            } else {
+                const float x1 = dp0, x2 = x1 * x1;
+                const float mono[5] = {1.0f, x1, x2, x2 * x1, x2 * x2};
+#pragma unroll
+                for (int m = 0; m < 5; m++) acc[m] = fmaf(w1, mono[m], acc[m]);
+                if (MODE != GT_MODE_LINEAR) {
+#pragma unroll
+                    for (int m = 0; m < 5; m++) acc[15 + m] = fmaf(w2, mono[m], acc[15 + m]);
+                }
+            }
+        }
+        cidx += dc;
+        r += dr;
+        if (cidx >= W) {
+            cidx -= W;
+            r++;
+        }
+    }
+}
+
+// Sum acc[32] over the 64 lanes of the calling wave through an LDS tile
+// red[64][33] owned by this wave; lanes 0..31 return their column's total.
+// The caller's wave must not touch red[] concurrently; callers place a
+// workgroup barrier or wave barrier around it as described at call sites.
+__device__ inline float wave_sum32(float (*red)[MOMW + 1], const float *acc, int lane)
+{
+#pragma unroll
+    for (int v = 0; v < MOMW; v++) red[lane][v] = acc[v];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int col = lane & 31, half = lane >> 5;
+    float s = 0.0f;
+    for (int i = 0; i < 32; i++) s += red[half * 32 + i][col];
+    s += __shfl_xor(s, 32);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return s;
+}
+
+__device__ inline float wave_sum_f32(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ inline double wave_sum_f64(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+}  // namespace gt

@@ -13,7 +13,7 @@
 // __RadProj (NewtonBases2D.py:284-296), __L2derivs_RadProj_aniso_3D and
 // __derivs_RadProj_aniso_3D (NewtonBases.py:792-870, 993-1069) and their 2D
 // versions (NewtonBases2D.py:329-401, 444-515).
-#include "gt_common.h"
+#include "gt_pixel.h"
 
 namespace gt {
 
@@ -191,62 +191,7 @@ __global__ __launch_bounds__(256) void moments_kernel(const PairRec *__restrict_
     if (valid) {
         const PairRec rec = recs[pair];
         const int a = (int)(pair / N);
-        const int W = (K == 2) ? rec.hi[1] - rec.lo[1] + 1 : 1;
-        const int H = rec.hi[0] - rec.lo[0] + 1;
-        const long total = (W > 0 && H > 0) ? (long)W * H : 0;
-        int r = (W > 0) ? lane / W : 0, cidx = (W > 0) ? lane - r * W : 0;
-        const int dr = (W > 0) ? 64 / W : 0, dc = (W > 0) ? 64 - dr * W : 0;
-        const float *Ca = C + (size_t)a * n0 * n1;
-        const float kap = (float)KAPPA;
-        for (long k = lane; k < total; k += 64) {
-            const int i0 = rec.lo[0] + r;
-            const int i1 = (K == 2) ? rec.lo[1] + cidx : 0;
-            const float dp0 = (p0[i0] - rec.c_hi[0]) - rec.c_lo[0];
-            const float dp1 = (K == 2) ? (p1[i1] - rec.c_hi[1]) - rec.c_lo[1] : 0.0f;
-            const float q = pair_q(rec.q, dp0, dp1);
-            if (q >= kap) {
-                const float E = __builtin_amdgcn_exp2f(q);
-                const float cv = Ca[(size_t)i0 * n1 + i1];
-                float w1;
-                if (MODE == MODE_L2) {
-                    const float res = fmaf(rec.alpha, E, -cv);
-                    w1 = res * E;
-                    acc[30] = fmaf(res, res, acc[30]);
-                    acc[31] = fmaf(cv, cv, acc[31]);
-                } else {
-                    w1 = cv * E;
-                }
-                const float w2 = E * E;
-                if (K == 2) {
-                    const float x1 = dp0, y1 = dp1;
-                    const float x2 = x1 * x1, xy = x1 * y1, y2 = y1 * y1;
-                    const float mono[15] = {1.0f,    x1,      y1,      x2,      xy,
-                                            y2,      x2 * x1, x2 * y1, x1 * y2, y2 * y1,
-                                            x2 * x2, x2 * xy, x2 * y2, xy * y2, y2 * y2};
-#pragma unroll
-                    for (int m = 0; m < 15; m++) acc[m] = fmaf(w1, mono[m], acc[m]);
-                    if (MODE != MODE_LINEAR) {
-#pragma unroll
-                        for (int m = 0; m < 15; m++) acc[15 + m] = fmaf(w2, mono[m], acc[15 + m]);
-                    }
-                } else {
-                    const float x1 = dp0, x2 = x1 * x1;
-                    const float mono[5] = {1.0f, x1, x2, x2 * x1, x2 * x2};
-#pragma unroll
-                    for (int m = 0; m < 5; m++) acc[m] = fmaf(w1, mono[m], acc[m]);
-                    if (MODE != MODE_LINEAR) {
-#pragma unroll
-                        for (int m = 0; m < 5; m++) acc[15 + m] = fmaf(w2, mono[m], acc[15 + m]);
-                    }
-                }
-            }
-            cidx += dc;
-            r += dr;
-            if (cidx >= W) {
-                cidx -= W;
-                r++;
-            }
-        }
+        pair_moments<K, MODE>(rec, p0, p1, C + (size_t)a * n0 * n1, n1, lane, acc);
     }
 #pragma unroll
     for (int v = 0; v < MOM_STRIDE; v++) red[wave][lane][v] = acc[v];

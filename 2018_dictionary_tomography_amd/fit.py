@@ -1,0 +1,212 @@
+"""Fitters over the HIP library.
+
+newton_linesearch   NewtonGaussian.linesearch (NewtonGaussian.py:18-109) with
+                    GD_lib._step (GD_lib.py:492-536): device-resident, one
+                    workgroup per fit (gt_fit_run), FP64 block solves and FP64
+                    energy deltas.
+gd_linesearch_block GD_lib.linesearch_block (GD_lib.py:82-178): block gradient
+                    descent driven from the host over the batched forward and
+                    derivative kernels.
+
+Both take the null regulariser and the l2 fidelity, which is what every
+configuration uses (GaussDictCode/__init__.py:177-178, transport_loss.py:567-600).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import ops
+from ._lib import check, load, stream_handle
+
+
+class gt_fit_slot(ctypes.Structure):
+    _fields_ = [
+        ("active", ctypes.c_int), ("iso", ctypes.c_int), ("N", ctypes.c_int), ("nA", ctypes.c_int),
+        ("n0", ctypes.c_int), ("n1", ctypes.c_int), ("lenT", ctypes.c_int), ("sorted", ctypes.c_int),
+        ("max_outer", ctypes.c_int), ("max_inner", ctypes.c_int), ("min_iter", ctypes.c_int), ("pad0", ctypes.c_int),
+        ("tol", ctypes.c_double),
+        ("T", ctypes.c_void_p), ("w0", ctypes.c_void_p), ("w1", ctypes.c_void_p), ("p0", ctypes.c_void_p),
+        ("p1", ctypes.c_void_p), ("perm", ctypes.c_void_p),
+        ("I", ctypes.c_void_p), ("x", ctypes.c_void_p), ("r", ctypes.c_void_p), ("resid", ctypes.c_void_p),
+        ("h", ctypes.c_void_p), ("Ehist", ctypes.c_void_p), ("scratch", ctypes.c_void_p),
+        ("it", ctypes.c_int), ("pos", ctypes.c_int), ("inner", ctypes.c_int), ("jj", ctypes.c_int),
+        ("brk0", ctypes.c_int), ("brk1", ctypes.c_int), ("done", ctypes.c_int), ("pad1", ctypes.c_int),
+        ("E", ctypes.c_double), ("updates", ctypes.c_longlong), ("accepted", ctypes.c_longlong),
+    ]
+
+
+SLOT_BYTES = ctypes.sizeof(gt_fit_slot)
+MAX_ANGLES = 128
+
+
+def _declare():
+    lib = load()
+    if not hasattr(lib.gt_fit_run, "_declared"):
+        lib.gt_fit_run.restype = ctypes.c_int
+        lib.gt_fit_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+        lib.gt_fit_run._declared = True
+    return lib
+
+
+def permutations(n, max_outer, seed=1):
+    """Atom order of every outer iteration: random.seed(1) then one
+    random.permutation(arange(n)) per outer iteration (NewtonGaussian.py:37, 55)."""
+    rs = np.random.RandomState(seed)
+    return np.stack([rs.permutation(np.arange(n)) for _ in range(max_outer)]).astype(np.int32)
+
+
+class FitRequest:
+    """Device state of one Newton fit: atoms, residual, step sizes, history."""
+
+    def __init__(self, atoms, space, data, max_iter, tol=1e-4, min_iter=1, seed=1, record=True, device=None):
+        dev = device or atoms.I.device
+        if not hasattr(max_iter, "__iter__"):
+            max_iter = [max_iter, 1]
+        max_iter = list(max_iter) + [1] * (2 - len(max_iter))
+        self.max_outer, self.max_inner = int(max_iter[0]), int(max_iter[1])
+        self.geom = ops.device_geometry(space, dev)
+        if self.geom.nA > MAX_ANGLES:
+            raise ValueError(f"the fitter supports at most {MAX_ANGLES} angles, got {self.geom.nA}")
+        self.dim = self.geom.dim
+        self.I = atoms.I.detach().to(dev, torch.float32).clone().contiguous()
+        self.x = atoms.x.detach().to(dev, torch.float32).clone().contiguous()
+        self.r = atoms.r.detach().to(dev, torch.float32).clone().contiguous()
+        self.N = int(self.I.numel())
+        self.iso = int(self.r.shape[1] == 1)
+        data = data.data if hasattr(data, "data") and not isinstance(data, torch.Tensor) else data
+        self.data = torch.as_tensor(data, device=dev).to(torch.float32).reshape(self.geom.shape).contiguous()
+        self.resid = ops.forward(self, space).sub_(self.data)
+        r64 = self.resid.double()
+        self.E0 = float(0.5 * (r64 * r64).sum())
+        self.perm = torch.as_tensor(permutations(self.N, self.max_outer, seed), device=dev).contiguous()
+        self.h = torch.ones(self.N, dtype=torch.float64, device=dev)
+        nh = self.max_outer * self.max_inner * self.N + 1
+        self.Ehist = torch.zeros(nh, dtype=torch.float64, device=dev) if record else None
+        if record:
+            self.Ehist[0] = self.E0
+        self.scratch = torch.zeros(self.geom.nA * 32, dtype=torch.float32, device=dev)
+        g = self.geom
+        self.slot = gt_fit_slot(
+            active=1 if self.N > 0 else 0, iso=self.iso, N=self.N, nA=g.nA, n0=g.n0, n1=g.n1, lenT=g.lenT,
+            sorted=g.sorted, max_outer=self.max_outer, max_inner=self.max_inner, min_iter=int(min_iter), pad0=0,
+            tol=float(tol), T=g.T.data_ptr(), w0=g.w0.data_ptr(), w1=g.w1.data_ptr() if g.w1 is not None else None,
+            p0=g.p0.data_ptr(), p1=g.p1.data_ptr() if g.p1 is not None else None, perm=self.perm.data_ptr(),
+            I=self.I.data_ptr(), x=self.x.data_ptr(), r=self.r.data_ptr(), resid=self.resid.data_ptr(),
+            h=self.h.data_ptr(), Ehist=self.Ehist.data_ptr() if record else None, scratch=self.scratch.data_ptr(),
+            it=0, pos=0, inner=0, jj=0, brk0=1, brk1=1, done=0 if self.N > 0 else 1, pad1=0, E=self.E0,
+            updates=0, accepted=0)
+
+    def slot_bytes(self):
+        return bytes(self.slot)
+
+
+def read_slots(dev_table, n):
+    host = dev_table[: n * SLOT_BYTES].cpu().numpy().tobytes()
+    return [gt_fit_slot.from_buffer_copy(host, i * SLOT_BYTES) for i in range(n)]
+
+
+def run_slots(dev_table, nslots, dim, budget):
+    lib = _declare()
+    check(lib.gt_fit_run(ctypes.c_void_p(dev_table.data_ptr()), nslots, dim, budget, stream_handle(dev_table.device)),
+          "gt_fit_run")
+
+
+def newton_fit(atoms, space, data, max_iter, tol=1e-4, min_iter=1, seed=1, budget=None, record=True):
+    """Run one Newton fit to completion on the device; returns (request, final slot)."""
+    req = FitRequest(atoms, space, data, max_iter, tol, min_iter, seed, record)
+    dev = req.I.device
+    table = torch.as_tensor(np.frombuffer(req.slot_bytes(), dtype=np.uint8).copy(), device=dev)
+    budget = budget or max(req.N * req.max_inner, 1)
+    while True:
+        run_slots(table, 1, req.dim, budget)
+        s = read_slots(table, 1)[0]
+        if s.done or not s.active:
+            break
+    req.final = s
+    return req, s
+
+
+def newton_linesearch(recon, data, max_iter, fidelity=None, reg=None, Radon=None, view=None, guess=None, tol=1e-4,
+                      min_iter=1, **kwargs):
+    """Drop-in for NewtonGaussian.linesearch: fits `recon` in place and returns
+    (recon, E, F) with E == F the energy history (null regulariser)."""
+    if guess is not None:
+        raise NotImplementedError("the initial-guess path (KL_GaussRadon) is not part of this build")
+    space = Radon.range if Radon is not None else kwargs["space"]
+    req, s = newton_fit(recon, space, data, max_iter, tol, min_iter)
+    if s.done == 2:
+        raise np.linalg.LinAlgError("singular Newton system")
+    recon.I.copy_(req.I)
+    recon.x.copy_(req.x)
+    recon.r.copy_(req.r)
+    E = req.Ehist[: s.jj + 1].cpu().numpy()
+    return recon, E, E.copy()
+
+
+# ---------------------------------------------------------------------------
+# Block gradient descent, GD_lib.linesearch_block (GD_lib.py:82-178)
+# ---------------------------------------------------------------------------
+def gd_linesearch_block(recon, data, max_iter, space, dim="xrI", tol=1e-4, min_iter=1, seed=1):
+    """Block GD over (x, r, I) of each atom with the reference's step-size rules.
+    Energies in FP64.  Divergence from the reference, kept on purpose: on a
+    rejected step the sinogram of the reverted atoms is used again
+    (GD_lib.py:157-160 leaves the rejected sinogram in place)."""
+    dev = recon.I.device
+    D = recon.x.shape[1]
+    dims = "".join(c for c in "xrI" if c.lower() in dim.lower())
+    if not hasattr(max_iter, "__iter__"):
+        max_iter = [max_iter, 1]
+    max_iter = list(max_iter) + [1] * (2 - len(max_iter))
+    eps = 1e-8
+    n = recon.I.numel()
+    h = {c: np.full(n, 1e-2) for c in dims}
+    data = torch.as_tensor(data.data if hasattr(data, "data") and not isinstance(data, torch.Tensor) else data,
+                           device=dev).to(torch.float32)
+    R = ops.forward(recon, space)
+    E = [0.5 * float(((R - data).double() ** 2).sum())]
+    rs = np.random.RandomState(seed)
+    resid = R - data
+    for it in range(max_iter[0]):
+        brk = [True, True]
+        for j in rs.permutation(np.arange(n)):
+            for _ in range(max_iter[1]):
+                brk[1] = True
+                for name in dims:
+                    one = ops._AtomView(recon.I[j:j + 1], recon.x[j:j + 1], recon.r[j:j + 1])
+                    _, g, _ = ops.derivs(one, space, resid, "resid", order=1, want_f=False)
+                    g = g[0].cpu().numpy()
+                    if name == "I":
+                        grad = g[0:1]
+                    elif name == "x":
+                        grad = g[1:1 + D]
+                    else:
+                        grad = g[1 + D:]
+                    arr = {"I": recon.I, "x": recon.x, "r": recon.r}[name]
+                    old = arr[j].detach().clone()
+                    h[name][j] = min(h[name][j], 1.0 / max(np.sqrt(np.sum(grad * grad)), eps))
+                    step = torch.as_tensor(h[name][j] * grad, dtype=torch.float32, device=dev)
+                    new = old - (step[0] if name == "I" else step)
+                    arr[j] = new
+                    Rn = ops.forward(recon, space)
+                    rn = Rn - data
+                    En = 0.5 * float((rn.double() ** 2).sum())
+                    if En > E[-1]:
+                        arr[j] = old
+                        E.append(E[-1])
+                        h[name][j] /= 2
+                    else:
+                        resid = rn
+                        E.append(En)
+                        h[name][j] *= 1.1
+                        o64, n64 = old.double().cpu().numpy(), new.double().cpu().numpy()
+                        if np.sqrt(np.sum((o64 - n64) ** 2)) > tol * np.sqrt(np.sum(o64 * o64)):
+                            brk = [False, False]
+                    h[name][j] = max(h[name][j], eps)
+                if brk[1]:
+                    break
+        if brk[0] and it > min_iter:
+            break
+    return recon, np.array(E)

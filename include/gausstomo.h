@@ -128,6 +128,49 @@ size_t gt_ft_workspace(const gt_atoms *atoms, const gt_kgrid *grid);
 int gt_ft_eval(const gt_atoms *atoms, const gt_kgrid *grid, float *out, void *ws, size_t ws_bytes,
                void *stream);
 
+/*
+ * Device-resident damped-Newton fitter (NewtonGaussian.linesearch with
+ * GD_lib._step, NewtonGaussian.py:18-109, GD_lib.py:492-536; null
+ * regulariser, l2 fidelity).  One gt_fit_slot per fit request, in device
+ * memory; one workgroup advances one slot.  The host fills the problem and
+ * state pointers, the initial residual (forward - data) and energy, and the
+ * per-outer-iteration atom permutations; gt_fit_run then performs up to
+ * `budget` atom updates on every active slot and returns (stream-ordered),
+ * leaving the progress fields updated.  Slots with done != 0 or active == 0
+ * are skipped, so the same launch (or a captured graph of it) can be replayed
+ * while requests are admitted into and retired from the table.
+ */
+typedef struct gt_fit_slot {
+    /* problem (read-only for the kernel) */
+    int active;       /* 0: slot unused */
+    int iso;          /* atoms.iso */
+    int N, nA, n0, n1;
+    int lenT, sorted;
+    int max_outer, max_inner, min_iter;
+    int pad0;
+    double tol;       /* convergence threshold on ||dx|| */
+    const float *T, *w0, *w1, *p0, *p1; /* geometry (gt_geom layout) */
+    const int *perm;  /* [max_outer][N] atom order of each outer iteration */
+    /* state (read-write) */
+    float *I, *x, *r; /* atoms, gt_atoms layout */
+    float *resid;     /* [nA][n0][n1] current projection minus data */
+    double *h;        /* [N] per-atom step sizes, initialised to 1 */
+    double *Ehist;    /* [max_outer*max_inner*N + 1] energy after each update, or NULL */
+    float *scratch;   /* [nA][32] pixel-moment scratch of this slot */
+    /* progress (read-write) */
+    int it, pos, inner, jj;  /* outer iteration, position in perm, inner step, history index */
+    int brk0, brk1;          /* convergence flags of the current outer / inner loop */
+    int done;                /* 1: converged or out of iterations; 2: singular solve */
+    int pad1;
+    double E;                /* current energy 1/2 ||resid||^2 */
+    long long updates;       /* atom updates performed */
+    long long accepted;      /* accepted updates */
+} gt_fit_slot;
+
+/* nA must be <= GT_FIT_MAX_ANGLES. dim 2 or 3. */
+#define GT_FIT_MAX_ANGLES 128
+int gt_fit_run(gt_fit_slot *slots, int nslots, int dim, int budget, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,122 @@
+"""Device Newton fitter and block GD vs the FP64 oracle restatements of
+NewtonGaussian.linesearch / GD_lib.linesearch_block (same inputs, same
+RandomState(1) permutations).
+
+The GPU computes derivatives in FP32 (1e-7 relative) and solves / compares
+energies in FP64; the oracle is FP64 throughout.  Accept/reject decisions and
+hence trajectories agree as long as no decision is within ~1e-7 of a tie, so
+the energy histories are compared entry-wise at 1e-5 relative to E[0].
+"""
+import importlib
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import f32
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup3(gt, n, nA, det, seed):
+    W = importlib.import_module("2018_dictionary_tomography_amd.workloads")
+    rng = np.random.default_rng(seed)
+    I, x, r, _ = W.draw_aniso3(n, rng, rho=(0.05, 0.1))
+    T, ortho, dv = W.tilt_series(nA, 60.0, det=det)
+    space = gt.ProjSpace(orientations=T, ortho=ortho, detector=dv)
+    return I, x, r, space, rng
+
+
+def _oracle_geom(oracle, space):
+    return oracle.Geometry(f32(space.orientations), tuple(f32(o) for o in space.ortho),
+                           [f32(p) for p in space.detector])
+
+
+def test_newton_3d_matches_oracle(gt, oracle, cuda):
+    F = importlib.import_module("2018_dictionary_tomography_amd.fit")
+    I, x, r, space, rng = _setup3(gt, 6, 11, 40, seed=5)
+    aspace = gt.AtomSpace(3, isotropic=False)
+    truth = gt.AtomElement(aspace, x, r, I, device=cuda)
+    data = gt.ops.forward(truth, space)
+    data = data + torch.as_tensor(rng.normal(0, 0.01 * float(data.max()), data.shape), dtype=torch.float32,
+                                  device=cuda)
+    x0 = x + rng.normal(0, 0.02, x.shape)
+    I0 = I * 0.8
+    start = gt.AtomElement(aspace, x0, r, I0, device=cuda)
+    req, s = F.newton_fit(start, space, data, [4, 1], tol=1e-6)
+    assert s.done == 1 and s.jj == 4 * 6
+    Eg = req.Ehist[: s.jj + 1].cpu().numpy()
+    og = _oracle_geom(oracle, space)
+    Io, xo, ro, Eo = oracle.newton_linesearch(f32(I0), f32(x0), f32(r), og, data.double().cpu().numpy(), [4, 1],
+                                              tol=1e-6)
+    assert abs(Eg[0] - Eo[0]) < 1e-6 * Eo[0]
+    assert np.all(np.diff(Eg) <= 0)
+    assert np.abs(Eg - Eo).max() < 1e-5 * Eo[0]
+    assert np.abs(req.x.cpu().numpy() - xo).max() < 1e-4
+    # energy bookkeeping: the tracked E equals 1/2||resid||^2 of the stored residual
+    r64 = req.resid.double()
+    assert abs(float(0.5 * (r64 * r64).sum()) - s.E) < 1e-9 * s.E
+    # the stored residual equals a fresh forward of the fitted atoms minus the data
+    fresh = gt.ops.forward(req, space) - req.data
+    assert float((fresh - req.resid).abs().max()) < 1e-5 * float(req.data.abs().max())
+
+
+def test_newton_2d_iso_matches_oracle(gt, oracle, cuda):
+    F = importlib.import_module("2018_dictionary_tomography_amd.fit")
+    W = importlib.import_module("2018_dictionary_tomography_amd.workloads")
+    rng = np.random.default_rng(9)
+    I, x, r = W.draw_iso2(5, rng, rho=(0.04, 0.08))
+    T, ortho, det = W.fan_2d(60, 96)
+    space = gt.ProjSpace(orientations=T, ortho=ortho, detector=det)
+    aspace = gt.AtomSpace(2, isotropic=True)
+    data = gt.ops.forward(gt.AtomElement(aspace, x, r, I, device=cuda), space)
+    x0, I0 = x + rng.normal(0, 0.01, x.shape), I * 1.2
+    req, s = F.newton_fit(gt.AtomElement(aspace, x0, r * 1.1, I0, device=cuda), space, data, [6, 1], tol=1e-7)
+    Eg = req.Ehist[: s.jj + 1].cpu().numpy()
+    Io, xo, ro, Eo = oracle.newton_linesearch(f32(I0), f32(x0), f32(r * 1.1), _oracle_geom(oracle, space),
+                                              data.double().cpu().numpy(), [6, 1], tol=1e-7)
+    n = min(len(Eg), len(Eo))
+    assert np.all(np.diff(Eg) <= 0)
+    assert np.abs(Eg[:n] - Eo[:n]).max() < 1e-5 * Eo[0]
+    assert Eg[-1] < 1e-2 * Eg[0]
+
+
+def test_newton_multi_slot_equals_single(gt, cuda):
+    """Several fits in one launch give bit-identical results to each fit alone."""
+    F = importlib.import_module("2018_dictionary_tomography_amd.fit")
+    reqs = []
+    for seed in (1, 2, 3):
+        I, x, r, space, rng = _setup3(gt, 4 + seed, 7 + seed, 24 + 8 * seed, seed)
+        aspace = gt.AtomSpace(3, isotropic=False)
+        data = gt.ops.forward(gt.AtomElement(aspace, x, r, I, device=cuda), space)
+        start = gt.AtomElement(aspace, x + 0.01, r, I * 0.9, device=cuda)
+        reqs.append((start, space, data))
+    alone = [F.newton_fit(a, s, d, [3, 1])[0] for a, s, d in reqs]
+    mixed = [F.FitRequest(a, s, d, [3, 1]) for a, s, d in reqs]
+    table = torch.as_tensor(np.frombuffer(b"".join(m.slot_bytes() for m in mixed), dtype=np.uint8).copy(), device=cuda)
+    for _ in range(100):
+        F.run_slots(table, 3, 3, 5)
+        if all(s.done for s in F.read_slots(table, 3)):
+            break
+    for a, m in zip(alone, mixed):
+        assert torch.equal(a.x, m.x) and torch.equal(a.r, m.r) and torch.equal(a.I, m.I)
+        assert torch.equal(a.resid, m.resid)
+
+
+def test_gd_block_2d_matches_oracle(gt, oracle, cuda):
+    F = importlib.import_module("2018_dictionary_tomography_amd.fit")
+    W = importlib.import_module("2018_dictionary_tomography_amd.workloads")
+    rng = np.random.default_rng(21)
+    I, x, r = W.draw_iso2(4, rng, rho=(0.04, 0.08))
+    T, ortho, det = W.fan_2d(36, 64)
+    space = gt.ProjSpace(orientations=T, ortho=ortho, detector=det)
+    aspace = gt.AtomSpace(2, isotropic=True)
+    data = gt.ops.forward(gt.AtomElement(aspace, x, r, I, device=cuda), space)
+    x0 = x + rng.normal(0, 0.02, x.shape)
+    start = gt.AtomElement(aspace, x0, r, I, device=cuda)
+    _, Eg = F.gd_linesearch_block(start, data, [3, 1], space)
+    _, _, _, Eo = oracle.gd_linesearch_block(f32(I), f32(x0), f32(r), _oracle_geom(oracle, space),
+                                             data.double().cpu().numpy(), [3, 1])
+    assert np.all(np.diff(Eg) <= 0)
+    n = min(len(Eg), len(Eo))
+    assert np.abs(Eg[:n] - Eo[:n]).max() < 1e-4 * Eo[0]
