@@ -7,23 +7,24 @@
 //   1. C = data + R_j - R is never formed: the L2 derivatives of atom j
 //      against C equal the derivatives of 1/2||R - data||^2 w.r.t. atom j,
 //      so the pixel moments are taken against the stored residual R - data.
-//   2. (f, g, H) of atom j: per-angle FP32 pixel moments + FP64 contraction.
+//   2. (g, H) of atom j: per-angle FP32 pixel moments + FP64 contraction.
 //   3. dx = solve(H + I/h_j, -g) in FP64 with partial pivoting (numpy.linalg.
 //      solve, NewtonGaussian.py:73-76); singular -> the fit stops (done = 2).
 //   4. _step projections: I >= 0, rows of r with a negative diagonal negated.
 //   5. Energy change of the committed FP32 residual, accumulated in FP64 over
 //      the union of the old and new footprints; reject iff it is > 0 (or not
-//      finite), exactly the reference's E[jj] > E[jj-1] test on the
-//      full energies.
+//      finite), the reference's E[jj] > E[jj-1] test on the full energies.
 //   6. Accepted: atom written back, residual updated over the same pixels.
 //      Step size h_j x2 / /10 clamped to [1e-4, 1e4], convergence flags as in
 //      NewtonGaussian.py:78-98.
 //
-// Everything for one slot happens inside one workgroup (16 waves): no host
-// round trip and no inter-workgroup synchronisation.  Between the phases the
-// workgroup barrier orders LDS and global-memory traffic of the slot.  The
-// pixel phases give one (atom, angle) pair to each wave; the FP64 contraction
-// gives one pair to each thread and reduces with wave shuffles.
+// Everything for one slot happens inside one workgroup of 16 waves: no host
+// round trip and no inter-workgroup synchronisation; workgroup barriers order
+// the phases.  Pixel phases give one (atom, angle) pair to each wave with four
+// pixels in flight per lane; the FP64 contraction stages per-pair ingredients
+// in LDS (pair_prepare) and spreads the 10 + 55 gradient/Hessian entries over
+// the waves (fin_grad / fin_hess, one pair per lane); the solve runs on one
+// wave with one matrix row per lane.
 #include "gt_pixel.h"
 
 namespace gt {
@@ -31,19 +32,22 @@ namespace gt {
 constexpr int FIT_THREADS = 1024;
 constexpr int FIT_WAVES = FIT_THREADS / 64;
 constexpr int NAMAX = GT_FIT_MAX_ANGLES;
+constexpr int DETMAX = GT_FIT_MAX_DETECTOR;
+constexpr int FCH = 64;  // pairs per contraction chunk
 constexpr double FIT_EPS = 1e-4;  // NewtonGaussian.py:21
 
 template <int D> struct FitLDS {
     static constexpr int P = Dims<D>::P;
-    static constexpr int NV = P + P * (P + 1) / 2;
     PairRec rec_old[NAMAX];
     PairRec rec_new[NAMAX];
-    double wred[FIT_WAVES][NV];
+    PairFin<D> fin[FCH];
+    float p0[DETMAX], p1[DETMAX];
     double dE[FIT_WAVES];
     double th_old[10], th_new[10];
     double g[10], H[100];
+    double A[100], b[10], x[10];
     double dxnorm;
-    int accept, j, stop;
+    int accept, j, stop, solved;
 };
 
 // theta (FP64 values of the FP32 atom) -> PairGeom atom part
@@ -81,9 +85,9 @@ __device__ inline gt_geom slot_geom(const gt_fit_slot &S, int dim)
 // Union of two (possibly empty) boxes.
 __device__ inline void box_union(const PairRec &a, const PairRec &b, int K, int *lo, int *hi)
 {
+    const bool ea = a.lo[0] > a.hi[0] || (K == 2 && a.lo[1] > a.hi[1]);
+    const bool eb = b.lo[0] > b.hi[0] || (K == 2 && b.lo[1] > b.hi[1]);
     for (int k = 0; k < 2; k++) {
-        const bool ea = a.lo[0] > a.hi[0] || (K == 2 && a.lo[1] > a.hi[1]);
-        const bool eb = b.lo[0] > b.hi[0] || (K == 2 && b.lo[1] > b.hi[1]);
         if (ea && eb) {
             lo[k] = 1;
             hi[k] = 0;
@@ -110,9 +114,11 @@ __device__ inline float pair_value(const PairRec &rec, float p0v, float p1v)
 
 // Energy change (apply == 0) or residual update (apply == 1) of replacing
 // rec_old by rec_new over the union box; returns the lane's FP64 partial.
+// Both passes evaluate the same FP32 expressions, so the committed residual
+// is exactly the one whose energy was tested.
 template <int K>
-__device__ inline double pair_delta(const PairRec &ro, const PairRec &rn, const float *__restrict__ p0,
-                                    const float *__restrict__ p1, float *__restrict__ Ra, int n1, int lane, int apply)
+__device__ inline double pair_delta4(const PairRec &ro, const PairRec &rn, const float *p0, const float *p1,
+                                     float *__restrict__ Ra, int n1, int lane, int apply)
 {
     int lo[2], hi[2];
     box_union(ro, rn, K, lo, hi);
@@ -121,75 +127,129 @@ __device__ inline double pair_delta(const PairRec &ro, const PairRec &rn, const 
     const long total = (W > 0 && H > 0) ? (long)W * H : 0;
     double acc = 0.0;
     if (total == 0) return acc;
-    int r = lane / W, c = lane - (lane / W) * W;
-    const int dr = 64 / W, dc = 64 - dr * W;
-    for (long k = lane; k < total; k += 64) {
-        const int i0 = lo[0] + r;
-        const int i1 = (K == 2) ? lo[1] + c : 0;
-        const float pv0 = p0[i0];
-        const float pv1 = (K == 2) ? p1[i1] : 0.0f;
-        const float dR = pair_value(rn, pv0, pv1) - pair_value(ro, pv0, pv1);
-        if (dR != 0.0f) {
-            float *px = Ra + (size_t)i0 * n1 + i1;
-            const float old = *px;
-            const float nw = old + dR;
-            if (apply) {
-                *px = nw;
-            } else {
-                acc += 0.5 * ((double)nw - (double)old) * ((double)nw + (double)old);
+    Track4 tr;
+    tr.init(lane, W);
+    for (long k = lane; k < total; k += 256) {
+        float old[4], pv0[4], pv1[4];
+        size_t idx[4];
+        bool ok[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            ok[u] = k + 64 * u < total;
+            const int i0 = lo[0] + (ok[u] ? tr.r[u] : 0);
+            const int i1 = (K == 2) ? lo[1] + (ok[u] ? tr.c[u] : 0) : 0;
+            idx[u] = (size_t)i0 * n1 + i1;
+            old[u] = Ra[idx[u]];
+            pv0[u] = p0[i0];
+            pv1[u] = (K == 2) ? p1[i1] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const float dR = pair_value(rn, pv0[u], pv1[u]) - pair_value(ro, pv0[u], pv1[u]);
+            if (ok[u] && dR != 0.0f) {
+                const float nw = old[u] + dR;
+                if (apply)
+                    Ra[idx[u]] = nw;
+                else
+                    acc += 0.5 * ((double)nw - (double)old[u]) * ((double)nw + (double)old[u]);
             }
         }
-        c += dc;
-        r += dr;
-        if (c >= W) {
-            c -= W;
-            r++;
-        }
+        tr.step();
     }
     return acc;
 }
 
-// LU solve with partial pivoting, n <= 10; returns false if singular.
-__device__ inline bool lu_solve(double *A, double *b, int n)
+// Solve A x = b (n x n, row-major in LDS) with partial pivoting on one wave,
+// lane i holding row i.  Pivot = first row of largest magnitude (LAPACK
+// i*amax).  Returns false if a pivot is zero.
+template <int n> __device__ inline bool wave_solve(const double *A, const double *b, double *x, int lane)
 {
+    const bool own = lane < n;
+    double a[n], bb = own ? b[lane] : 0.0;
+#pragma unroll
+    for (int c = 0; c < n; c++) a[c] = own ? A[lane * n + c] : 0.0;
+    bool ok = true;
+#pragma unroll
     for (int k = 0; k < n; k++) {
-        int p = k;
-        double best = fabs(A[k * n + k]);
-        for (int i = k + 1; i < n; i++)
-            if (fabs(A[i * n + k]) > best) {
-                best = fabs(A[i * n + k]);
-                p = i;
+        double val = (own && lane >= k) ? fabs(a[k]) : -1.0;
+        int idx = lane;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ov = __shfl_xor(val, o);
+            const int oi = __shfl_xor(idx, o);
+            if (ov > val || (ov == val && oi < idx)) {
+                val = ov;
+                idx = oi;
             }
-        if (!(best > 0.0)) return false;
+        }
+        if (!(val > 0.0)) ok = false;
+        const int p = idx;
         if (p != k) {
+#pragma unroll
             for (int c = 0; c < n; c++) {
-                const double t = A[k * n + c];
-                A[k * n + c] = A[p * n + c];
-                A[p * n + c] = t;
+                const double ak = __shfl(a[c], k), ap = __shfl(a[c], p);
+                if (lane == k) a[c] = ap;
+                if (lane == p) a[c] = ak;
             }
-            const double t = b[k];
-            b[k] = b[p];
-            b[p] = t;
+            const double bk = __shfl(bb, k), bp = __shfl(bb, p);
+            if (lane == k) bb = bp;
+            if (lane == p) bb = bk;
         }
-        for (int i = k + 1; i < n; i++) {
-            const double f = A[i * n + k] / A[k * n + k];
-            for (int c = k; c < n; c++) A[i * n + c] -= f * A[k * n + c];
-            b[i] -= f * b[k];
+        double pk[n];
+#pragma unroll
+        for (int c = 0; c < n; c++) pk[c] = __shfl(a[c], k);
+        const double pb = __shfl(bb, k);
+        if (own && lane > k) {
+            const double f = a[k] / pk[k];
+#pragma unroll
+            for (int c = 0; c < n; c++)
+                if (c >= k) a[c] -= f * pk[c];
+            bb -= f * pb;
         }
     }
+    double xs[n];
+#pragma unroll
     for (int i = n - 1; i >= 0; i--) {
-        double s = b[i];
-        for (int c = i + 1; c < n; c++) s -= A[i * n + c] * b[c];
-        b[i] = s / A[i * n + i];
+        double s = bb;
+#pragma unroll
+        for (int c = 0; c < n; c++)
+            if (c > i) s -= a[c] * xs[c];
+        const double xi = (lane == i) ? s / a[i] : 0.0;
+        xs[i] = __shfl(xi, i);
     }
-    return true;
+    if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < n; i++) x[i] = xs[i];
+    return ok;
+}
+
+// Upper-triangle entry index e (0-based, row-major over i <= j) -> (i, j).
+__device__ inline void upper_ij(int e, int P, int &i, int &j)
+{
+    i = 0;
+    while (e >= P - i) {
+        e -= P - i;
+        i++;
+    }
+    j = i + e;
+}
+
+// Iso collapse: theta_iso index l -> aniso indices {l} (l <= D) or the D diagonal slots.
+template <int D> __device__ inline double collapse_H(const double *H, int P, int i, int l)
+{
+    double s = 0.0;
+    const int ni = (i <= D) ? 1 : D, nl = (l <= D) ? 1 : D;
+    for (int u = 0; u < ni; u++)
+        for (int v = 0; v < nl; v++) s += H[(i <= D ? i : 1 + D + u) * P + (l <= D ? l : 1 + D + v)];
+    return s;
 }
 
 template <int D>
 __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restrict__ slots, int nslots, int budget)
 {
     using Dm = Dims<D>;
-    constexpr int K = Dm::K, P = Dm::P, NR = Dm::NR, NM = Dm::NM, NV = FitLDS<D>::NV;
+    constexpr int K = Dm::K, P = Dm::P, NR = Dm::NR, NM = Dm::NM;
+    constexpr int NV = P + P * (P + 1) / 2;
     __shared__ FitLDS<D> L;
     const int sid = blockIdx.x;
     if (sid >= nslots) return;
@@ -204,6 +264,18 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
     const size_t slab = (size_t)S.n0 * S.n1;
     const float *det[2] = {S.p0, S.p1};
     const int nd[2] = {S.n0, S.n1};
+    for (int i = tid; i < S.n0; i += FIT_THREADS) L.p0[i] = S.p0[i];
+    if (K == 2)
+        for (int i = tid; i < S.n1; i += FIT_THREADS) L.p1[i] = S.p1[i];
+
+    long long ticks[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long long last = wall_clock64();
+#define GT_FIT_TICK(k)                             \
+    if (tid == 0 && S.phase_ticks) {               \
+        const long long now_ = wall_clock64();     \
+        ticks[k] += now_ - last;                   \
+        last = now_;                               \
+    }
 
     for (int step = 0; step < budget; step++) {
         // ---- choose the atom, load its parameters -----------------------
@@ -221,6 +293,16 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                 }
             }
         }
+        if (tid < NV) {
+            if (tid < P)
+                L.g[tid] = 0.0;
+            else {
+                int i, l;
+                upper_ij(tid - P, P, i, l);
+                L.H[i * P + l] = 0.0;
+                L.H[l * P + i] = 0.0;
+            }
+        }
         __syncthreads();
         if (L.stop) break;
         const int j = L.j;
@@ -232,121 +314,95 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
             pair_setup<D>(g, det, nd, S.sorted, L.rec_old[a]);
         }
         __syncthreads();
+        GT_FIT_TICK(0);
         // ---- phase B: pixel moments against the residual ------------------
         for (int a = wave; a < nA; a += FIT_WAVES) {
             float acc[MOMW];
 #pragma unroll
             for (int v = 0; v < MOMW; v++) acc[v] = 0.0f;
-            pair_moments<K, GT_MODE_RESID>(L.rec_old[a], S.p0, S.p1, S.resid + (size_t)a * slab, n1, lane, acc);
+            pair_moments4<K, GT_MODE_RESID>(L.rec_old[a], L.p0, L.p1, S.resid + (size_t)a * slab, n1, lane, acc);
             float mine = 0.0f;
 #pragma unroll
-            for (int v = 0; v < MOMW; v++) {
+            for (int v = 0; v < 30; v++) {
                 const float t = wave_sum_f32(acc[v]);
                 if (lane == v) mine = t;
             }
             if (lane < MOMW) S.scratch[(size_t)a * MOMW + lane] = mine;
         }
         __syncthreads();
+        GT_FIT_TICK(1);
         // ---- phase C: FP64 gradient and Hessian of atom j ----------------
-        {
-            double gl[P], hl[P * P];
-            for (int i = 0; i < P; i++) gl[i] = 0.0;
-            for (int i = 0; i < P * P; i++) hl[i] = 0.0;
-            for (int a = tid; a < nA; a += FIT_THREADS) {
+        for (int c0 = 0; c0 < nA; c0 += FCH) {
+            const int cnt = min(FCH, nA - c0);
+            if (tid < cnt) {
+                const int a = c0 + tid;
                 const PairRec &rec = L.rec_old[a];
                 const bool empty = rec.lo[0] > rec.hi[0] || (K == 2 && rec.lo[1] > rec.hi[1]);
-                if (empty) continue;
                 PairGeom<D> g;
                 atom_from_theta<D>(L.th_old, iso, g);
                 load_angle<D>(G, a, g);
                 double S1[NM], S2[NM];
                 const float *m = S.scratch + (size_t)a * MOMW;
                 for (int k = 0; k < NM; k++) {
-                    S1[k] = m[k];
-                    S2[k] = m[15 + k];
+                    S1[k] = empty ? 0.0 : (double)m[k];
+                    S2[k] = empty ? 0.0 : (double)m[15 + k];
                 }
                 const double pstar[2] = {(double)rec.c_hi[0] + (double)rec.c_lo[0],
                                          (double)rec.c_hi[1] + (double)rec.c_lo[1]};
-                pair_finish<D>(g, pstar, S1, S2, 1, 1, gl, hl);
+                pair_prepare<D>(g, pstar, S1, S2, 1, L.fin[tid]);
             }
-            int k = 0;
-            for (int i = 0; i < P; i++) {
-                const double v = wave_sum_f64(gl[i]);
-                if (lane == 0) L.wred[wave][k] = v;
-                k++;
-            }
-            for (int i = 0; i < P; i++)
-                for (int l = i; l < P; l++) {
-                    const double v = wave_sum_f64(hl[i * P + l]);
-                    if (lane == 0) L.wred[wave][k] = v;
-                    k++;
-                }
-        }
-        __syncthreads();
-        if (tid < NV) {
-            double s = 0.0;
-            for (int w = 0; w < FIT_WAVES; w++) s += L.wred[w][tid];
-            if (tid < P) {
-                L.g[tid] = s;
-            } else {
-                int k = P;
-                for (int i = 0; i < P; i++)
-                    for (int l = i; l < P; l++) {
-                        if (k == tid) {
-                            L.H[i * P + l] = s;
-                            L.H[l * P + i] = s;
-                        }
-                        k++;
+            __syncthreads();
+            for (int e = wave; e < NV; e += FIT_WAVES) {
+                int i = e, l = e;
+                if (e >= P) upper_ij(e - P, P, i, l);
+                double v = 0.0;
+                if (lane < cnt) v = (e < P) ? fin_grad<D>(L.fin[lane], e) : fin_hess<D>(L.fin[lane], i, l);
+                v = wave_sum_f64(v);
+                if (lane == 0) {
+                    if (e < P) {
+                        L.g[e] += v;
+                    } else {
+                        L.H[i * P + l] += v;
+                        if (l != i) L.H[l * P + i] += v;
                     }
+                }
             }
+            __syncthreads();
+        }
+        GT_FIT_TICK(2);
+        // ---- phase D: damped Newton system, solve, projections ------------
+        if (tid < PI * PI) {
+            const int i = tid / PI, l = tid - (tid / PI) * PI;
+            double v = iso ? collapse_H<D>(L.H, P, i, l) : L.H[i * P + l];
+            if (i == l) v += 1.0 / S.h[j];
+            L.A[tid] = v;
+        }
+        if (tid < PI) {
+            double gv;
+            if (iso && tid == 1 + D) {
+                gv = 0.0;
+                for (int u = 0; u < D; u++) gv += L.g[1 + D + u];
+            } else {
+                gv = L.g[tid];
+            }
+            L.b[tid] = -gv;
         }
         __syncthreads();
-        // ---- phase D: damped Newton step and projections (one thread) -----
+        if (wave == 0) {
+            bool ok;
+            if (PI == P)
+                ok = wave_solve<P>(L.A, L.b, L.x, lane);
+            else
+                ok = wave_solve<2 + D>(L.A, L.b, L.x, lane);
+            if (lane == 0) L.solved = ok;
+        }
+        __syncthreads();
         if (tid == 0) {
-            double A[100], b[10];
-            double gs[10];
-            if (iso) {
-                // collapse [I, x, r_0..r_{D-1}, offdiag] -> [I, x, rho]
-                double Hc[10][10];
-                for (int i = 0; i < PI; i++) {
-                    for (int l = 0; l < PI; l++) {
-                        double s = 0.0;
-                        for (int a2 = 0; a2 < P; a2++) {
-                            const double ea = (a2 <= D) ? (a2 == i ? 1.0 : 0.0)
-                                                        : ((i == 1 + D && a2 < 1 + 2 * D) ? 1.0 : 0.0);
-                            if (ea == 0.0) continue;
-                            for (int b2 = 0; b2 < P; b2++) {
-                                const double eb = (b2 <= D) ? (b2 == l ? 1.0 : 0.0)
-                                                            : ((l == 1 + D && b2 < 1 + 2 * D) ? 1.0 : 0.0);
-                                if (eb != 0.0) s += L.H[a2 * P + b2];
-                            }
-                        }
-                        Hc[i][l] = s;
-                    }
-                    double s = 0.0;
-                    for (int a2 = 0; a2 < P; a2++) {
-                        const double ea = (a2 <= D) ? (a2 == i ? 1.0 : 0.0)
-                                                    : ((i == 1 + D && a2 < 1 + 2 * D) ? 1.0 : 0.0);
-                        s += ea * L.g[a2];
-                    }
-                    gs[i] = s;
-                }
-                for (int i = 0; i < PI; i++)
-                    for (int l = 0; l < PI; l++) A[i * PI + l] = Hc[i][l];
-            } else {
-                for (int i = 0; i < P * P; i++) A[i] = L.H[i];
-                for (int i = 0; i < P; i++) gs[i] = L.g[i];
-            }
-            const double hj = S.h[j];
-            for (int i = 0; i < PI; i++) {
-                A[i * PI + i] += 1.0 / hj;
-                b[i] = -gs[i];
-            }
-            const bool ok = lu_solve(A, b, PI);
-            if (!ok) {
+            if (!L.solved) {
                 S.done = 2;
                 L.stop = 1;
             } else {
+                const double *b = L.x;
                 double n2 = 0.0;
                 for (int i = 0; i < PI; i++) n2 += b[i] * b[i];
                 L.dxnorm = sqrt(n2);
@@ -360,11 +416,21 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                     for (int k = 0; k < NR; k++) t[1 + D + k] = L.th_old[1 + D + k] + b[1 + D + k];
                     double *rr = t + 1 + D;
                     if (D == 2) {
-                        if (rr[0] < 0) { rr[0] = -rr[0]; rr[2] = -rr[2]; }
+                        if (rr[0] < 0) {
+                            rr[0] = -rr[0];
+                            rr[2] = -rr[2];
+                        }
                         if (rr[1] < 0) rr[1] = -rr[1];
                     } else {
-                        if (rr[0] < 0) { rr[0] = -rr[0]; rr[3] = -rr[3]; rr[5] = -rr[5]; }
-                        if (rr[1] < 0) { rr[1] = -rr[1]; rr[4] = -rr[4]; }
+                        if (rr[0] < 0) {
+                            rr[0] = -rr[0];
+                            rr[3] = -rr[3];
+                            rr[5] = -rr[5];
+                        }
+                        if (rr[1] < 0) {
+                            rr[1] = -rr[1];
+                            rr[4] = -rr[4];
+                        }
                         if (rr[2] < 0) rr[2] = -rr[2];
                     }
                 }
@@ -372,6 +438,7 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
             }
         }
         __syncthreads();
+        GT_FIT_TICK(3);
         if (L.stop) break;
         // ---- phase E: detector Gaussians of the proposal -------------------
         for (int a = tid; a < nA; a += FIT_THREADS) {
@@ -381,15 +448,17 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
             pair_setup<D>(g, det, nd, S.sorted, L.rec_new[a]);
         }
         __syncthreads();
+        GT_FIT_TICK(4);
         // ---- phase F: FP64 energy change ---------------------------------
         {
             double acc = 0.0;
             for (int a = wave; a < nA; a += FIT_WAVES)
-                acc += pair_delta<K>(L.rec_old[a], L.rec_new[a], S.p0, S.p1, S.resid + (size_t)a * slab, n1, lane, 0);
+                acc += pair_delta4<K>(L.rec_old[a], L.rec_new[a], L.p0, L.p1, S.resid + (size_t)a * slab, n1, lane, 0);
             acc = wave_sum_f64(acc);
             if (lane == 0) L.dE[wave] = acc;
         }
         __syncthreads();
+        GT_FIT_TICK(5);
         // ---- phase G: accept / reject, step size, flags (one thread) -------
         if (tid == 0) {
             double dE = 0.0;
@@ -421,10 +490,11 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
             L.accept = accept;
         }
         __syncthreads();
+        GT_FIT_TICK(6);
         // ---- phase H: commit the residual ---------------------------------
         if (L.accept) {
             for (int a = wave; a < nA; a += FIT_WAVES)
-                pair_delta<K>(L.rec_old[a], L.rec_new[a], S.p0, S.p1, S.resid + (size_t)a * slab, n1, lane, 1);
+                pair_delta4<K>(L.rec_old[a], L.rec_new[a], L.p0, L.p1, S.resid + (size_t)a * slab, n1, lane, 1);
         }
         // ---- loop control (NewtonGaussian.py:50-98) ------------------------
         if (tid == 0) {
@@ -446,7 +516,11 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
             }
         }
         __syncthreads();
+        GT_FIT_TICK(7);
     }
+#undef GT_FIT_TICK
+    if (tid == 0 && S.phase_ticks)
+        for (int k = 0; k < 8; k++) S.phase_ticks[k] += ticks[k];
 }
 
 }  // namespace gt

@@ -474,4 +474,186 @@ __host__ __device__ inline void pair_finish(const PairGeom<D> &g, const double *
     }
 }
 
+// The same contraction split in two, for callers that spread one pair's
+// (gradient, Hessian) entries over many threads: pair_prepare() stores every
+// per-pair ingredient (Jacobian J of gamma, the gamma-space matrix Kg, the
+// weights v and the pieces of the Hessian of Phi = v . gamma(theta)) in a
+// PairFin, typically in LDS; fin_grad()/fin_hess() then produce one entry each.
+// The algebra is identical to pair_finish().
+template <int D> struct PairFin {
+    static constexpr int P = Dims<D>::P, NG = Dims<D>::NG;
+    double J[NG][P];
+    double Kg[NG][NG];
+    double v[NG];
+    double va, I, s, rs3, rs5, VKK;
+    double t[3], T[3], z[3], qt[3], uq[3];
+    double MUc[3][3], Hh[3][3], Uh[3][3], Uu[3][3], Q[3][3];
+};
+
+template <int D>
+__host__ __device__ inline void pair_prepare(const PairGeom<D> &g, const double *pstar, const double *S1,
+                                             const double *S2, int gn, PairFin<D> &F)
+{
+    using Dm = Dims<D>;
+    constexpr int K = Dm::K, NB = Dm::NB, NG = Dm::NG, P = Dm::P, NR = Dm::NR;
+    PairAlg<D> a;
+    pair_alg<D>(g, pstar, a);
+    const double al = a.alpha, s = a.s;
+    for (int p = 0; p < NB; p++) {
+        int x0, x1, ea0, ea1, eb0, eb1;
+        bpair<K>(p, x0, x1);
+        dexp<K>(x0, ea0, ea1);
+        dexp<K>(x1, eb0, eb1);
+        const double fp = (x0 == x1) ? 1.0 : 2.0;
+        const int m = mono_index<K>(ea0 + eb0, ea1 + eb1);
+        F.v[1 + p] = -30.0 * al * fp * S1[m];
+        const double val = -30.0 * fp * (S1[m] + (gn ? al * S2[m] : 0.0));
+        F.Kg[0][1 + p] = val;
+        F.Kg[1 + p][0] = val;
+        for (int q = p; q < NB; q++) {
+            int y0, y1, fa0, fa1, fb0, fb1;
+            bpair<K>(q, y0, y1);
+            dexp<K>(y0, fa0, fa1);
+            dexp<K>(y1, fb0, fb1);
+            const double fq = (y0 == y1) ? 1.0 : 2.0;
+            const int mm = mono_index<K>(ea0 + eb0 + fa0 + fb0, ea1 + eb1 + fa1 + fb1);
+            const double w = 900.0 * fp * fq * (al * S1[mm] + (gn ? al * al * S2[mm] : 0.0));
+            F.Kg[1 + p][1 + q] = w;
+            F.Kg[1 + q][1 + p] = w;
+        }
+    }
+    F.v[0] = S1[0];
+    F.Kg[0][0] = gn ? S2[0] : 0.0;
+
+    const double rs = 1.0 / sqrt(s), rs3 = rs / s;
+    for (int i = 0; i < NG; i++)
+        for (int j = 0; j < P; j++) F.J[i][j] = 0.0;
+    F.J[0][0] = rs;
+    for (int k = 0; k < NR; k++) F.J[0][1 + D + k] = -g.I * rs3 * a.t[slot_row(D, k)] * g.T[slot_col(D, k)];
+    for (int p = 0; p < NB; p++) {
+        int x0, x1;
+        bpair<K>(p, x0, x1);
+        for (int c = 0; c < D; c++) {
+            double val = 0.0;
+            if (x0 == K) {
+                double acc = 0.0;
+                for (int i = 0; i <= c; i++) acc += g.U[i][c] * a.ut[x1][i];
+                val -= acc;
+            }
+            if (x1 == K) {
+                double acc = 0.0;
+                for (int i = 0; i <= c; i++) acc += g.U[i][c] * a.ut[x0][i];
+                val -= acc;
+            }
+            F.J[1 + p][1 + c] = val;
+        }
+        for (int k = 0; k < NR; k++) {
+            const int i = slot_row(D, k), j = slot_col(D, k);
+            F.J[1 + p][1 + D + k] = a.ut[x0][i] * a.ht[x1][j] + a.ut[x1][i] * a.ht[x0][j];
+        }
+    }
+
+    double V[3][3];
+    for (int p = 0; p < NB; p++) {
+        int x0, x1;
+        bpair<K>(p, x0, x1);
+        const double w = (x0 == x1) ? F.v[1 + p] : 0.5 * F.v[1 + p];
+        V[x0][x1] = w;
+        V[x1][x0] = w;
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double hh = 0.0, uh = 0.0, uu = 0.0;
+            if (i < D && j < D)
+                for (int b0 = 0; b0 <= K; b0++)
+                    for (int b1 = 0; b1 <= K; b1++) {
+                        hh += V[b0][b1] * a.ht[b0][i] * a.ht[b1][j];
+                        uh += V[b0][b1] * a.ut[b0][i] * a.ht[b1][j];
+                        uu += V[b0][b1] * a.ut[b0][i] * a.ut[b1][j];
+                    }
+            F.Hh[i][j] = hh;
+            F.Uh[i][j] = uh;
+            F.Uu[i][j] = uu;
+        }
+    F.va = F.v[0];
+    F.I = g.I;
+    F.s = s;
+    F.rs3 = rs3;
+    F.rs5 = rs3 / s;
+    F.VKK = V[K][K];
+    for (int c = 0; c < 3; c++) {
+        double acc = 0.0;
+        for (int i = 0; i <= c && c < D; i++) acc += g.U[i][c] * a.t[i];
+        F.z[c] = acc;
+        F.t[c] = a.t[c];
+        F.T[c] = g.T[c];
+    }
+    for (int c = 0; c < 3; c++)
+        for (int i = 0; i < 3; i++)
+            F.MUc[c][i] = (c < D && i < D) ? ((i <= c ? g.U[i][c] : 0.0) - F.z[c] / s * a.t[i]) : 0.0;
+    for (int c = 0; c < 3; c++)
+        for (int d = 0; d < 3; d++) {
+            double q = 0.0;
+            for (int i = 0; i <= c && c < D; i++) q += g.U[i][c] * F.MUc[d][i];
+            F.Q[c][d] = q;
+        }
+    for (int i = 0; i < 3; i++) {
+        double a1 = 0.0, a2 = 0.0;
+        if (i < D)
+            for (int b = 0; b <= K; b++) {
+                a1 += V[b][K] * a.ht[b][i];
+                a2 += V[b][K] * a.ut[b][i];
+            }
+        F.qt[i] = a1;
+        F.uq[i] = a2;
+    }
+}
+
+// Gradient entry i of one pair: (J^T v)_i.
+template <int D> __host__ __device__ inline double fin_grad(const PairFin<D> &F, int i)
+{
+    double acc = 0.0;
+    for (int l = 0; l < Dims<D>::NG; l++) acc += F.J[l][i] * F.v[l];
+    return acc;
+}
+
+// Hessian entry (i, j) of one pair: (J^T Kg J)_ij + d2 Phi / d theta_i d theta_j.
+template <int D> __host__ __device__ inline double fin_hess(const PairFin<D> &F, int i, int j)
+{
+    constexpr int NG = Dims<D>::NG;
+    double acc = 0.0;
+    for (int l = 0; l < NG; l++) {
+        double kj = 0.0;
+        for (int m = 0; m < NG; m++) kj += F.Kg[l][m] * F.J[m][j];
+        acc += F.J[l][i] * kj;
+    }
+    if (i > j) {
+        const int t = i;
+        i = j;
+        j = t;
+    }
+    const int r0 = 1 + D;  // first r slot
+    if (i == 0) {
+        if (j >= r0) acc += -F.va * F.rs3 * F.t[slot_row(D, j - r0)] * F.T[slot_col(D, j - r0)];
+    } else if (i < r0) {
+        const int c = i - 1;
+        if (j < r0) {
+            acc += 2.0 * F.VKK * F.Q[c][j - 1];
+        } else {
+            const int ii = slot_row(D, j - r0), jj = slot_col(D, j - r0);
+            const double ec = (jj == c ? 1.0 : 0.0) - F.z[c] / F.s * F.T[jj];
+            acc += -2.0 * (F.MUc[c][ii] * F.qt[jj] + F.uq[ii] * ec);
+        }
+    } else {
+        const int i1 = slot_row(D, i - r0), j1 = slot_col(D, i - r0);
+        const int i2 = slot_row(D, j - r0), j2 = slot_col(D, j - r0);
+        const double dik = (i1 == i2) ? 1.0 : 0.0;
+        acc += 2.0 * (dik * F.Hh[j2][j1] - (F.t[i1] * F.t[i2] * F.Hh[j2][j1] + F.t[i1] * F.T[j2] * F.Uh[i2][j1] +
+                                          F.T[j1] * F.t[i2] * F.Uh[i1][j2] + F.T[j1] * F.T[j2] * F.Uu[i1][i2]) /
+                                             F.s);
+        acc += F.va * F.I * (3.0 * F.rs5 * F.t[i1] * F.T[j1] * F.t[i2] * F.T[j2] - F.rs3 * dik * F.T[j1] * F.T[j2]);
+    }
+    return acc;
+}
+
 }  // namespace gt

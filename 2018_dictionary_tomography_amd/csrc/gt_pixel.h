@@ -77,25 +77,103 @@ __device__ inline void pair_moments(const PairRec &rec, const float *__restrict_
     }
 }
 
-// Sum acc[32] over the 64 lanes of the calling wave through an LDS tile
-// red[64][33] owned by this wave; lanes 0..31 return their column's total.
-// The caller's wave must not touch red[] concurrently; callers place a
-// workgroup barrier or wave barrier around it as described at call sites.
-__device__ inline float wave_sum32(float (*red)[MOMW + 1], const float *acc, int lane)
-{
+// Four pixel trackers of a wave over a W-wide box: tracker u visits the
+// linear indices lane + 64 u + 256 t, t = 0, 1, ...
+struct Track4 {
+    int r[4], c[4];
+    int dr, dc, W;
+    __device__ inline void init(int lane, int W_)
+    {
+        W = W_;
 #pragma unroll
-    for (int v = 0; v < MOMW; v++) red[lane][v] = acc[v];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const int col = lane & 31, half = lane >> 5;
-    float s = 0.0f;
-    for (int i = 0; i < 32; i++) s += red[half * 32 + i][col];
-    s += __shfl_xor(s, 32);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    return s;
+        for (int u = 0; u < 4; u++) {
+            const int k = lane + 64 * u;
+            r[u] = k / W;
+            c[u] = k - r[u] * W;
+        }
+        dr = 256 / W;
+        dc = 256 - dr * W;
+    }
+    __device__ inline void step()
+    {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            c[u] += dc;
+            r[u] += dr;
+            if (c[u] >= W) {
+                c[u] -= W;
+                r[u]++;
+            }
+        }
+    }
+};
+
+// pair_moments with four independent pixels per lane per iteration: the
+// four loads of C are issued before any of them is used, so one wave keeps
+// four memory requests in flight (the fitter runs few waves per CU).
+template <int K, int MODE>
+__device__ inline void pair_moments4(const PairRec &rec, const float *p0, const float *p1,
+                                     const float *__restrict__ Ca, int n1, int lane, float *acc)
+{
+    const int W = (K == 2) ? rec.hi[1] - rec.lo[1] + 1 : 1;
+    const int H = rec.hi[0] - rec.lo[0] + 1;
+    const long total = (W > 0 && H > 0) ? (long)W * H : 0;
+    if (total == 0) return;
+    Track4 tr;
+    tr.init(lane, W);
+    const float kap = (float)KAPPA;
+    for (long k = lane; k < total; k += 256) {
+        float cv[4], dp0[4], dp1[4];
+        bool ok[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            ok[u] = k + 64 * u < total;
+            const int i0 = rec.lo[0] + (ok[u] ? tr.r[u] : 0);
+            const int i1 = (K == 2) ? rec.lo[1] + (ok[u] ? tr.c[u] : 0) : 0;
+            cv[u] = Ca[(size_t)i0 * n1 + i1];
+            dp0[u] = (p0[i0] - rec.c_hi[0]) - rec.c_lo[0];
+            dp1[u] = (K == 2) ? (p1[i1] - rec.c_hi[1]) - rec.c_lo[1] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const float q = pair_q(rec.q, dp0[u], dp1[u]);
+            if (ok[u] && q >= kap) {
+                const float E = __builtin_amdgcn_exp2f(q);
+                float w1;
+                if (MODE == GT_MODE_L2) {
+                    const float res = fmaf(rec.alpha, E, -cv[u]);
+                    w1 = res * E;
+                    acc[30] = fmaf(res, res, acc[30]);
+                    acc[31] = fmaf(cv[u], cv[u], acc[31]);
+                } else {
+                    w1 = cv[u] * E;
+                }
+                const float w2 = E * E;
+                if (K == 2) {
+                    const float x1 = dp0[u], y1 = dp1[u];
+                    const float x2 = x1 * x1, xy = x1 * y1, y2 = y1 * y1;
+                    const float mono[15] = {1.0f,    x1,      y1,      x2,      xy,      y2,      x2 * x1, x2 * y1,
+                                            x1 * y2, y2 * y1, x2 * x2, x2 * xy, x2 * y2, xy * y2, y2 * y2};
+#pragma unroll
+                    for (int m = 0; m < 15; m++) acc[m] = fmaf(w1, mono[m], acc[m]);
+                    if (MODE != GT_MODE_LINEAR) {
+#pragma unroll
+                        for (int m = 0; m < 15; m++) acc[15 + m] = fmaf(w2, mono[m], acc[15 + m]);
+                    }
+                } else {
+                    const float x1 = dp0[u], x2 = x1 * x1;
+                    const float mono[5] = {1.0f, x1, x2, x2 * x1, x2 * x2};
+#pragma unroll
+                    for (int m = 0; m < 5; m++) acc[m] = fmaf(w1, mono[m], acc[m]);
+                    if (MODE != GT_MODE_LINEAR) {
+#pragma unroll
+                        for (int m = 0; m < 5; m++) acc[15 + m] = fmaf(w2, mono[m], acc[15 + m]);
+                    }
+                }
+            }
+        }
+        tr.step();
+    }
 }
 
 __device__ inline float wave_sum_f32(float v)

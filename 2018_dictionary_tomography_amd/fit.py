@@ -35,11 +35,13 @@ class gt_fit_slot(ctypes.Structure):
         ("it", ctypes.c_int), ("pos", ctypes.c_int), ("inner", ctypes.c_int), ("jj", ctypes.c_int),
         ("brk0", ctypes.c_int), ("brk1", ctypes.c_int), ("done", ctypes.c_int), ("pad1", ctypes.c_int),
         ("E", ctypes.c_double), ("updates", ctypes.c_longlong), ("accepted", ctypes.c_longlong),
+        ("phase_ticks", ctypes.c_void_p),
     ]
 
 
 SLOT_BYTES = ctypes.sizeof(gt_fit_slot)
 MAX_ANGLES = 128
+MAX_DETECTOR = 1024
 
 
 def _declare():
@@ -70,6 +72,8 @@ class FitRequest:
         self.geom = ops.device_geometry(space, dev)
         if self.geom.nA > MAX_ANGLES:
             raise ValueError(f"the fitter supports at most {MAX_ANGLES} angles, got {self.geom.nA}")
+        if max(self.geom.n0, self.geom.n1) > MAX_DETECTOR:
+            raise ValueError(f"the fitter supports detectors up to {MAX_DETECTOR} pixels per axis")
         self.dim = self.geom.dim
         self.I = atoms.I.detach().to(dev, torch.float32).clone().contiguous()
         self.x = atoms.x.detach().to(dev, torch.float32).clone().contiguous()

@@ -165,10 +165,12 @@ typedef struct gt_fit_slot {
     double E;                /* current energy 1/2 ||resid||^2 */
     long long updates;       /* atom updates performed */
     long long accepted;      /* accepted updates */
+    long long *phase_ticks;  /* [8] optional: wall-clock ticks (100 MHz) spent per phase, or NULL */
 } gt_fit_slot;
 
-/* nA must be <= GT_FIT_MAX_ANGLES. dim 2 or 3. */
+/* nA must be <= GT_FIT_MAX_ANGLES and n0, n1 <= GT_FIT_MAX_DETECTOR. dim 2 or 3. */
 #define GT_FIT_MAX_ANGLES 128
+#define GT_FIT_MAX_DETECTOR 1024
 int gt_fit_run(gt_fit_slot *slots, int nslots, int dim, int budget, void *stream);
 
 #ifdef __cplusplus

@@ -1,0 +1,79 @@
+"""Fourier-space evaluation (2000 anisotropic atoms drawn as the 3D workload,
+FT convention U = L^T, on the integer grid [-64, 64)^3) against
+  * the FP64 oracle at 30000 random grid points: 1e-5 relative to max |F|;
+  * an FFT of the real-space volume rendered at 256^3 on [-1, 1]^3 (spacing
+    1/128, so integer k are the even FFT indices): loose 2e-3 relative to
+    max |F|, limited by the truncation of atoms whose tails cross the box.
+"""
+import importlib
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import f32
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c4():
+    W = importlib.import_module("2018_dictionary_tomography_amd.workloads")
+    rng = np.random.default_rng(44)
+    I, x, r, L = W.draw_aniso3(2000, rng)
+    # FT convention: f = I exp(-|U (y - x)|^2 / 2) with U = L^T  ->  slots = sqrt(60) * projector slots
+    rU = r * np.sqrt(60.0)
+    return I, x, rU
+
+
+def test_fourier_grid_vs_fp64(gt, oracle, cuda, c4):
+    Fo = importlib.import_module("2018_dictionary_tomography_amd.fourier")
+    I, x, rU = c4
+    nI, nx, nr = Fo.analytic_gauss_fourier(I, x, rU)
+    k = [np.arange(-64, 64, dtype=np.float64)] * 3
+    out = Fo.evaluate(Fo.FTAtoms(nI, nx, nr, cuda), k).cpu().numpy()
+    assert out.shape == (128, 128, 128)
+    rng = np.random.default_rng(0)
+    idx = rng.integers(0, 128, (30000, 3))
+    idx[:10] = [[64, 64, 64], [65, 64, 64], [64, 70, 60], [0, 0, 0], [127, 127, 127], [64, 64, 90], [70, 71, 72],
+                [60, 64, 64], [64, 50, 64], [30, 64, 64]]
+    K = np.stack([k[0][idx[:, 0]], k[1][idx[:, 1]], k[2][idx[:, 2]]], 1)
+    ref = oracle.ft_points(f32(nI), f32(nx), f32(nr), K)
+    got = out[idx[:, 0], idx[:, 1], idx[:, 2]]
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() < 1e-5 * scale
+    # k = 0 is the total mass sum(newI)
+    assert abs(out[64, 64, 64] - np.sum(f32(nI))) < 1e-6 * np.sum(nI)
+
+
+def test_fourier_iso_vs_fp64(gt, oracle, cuda):
+    Fo = importlib.import_module("2018_dictionary_tomography_amd.fourier")
+    rng = np.random.default_rng(5)
+    nI = rng.uniform(0.5, 1.5, 300)
+    nx = rng.uniform(-4, 4, (300, 3))
+    nr = rng.uniform(2, 6, (300, 1))
+    k = [np.linspace(-20, 20, 33), np.linspace(-10, 12, 17), np.linspace(-15, 15, 70)]
+    out = Fo.evaluate(Fo.FTAtoms(nI, nx, nr, cuda), k).cpu().numpy()
+    mesh = np.stack(np.meshgrid(*[f32(v) for v in k], indexing="ij"), -1).reshape(-1, 3)
+    ref = oracle.ft_points(f32(nI), f32(nx), f32(nr), mesh, iso=True).reshape(out.shape)
+    assert np.abs(out - ref).max() < 1e-5 * np.abs(ref).max()
+
+
+def test_fourier_vs_fft_of_render(gt, cuda, c4):
+    Fo = importlib.import_module("2018_dictionary_tomography_amd.fourier")
+    I, x, rU = c4
+    n, h = 256, 2.0 / 256
+    y = (np.arange(n) + 0.5) * h - 1.0
+    # projector convention for the render: U_p = U / sqrt(60), I_p sqrt(30/pi) = I
+    atoms = gt.AtomElement(gt.AtomSpace(3, isotropic=False), x, rU / np.sqrt(60.0), I / np.sqrt(30 / np.pi),
+                           device=cuda)
+    u = gt.ops.volume(atoms, gt.VolSpace([y, y, y]))
+    F = torch.fft.fftn(u.double())
+    m = np.arange(-64, 64)
+    sel = torch.as_tensor((2 * m) % n, device=cuda)
+    Fk = F[sel][:, sel][:, :, sel]
+    ph = np.exp(-2j * np.pi * m * y[0])  # e^{-2 pi i k y_0} per axis
+    Fk = Fk.cpu().numpy() * h ** 3 * ph[:, None, None] * ph[None, :, None] * ph[None, None, :]
+    nI, nx, nr = Fo.analytic_gauss_fourier(I, x, rU)
+    ana = Fo.evaluate(Fo.FTAtoms(nI, nx, nr, cuda), [m.astype(float)] * 3).cpu().numpy()
+    assert np.abs(Fk - ana).max() < 2e-3 * np.abs(ana).max()
