@@ -41,6 +41,7 @@ template <int D> struct FitLDS {
     PairRec rec_old[NAMAX];
     PairRec rec_new[NAMAX];
     PairFin<D> fin[FCH];
+    PairAlg<D> alg[FCH];
     float p0[DETMAX], p1[DETMAX];
     double dE[FIT_WAVES];
     double th_old[10], th_new[10];
@@ -127,6 +128,54 @@ __device__ inline double pair_delta4(const PairRec &ro, const PairRec &rn, const
     const long total = (W > 0 && H > 0) ? (long)W * H : 0;
     double acc = 0.0;
     if (total == 0) return acc;
+    if (K == 2) {
+        // column-owning walk (ColChunk): per-column terms of both Gaussians once per lane
+        const float kap = (float)KAPPA;
+        for (int cb = 0; cb < W; cb += 64) {
+            ColChunk cc;
+            cc.init(lane, W, cb);
+            if (!cc.act) continue;
+            const int i1 = lo[1] + cb + cc.c;
+            const float pv1 = p1[i1];
+            const float d1o = (pv1 - ro.c_hi[1]) - ro.c_lo[1], d1n = (pv1 - rn.c_hi[1]) - rn.c_lo[1];
+            const float rlo = fmaf(ro.q[1], d1o, ro.q[3]), rln = fmaf(rn.q[1], d1n, rn.q[3]);
+            const float clo = fmaf(fmaf(ro.q[2], d1o, ro.q[4]), d1o, ro.q[5]);
+            const float cln = fmaf(fmaf(rn.q[2], d1n, rn.q[4]), d1n, rn.q[5]);
+            float *col = Ra + i1;
+            for (int r0 = cc.ro; r0 < H; r0 += 4 * cc.R) {
+                float old[4], pv0[4];
+                bool ok[4];
+                size_t off[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int r = r0 + u * cc.R;
+                    ok[u] = r < H;
+                    const int i0 = lo[0] + (ok[u] ? r : 0);
+                    off[u] = (size_t)i0 * n1;
+                    old[u] = col[off[u]];
+                    pv0[u] = p0[i0];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const float d0o = (pv0[u] - ro.c_hi[0]) - ro.c_lo[0];
+                    const float d0n = (pv0[u] - rn.c_hi[0]) - rn.c_lo[0];
+                    const float qo = fmaf(fmaf(ro.q[0], d0o, rlo), d0o, clo);
+                    const float qn = fmaf(fmaf(rn.q[0], d0n, rln), d0n, cln);
+                    const float vo = (qo >= kap) ? ro.alpha * __builtin_amdgcn_exp2f(qo) : 0.0f;
+                    const float vn = (qn >= kap) ? rn.alpha * __builtin_amdgcn_exp2f(qn) : 0.0f;
+                    const float dR = vn - vo;
+                    if (ok[u] && dR != 0.0f) {
+                        const float nw = old[u] + dR;
+                        if (apply)
+                            col[off[u]] = nw;
+                        else
+                            acc += 0.5 * ((double)nw - (double)old[u]) * ((double)nw + (double)old[u]);
+                    }
+                }
+            }
+        }
+        return acc;
+    }
     Track4 tr;
     tr.init(lane, W);
     for (long k = lane; k < total; k += 256) {
@@ -277,7 +326,8 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         last = now_;                               \
     }
 
-    for (int step = 0; step < budget; step++) {
+    const int my_budget = S.quota > 0 ? min(budget, S.quota) : budget;
+    for (int step = 0; step < my_budget; step++) {
         // ---- choose the atom, load its parameters -----------------------
         if (tid == 0) {
             L.stop = S.done != 0 || S.it >= S.max_outer || S.pos >= N;
@@ -320,7 +370,10 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
             float acc[MOMW];
 #pragma unroll
             for (int v = 0; v < MOMW; v++) acc[v] = 0.0f;
-            pair_moments4<K, GT_MODE_RESID>(L.rec_old[a], L.p0, L.p1, S.resid + (size_t)a * slab, n1, lane, acc);
+            if (K == 2)
+                pair_moments_rows<GT_MODE_RESID>(L.rec_old[a], L.p0, L.p1, S.resid + (size_t)a * slab, n1, lane, acc);
+            else
+                pair_moments4<K, GT_MODE_RESID>(L.rec_old[a], L.p0, L.p1, S.resid + (size_t)a * slab, n1, lane, acc);
             float mine = 0.0f;
 #pragma unroll
             for (int v = 0; v < 30; v++) {
@@ -334,8 +387,11 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         // ---- phase C: FP64 gradient and Hessian of atom j ----------------
         for (int c0 = 0; c0 < nA; c0 += FCH) {
             const int cnt = min(FCH, nA - c0);
-            if (tid < cnt) {
-                const int a = c0 + tid;
+            // pair k of the chunk runs on lane k / 16 of wave k % 16: the FP64
+            // prepare work is spread over all SIMDs instead of one wave
+            const int k = lane * FIT_WAVES + wave;
+            if (k < cnt) {
+                const int a = c0 + k;
                 const PairRec &rec = L.rec_old[a];
                 const bool empty = rec.lo[0] > rec.hi[0] || (K == 2 && rec.lo[1] > rec.hi[1]);
                 PairGeom<D> g;
@@ -343,13 +399,13 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                 load_angle<D>(G, a, g);
                 double S1[NM], S2[NM];
                 const float *m = S.scratch + (size_t)a * MOMW;
-                for (int k = 0; k < NM; k++) {
-                    S1[k] = empty ? 0.0 : (double)m[k];
-                    S2[k] = empty ? 0.0 : (double)m[15 + k];
+                for (int v = 0; v < NM; v++) {
+                    S1[v] = empty ? 0.0 : (double)m[v];
+                    S2[v] = empty ? 0.0 : (double)m[15 + v];
                 }
                 const double pstar[2] = {(double)rec.c_hi[0] + (double)rec.c_lo[0],
                                          (double)rec.c_hi[1] + (double)rec.c_lo[1]};
-                pair_prepare<D>(g, pstar, S1, S2, 1, L.fin[tid]);
+                pair_prepare<D>(g, pstar, S1, S2, 1, L.fin[k], L.alg[k]);
             }
             __syncthreads();
             for (int e = wave; e < NV; e += FIT_WAVES) {

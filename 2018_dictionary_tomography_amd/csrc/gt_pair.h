@@ -488,15 +488,19 @@ template <int D> struct PairFin {
     double va, I, s, rs3, rs5, VKK;
     double t[3], T[3], z[3], qt[3], uq[3];
     double MUc[3][3], Hh[3][3], Uh[3][3], Uu[3][3], Q[3][3];
+    double pad;  // odd number of doubles: lanes reading consecutive PairFins spread over LDS banks
 };
+static_assert((sizeof(PairFin<3>) / 8) % 2 == 1 && (sizeof(PairFin<2>) / 8) % 2 == 1,
+              "PairFin must hold an odd number of doubles");
 
+// `a` is caller-provided scratch (e.g. LDS) so that a thread does not have to
+// keep the whole PairAlg in registers.
 template <int D>
 __host__ __device__ inline void pair_prepare(const PairGeom<D> &g, const double *pstar, const double *S1,
-                                             const double *S2, int gn, PairFin<D> &F)
+                                             const double *S2, int gn, PairFin<D> &F, PairAlg<D> &a)
 {
     using Dm = Dims<D>;
     constexpr int K = Dm::K, NB = Dm::NB, NG = Dm::NG, P = Dm::P, NR = Dm::NR;
-    PairAlg<D> a;
     pair_alg<D>(g, pstar, a);
     const double al = a.alpha, s = a.s;
     for (int p = 0; p < NB; p++) {
@@ -607,6 +611,14 @@ __host__ __device__ inline void pair_prepare(const PairGeom<D> &g, const double 
         F.qt[i] = a1;
         F.uq[i] = a2;
     }
+}
+
+template <int D>
+__host__ __device__ inline void pair_prepare(const PairGeom<D> &g, const double *pstar, const double *S1,
+                                             const double *S2, int gn, PairFin<D> &F)
+{
+    PairAlg<D> a;
+    pair_prepare<D>(g, pstar, S1, S2, gn, F, a);
 }
 
 // Gradient entry i of one pair: (J^T v)_i.

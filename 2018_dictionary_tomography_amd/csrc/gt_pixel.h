@@ -176,6 +176,95 @@ __device__ inline void pair_moments4(const PairRec &rec, const float *p0, const 
     }
 }
 
+// Column-owning layout of a box of width W for one wave: for W <= 64 the wave
+// covers R = 64 / W rows at a time (lane = ro * W + c); wider boxes are walked
+// in 64-column chunks with R = 1.  A lane keeps its column for a whole chunk,
+// so every per-column quantity is computed once and the row loop only sees
+// terms in dp0.  Consecutive lanes read consecutive columns (coalesced).
+struct ColChunk {
+    int c, ro, R, Wc;
+    bool act;
+    __device__ inline void init(int lane, int W, int cb)
+    {
+        Wc = min(64, W - cb);
+        R = 64 / Wc;
+        ro = lane / Wc;
+        c = lane - ro * Wc;
+        act = ro < R;
+    }
+};
+
+// Moments of one 2D-detector pair, separable form: per lane and column,
+// A_i = sum_rows w1 dp0^i and B_i = sum_rows E^2 dp0^i (i <= 4); then
+// S1[i, j] += A_i dp1^j, S2[i, j] += B_i dp1^j.  Same sums as pair_moments.
+template <int MODE>
+__device__ inline void pair_moments_rows(const PairRec &rec, const float *p0, const float *p1,
+                                         const float *__restrict__ Ca, int n1, int lane, float *acc)
+{
+    const int W = rec.hi[1] - rec.lo[1] + 1;
+    const int H = rec.hi[0] - rec.lo[0] + 1;
+    if (W <= 0 || H <= 0) return;
+    const float kap = (float)KAPPA;
+    for (int cb = 0; cb < W; cb += 64) {
+        ColChunk cc;
+        cc.init(lane, W, cb);
+        if (!cc.act) continue;
+        const int i1 = rec.lo[1] + cb + cc.c;
+        const float dp1 = (p1[i1] - rec.c_hi[1]) - rec.c_lo[1];
+        const float rowlin = fmaf(rec.q[1], dp1, rec.q[3]);
+        const float colt = fmaf(fmaf(rec.q[2], dp1, rec.q[4]), dp1, rec.q[5]);
+        float A[5] = {0, 0, 0, 0, 0}, B[5] = {0, 0, 0, 0, 0};
+        const float *col = Ca + rec.lo[1] + cb + cc.c;
+        for (int r0 = cc.ro; r0 < H; r0 += 4 * cc.R) {
+            float cv[4], dp0[4];
+            bool ok[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int r = r0 + u * cc.R;
+                ok[u] = r < H;
+                const int i0 = rec.lo[0] + (ok[u] ? r : 0);
+                cv[u] = col[(size_t)i0 * n1];
+                dp0[u] = (p0[i0] - rec.c_hi[0]) - rec.c_lo[0];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const float q = fmaf(fmaf(rec.q[0], dp0[u], rowlin), dp0[u], colt);
+                if (ok[u] && q >= kap) {
+                    const float E = __builtin_amdgcn_exp2f(q);
+                    float w1;
+                    if (MODE == GT_MODE_L2) {
+                        const float res = fmaf(rec.alpha, E, -cv[u]);
+                        w1 = res * E;
+                        acc[30] = fmaf(res, res, acc[30]);
+                        acc[31] = fmaf(cv[u], cv[u], acc[31]);
+                    } else {
+                        w1 = cv[u] * E;
+                    }
+                    const float x = dp0[u], x2 = x * x;
+                    const float xp[5] = {1.0f, x, x2, x2 * x, x2 * x2};
+#pragma unroll
+                    for (int i = 0; i < 5; i++) A[i] = fmaf(w1, xp[i], A[i]);
+                    if (MODE != GT_MODE_LINEAR) {
+                        const float w2 = E * E;
+#pragma unroll
+                        for (int i = 0; i < 5; i++) B[i] = fmaf(w2, xp[i], B[i]);
+                    }
+                }
+            }
+        }
+        const float y2 = dp1 * dp1;
+        const float yp[5] = {1.0f, dp1, y2, y2 * dp1, y2 * y2};
+#pragma unroll
+        for (int i = 0; i <= 4; i++)
+#pragma unroll
+            for (int j = 0; i + j <= 4; j++) {
+                const int m = (i + j) * (i + j + 1) / 2 + j;
+                acc[m] = fmaf(A[i], yp[j], acc[m]);
+                if (MODE != GT_MODE_LINEAR) acc[15 + m] = fmaf(B[i], yp[j], acc[15 + m]);
+            }
+    }
+}
+
 __device__ inline float wave_sum_f32(float v)
 {
 #pragma unroll

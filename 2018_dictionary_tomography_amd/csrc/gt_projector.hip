@@ -191,7 +191,10 @@ __global__ __launch_bounds__(256) void moments_kernel(const PairRec *__restrict_
     if (valid) {
         const PairRec rec = recs[pair];
         const int a = (int)(pair / N);
-        pair_moments<K, MODE>(rec, p0, p1, C + (size_t)a * n0 * n1, n1, lane, acc);
+        if (K == 2)
+            pair_moments_rows<MODE>(rec, p0, p1, C + (size_t)a * n0 * n1, n1, lane, acc);
+        else
+            pair_moments<K, MODE>(rec, p0, p1, C + (size_t)a * n0 * n1, n1, lane, acc);
     }
 #pragma unroll
     for (int v = 0; v < MOM_STRIDE; v++) red[wave][lane][v] = acc[v];

@@ -26,7 +26,7 @@ class gt_fit_slot(ctypes.Structure):
     _fields_ = [
         ("active", ctypes.c_int), ("iso", ctypes.c_int), ("N", ctypes.c_int), ("nA", ctypes.c_int),
         ("n0", ctypes.c_int), ("n1", ctypes.c_int), ("lenT", ctypes.c_int), ("sorted", ctypes.c_int),
-        ("max_outer", ctypes.c_int), ("max_inner", ctypes.c_int), ("min_iter", ctypes.c_int), ("pad0", ctypes.c_int),
+        ("max_outer", ctypes.c_int), ("max_inner", ctypes.c_int), ("min_iter", ctypes.c_int), ("quota", ctypes.c_int),
         ("tol", ctypes.c_double),
         ("T", ctypes.c_void_p), ("w0", ctypes.c_void_p), ("w1", ctypes.c_void_p), ("p0", ctypes.c_void_p),
         ("p1", ctypes.c_void_p), ("perm", ctypes.c_void_p),
@@ -95,7 +95,7 @@ class FitRequest:
         g = self.geom
         self.slot = gt_fit_slot(
             active=1 if self.N > 0 else 0, iso=self.iso, N=self.N, nA=g.nA, n0=g.n0, n1=g.n1, lenT=g.lenT,
-            sorted=g.sorted, max_outer=self.max_outer, max_inner=self.max_inner, min_iter=int(min_iter), pad0=0,
+            sorted=g.sorted, max_outer=self.max_outer, max_inner=self.max_inner, min_iter=int(min_iter), quota=0,
             tol=float(tol), T=g.T.data_ptr(), w0=g.w0.data_ptr(), w1=g.w1.data_ptr() if g.w1 is not None else None,
             p0=g.p0.data_ptr(), p1=g.p1.data_ptr() if g.p1 is not None else None, perm=self.perm.data_ptr(),
             I=self.I.data_ptr(), x=self.x.data_ptr(), r=self.r.data_ptr(), resid=self.resid.data_ptr(),

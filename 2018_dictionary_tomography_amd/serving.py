@@ -1,0 +1,238 @@
+"""Continuous-batching server of Newton fits.
+
+Requests (atoms, geometry, data) arrive over time; each one becomes a slot of
+a device-resident slot table (gt_fit_slot) as soon as a slot is free and its
+HBM footprint fits the budget.  One launch of gt_fit_run advances every
+active slot by a work quota (one workgroup per slot, all slots of the table
+in one grid); the launch is captured once in a HIP graph and replayed, and
+between replays the host admits arrivals into free slots and retires finished
+fits.  State (atoms, residual, step sizes) never leaves the device while a
+fit runs.  Because each slot is computed by its own workgroup with a fixed
+reduction order, a request's result does not depend on what else is in the
+table (tests/test_gpu_serving.py checks this bit for bit).
+
+`plain_fits` runs the same fits one request at a time (one slot, the same
+kernel), the baseline the served throughput is compared with.
+"""
+from __future__ import annotations
+
+import ctypes
+import importlib
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import fit as F
+from . import ops
+from .spaces import AtomElement, AtomSpace, ProjSpace
+
+W = importlib.import_module(__package__ + ".workloads")
+
+
+@dataclass
+class RequestSpec:
+    """One fit request of the serving workload."""
+    rid: int
+    arrival: float  # seconds after the stream starts
+    N: int
+    nA: int
+    S: int
+    seed: int
+    max_iter: int = 30
+
+    def footprint_bytes(self):
+        """HBM held while the request is in flight: data + residual sinograms,
+        atoms (twice), permutations, step sizes, moment scratch, geometry."""
+        sino = self.nA * self.S * self.S * 4
+        return 2 * sino + self.N * (10 * 4 * 2 + 8) + self.max_iter * self.N * 4 + self.nA * (32 * 4 + 12 * 4) + 8 * self.S
+
+
+def c5_stream(n_requests=2000, rate=200.0, seed=0, n_range=(50, 5000), angle_range=(30, 120),
+              sides=(64, 96, 128, 192, 256), max_iter=30):
+    """Poisson arrivals at `rate`; N log-uniform in n_range, angle count uniform,
+    detector side uniform over `sides`."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, n_requests)
+    t = np.cumsum(gaps) - gaps[0]
+    lo, hi = np.log(n_range[0]), np.log(n_range[1])
+    out = []
+    for i in range(n_requests):
+        N = int(round(np.exp(rng.uniform(lo, hi))))
+        nA = int(rng.integers(angle_range[0], angle_range[1] + 1))
+        S = int(rng.choice(sides))
+        out.append(RequestSpec(i, float(t[i]), N, nA, S, int(rng.integers(1 << 30)), max_iter))
+    return out
+
+
+def build_request(spec: RequestSpec, device):
+    """Synthetic request: ground-truth atoms drawn as the 3D workload, tilt series
+    over +-60 deg about y, data = forward + 1% noise, start = truth with centres
+    perturbed by N(0, 0.01)."""
+    rng = np.random.default_rng(spec.seed)
+    I, x, r, _ = W.draw_aniso3(spec.N, rng)
+    T, ortho, det = W.tilt_series(spec.nA, 60.0, det=spec.S)
+    space = ProjSpace(orientations=T, ortho=ortho, detector=det)
+    aspace = AtomSpace(3, isotropic=False)
+    truth = AtomElement(aspace, x, r, I, device=device)
+    data = ops.forward(truth, space)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(spec.seed)
+    data += torch.randn(data.shape, generator=gen, device=device) * (0.01 * data.max())
+    start = AtomElement(aspace, x + rng.normal(0, 0.01, x.shape), r, I, device=device)
+    return start, space, data
+
+
+def work_per_update(spec):
+    """Relative cost of one atom update: pixels touched ~ angles x detector area."""
+    return spec.nA * spec.S * spec.S
+
+
+@dataclass
+class ServeStats:
+    completed: int = 0
+    makespan: float = 0.0
+    latencies: list = field(default_factory=list)
+    service: list = field(default_factory=list)
+    launches: int = 0
+    admitted_bytes_peak: int = 0
+    queued_peak: int = 0
+    refused: int = 0
+    results: dict = field(default_factory=dict)
+
+    def summary(self):
+        lat = np.array(self.latencies) if self.latencies else np.zeros(1)
+        return {
+            "completed": self.completed,
+            "fits_per_s": self.completed / self.makespan if self.makespan > 0 else 0.0,
+            "makespan_s": self.makespan,
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "launches": self.launches,
+            "peak_admitted_bytes": self.admitted_bytes_peak,
+            "peak_queue": self.queued_peak,
+            "refused": self.refused,
+        }
+
+
+class FitServer:
+    def __init__(self, device=None, max_slots=512, hbm_budget=64 << 30, work_per_launch=2.0e7, use_graph=True,
+                 keep_results=False):
+        self.dev = device or torch.device("cuda", torch.cuda.current_device())
+        self.max_slots = max_slots
+        self.hbm_budget = hbm_budget
+        self.work_per_launch = work_per_launch
+        self.use_graph = use_graph
+        self.keep_results = keep_results
+        self.table = torch.zeros(max_slots * F.SLOT_BYTES, dtype=torch.uint8, device=self.dev)
+        self.host = torch.zeros(max_slots * F.SLOT_BYTES, dtype=torch.uint8).pin_memory()
+        self.slots = [None] * max_slots  # (spec, FitRequest, t_admit)
+        self.graph = None
+
+    def _launch(self):
+        if not self.use_graph:
+            F.run_slots(self.table, self.max_slots, 3, 1 << 30)
+            return
+        if self.graph is None:
+            s = torch.cuda.Stream(self.dev)
+            s.wait_stream(torch.cuda.current_stream(self.dev))
+            with torch.cuda.stream(s):
+                F.run_slots(self.table, self.max_slots, 3, 1 << 30)  # warm-up on the side stream
+            torch.cuda.current_stream(self.dev).wait_stream(s)
+            torch.cuda.synchronize(self.dev)
+            self.graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(self.graph):
+                F.run_slots(self.table, self.max_slots, 3, 1 << 30)
+        self.graph.replay()
+
+    def _write_slot(self, i, slot):
+        b = torch.frombuffer(bytearray(bytes(slot)), dtype=torch.uint8)
+        self.table[i * F.SLOT_BYTES:(i + 1) * F.SLOT_BYTES].copy_(b.to(self.dev))
+
+    def _read_all(self):
+        self.host.copy_(self.table)
+        raw = self.host.numpy().tobytes()
+        return [F.gt_fit_slot.from_buffer_copy(raw, i * F.SLOT_BYTES) for i in range(self.max_slots)]
+
+    def run(self, specs, time_scale=1.0, log=None):
+        """Serve `specs` (sorted by arrival).  time_scale stretches the arrival
+        clock (1.0 = real time).  Returns ServeStats."""
+        st = ServeStats()
+        pending = sorted(specs, key=lambda s: s.arrival)
+        queue = []
+        reserved = 0
+        t0 = time.perf_counter()
+        nxt = 0
+        active = 0
+        # the empty table must not be left with stale slots
+        self.table.zero_()
+        torch.cuda.synchronize(self.dev)
+        while nxt < len(pending) or queue or active:
+            now = time.perf_counter() - t0
+            while nxt < len(pending) and pending[nxt].arrival * time_scale <= now:
+                spec = pending[nxt]
+                if spec.footprint_bytes() > self.hbm_budget:
+                    st.refused += 1
+                else:
+                    queue.append(spec)
+                nxt += 1
+            st.queued_peak = max(st.queued_peak, len(queue))
+            # admit FIFO while a slot and memory are free
+            free = [i for i in range(self.max_slots) if self.slots[i] is None]
+            while queue and free and reserved + queue[0].footprint_bytes() <= self.hbm_budget:
+                spec = queue.pop(0)
+                i = free.pop(0)
+                start, space, data = build_request(spec, self.dev)
+                req = F.FitRequest(start, space, data, [spec.max_iter, 1], record=False)
+                req.slot.quota = max(1, int(self.work_per_launch / work_per_update(spec)))
+                self._write_slot(i, req.slot)
+                self.slots[i] = (spec, req, time.perf_counter() - t0)
+                reserved += spec.footprint_bytes()
+                active += 1
+            st.admitted_bytes_peak = max(st.admitted_bytes_peak, reserved)
+            if active == 0:
+                if nxt < len(pending):
+                    time.sleep(max(0.0, min(0.01, pending[nxt].arrival * time_scale - now)))
+                continue
+            self._launch()
+            st.launches += 1
+            torch.cuda.synchronize(self.dev)
+            for i, s in enumerate(self._read_all()):
+                if self.slots[i] is None or not (s.done or not s.active):
+                    continue
+                spec, req, t_admit = self.slots[i]
+                t_done = time.perf_counter() - t0
+                st.latencies.append(t_done - spec.arrival * time_scale)
+                st.service.append(t_done - t_admit)
+                st.completed += 1
+                if self.keep_results:
+                    st.results[spec.rid] = (req.I.clone(), req.x.clone(), req.r.clone(), s.E, s.done, s.updates)
+                reserved -= spec.footprint_bytes()
+                active -= 1
+                self.slots[i] = None
+                self._write_slot(i, F.gt_fit_slot())  # inactive
+                if log:
+                    log(f"done rid={spec.rid} N={spec.N} nA={spec.nA} S={spec.S} updates={s.updates} "
+                        f"latency={st.latencies[-1]:.2f}s")
+        st.makespan = time.perf_counter() - t0
+        return st
+
+
+def plain_fits(specs, device=None, keep_results=False):
+    """The same fits one request at a time (one slot per launch sequence).
+    Returns (service times in s, results)."""
+    dev = device or torch.device("cuda", torch.cuda.current_device())
+    times, results = [], {}
+    for spec in specs:
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        start, space, data = build_request(spec, dev)
+        req, s = F.newton_fit(start, space, data, [spec.max_iter, 1], record=False,
+                              budget=max(1, int(2.0e7 / work_per_update(spec))))
+        torch.cuda.synchronize(dev)
+        times.append(time.perf_counter() - t0)
+        if keep_results:
+            results[spec.rid] = (req.I.clone(), req.x.clone(), req.r.clone(), s.E, s.done, s.updates)
+    return times, results

@@ -147,7 +147,7 @@ typedef struct gt_fit_slot {
     int N, nA, n0, n1;
     int lenT, sorted;
     int max_outer, max_inner, min_iter;
-    int pad0;
+    int quota;        /* if > 0: at most this many atom updates per gt_fit_run (else `budget`) */
     double tol;       /* convergence threshold on ||dx|| */
     const float *T, *w0, *w1, *p0, *p1; /* geometry (gt_geom layout) */
     const int *perm;  /* [max_outer][N] atom order of each outer iteration */
