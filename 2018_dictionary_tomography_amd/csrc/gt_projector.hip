@@ -3,9 +3,10 @@
 // Pipeline (all stream-ordered, no host synchronisation):
 //   setup    one thread per (atom, angle): FP64 detector Gaussian -> PairRec
 //   forward  one block per (angle, 16x64 pixel tile): stages the angle's
-//            records that touch the tile in LDS, 4 pixels per lane, Kahan sums
-//   moments  one wave per (atom, angle): FP32 pixel moments over the
-//            footprint box, LDS-transposed wave reduction, no atomics
+//            records that touch the tile in LDS, 4 rows per wave, 4 pixels
+//            per lane, Kahan sums
+//   moments  one wave per (atom, angle): FP32 separable pixel moments over
+//            the footprint box, LDS-transposed wave reduction, no atomics
 //   finish   one wave per atom, one lane per angle: FP64 moment -> (f, g, H)
 //            contraction (gt_pair.h), FP64 wave reduction over angles
 //
@@ -91,6 +92,10 @@ __device__ inline int stage_chunk(const PairRec *__restrict__ ra, int base, int 
 // ---------------------------------------------------------------------------
 // forward, 2D detectors (3D and 2.5D geometries)
 // ---------------------------------------------------------------------------
+// Block tile 16 x 64 pixels; wave w owns rows 4w .. 4w+3 of it and lane l
+// column l.  Each of a lane's 4 rows is shared by the whole wave, so the
+// exp/accumulate branch below is taken row-coherently (square wave tiles were
+// measured slower for that reason).
 __global__ __launch_bounds__(256) void forward3_kernel(const PairRec *__restrict__ recs, int N, int n0, int n1,
                                                        const float *__restrict__ p0, const float *__restrict__ p1,
                                                        float *__restrict__ R)
@@ -119,6 +124,7 @@ __global__ __launch_bounds__(256) void forward3_kernel(const PairRec *__restrict
         const int total = stage_chunk(ra, base, N, row0, thi0, col0, thi1, list, wcnt);
         for (int e = 0; e < total; e++) {
             const PairRec &q = list[e];
+            // wave-uniform cull against the wave's 4 rows
             if (q.lo[0] > r0 + ROWS_PER_LANE - 1 || q.hi[0] < r0) continue;
             const float dp1 = (pc - q.c_hi[1]) - q.c_lo[1];
             const float rowlin = fmaf(q.q[1], dp1, q.q[3]);
