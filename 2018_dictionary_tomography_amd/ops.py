@@ -246,7 +246,10 @@ def _single_atom_derivs(atom, Rad, C, order, mode):
     f, g, H = derivs(a0, _range_space(Rad), C.to(a0.I.device), mode, 2 if order > 1 else 1)
     f = f[0].item()
     g = g[0].cpu().numpy()
-    H = H[0].cpu().numpy() if H is not None else None
+    # The reference returns a (P, P) array at every order (left unfilled at
+    # order 1, NewtonBases.py:393,444); GaussTomo.derivs indexes it, so order 1
+    # gives zeros here.
+    H = H[0].cpu().numpy() if H is not None else np.zeros((g.size, g.size))
     return f, g, H
 
 

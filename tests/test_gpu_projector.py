@@ -201,6 +201,9 @@ def test_dropin_module_functions(gt, oracle, cuda):
     f, Df, DDf = gt.derivs_RadProj(atom, Rad, C)
     fo, go, Ho = oracle.derivs(f32(I[:1]), f32(x[:1]), f32(r[:1]), og, C, "linear", 2)
     assert relerr(Df, go[0]) < TOL and relerr(DDf, Ho[0]) < TOL
+    # order 1 still returns a (P, P) third value, as GaussTomo.derivs indexes it
+    f1, Df1, DDf1 = gt.derivs_RadProj(atom, Rad, C, order=1)
+    assert DDf1.shape == (10, 10) and relerr(Df1, go[0]) < TOL
     y = [f32(c) for c in gt.VolSpace(box=(-1, 1), shape=20, dim=3).coord_vectors]
     u = np.zeros((20, 20, 20), dtype=np.float32)
     gt.VolProj(atom, gt.VolSpace(y), u)
