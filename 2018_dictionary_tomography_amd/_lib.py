@@ -58,6 +58,9 @@ SIGNATURES = {
     "gt_ft_workspace": (_sz, [_P(gt_atoms), _P(gt_kgrid)]),
     "gt_ft_eval": (_ci, [_P(gt_atoms), _P(gt_kgrid), _vp, _vp, _sz, _vp]),
     "gt_fit_run": (_ci, [_vp, _ci, _ci, _ci, _vp]),
+    "gt_fit_scratch_bytes": (_sz, [_ci]),
+    "gt_fit_max_blocks": (_ci, []),
+    "gt_fit_run_blocks": (_ci, [_vp, _ci, _vp, _ci, _ci, _ci, _vp]),
     "gt_profile_enable": (_ci, [_ci]),
     "gt_profile_read": (_ci, [_ci, _P(ctypes.c_double), _P(ctypes.c_long)]),
 }

@@ -34,7 +34,7 @@ void prof_end(hipStream_t st, int id);
     } while (0)
 
 constexpr size_t ALIGN = 256;
-inline size_t align_up(size_t v) { return (v + ALIGN - 1) / ALIGN * ALIGN; }
+__host__ __device__ inline size_t align_up(size_t v) { return (v + ALIGN - 1) / ALIGN * ALIGN; }
 
 int check_atoms(const gt_atoms *a);
 int check_geom(const gt_geom *g, int dim);

@@ -1,4 +1,4 @@
-// gt_fit.hip -- device-resident damped-Newton fitter (one workgroup per fit).
+// gt_fit.hip -- device-resident damped-Newton fitter.
 //
 // Restates NewtonGaussian.linesearch (NewtonGaussian.py:18-109) with
 // GD_lib._step (GD_lib.py:492-536) for the null regulariser and the l2
@@ -18,13 +18,22 @@
 //      Step size h_j x2 / /10 clamped to [1e-4, 1e4], convergence flags as in
 //      NewtonGaussian.py:78-98.
 //
-// Everything for one slot happens inside one workgroup of 16 waves: no host
-// round trip and no inter-workgroup synchronisation; workgroup barriers order
-// the phases.  Pixel phases give one (atom, angle) pair to each wave with four
-// pixels in flight per lane; the FP64 contraction stages per-pair ingredients
-// in LDS (pair_prepare) and spreads the 10 + 55 gradient/Hessian entries over
-// the waves (fin_grad / fin_hess, one pair per lane); the solve runs on one
-// wave with one matrix row per lane.
+// A slot is advanced by a group of G workgroups of 16 waves (G = 1 for
+// gt_fit_run).  Member m owns angles m, m + G, ...: their detector Gaussians
+// (LDS), pixel moments and residual slices.  Every
+// cross-angle quantity -- the 10 + 55 gradient/Hessian entries and the energy
+// change -- is first formed per angle in the slot's scratch and then summed
+// over angles in one fixed order by every member, so all members take the
+// same decisions from the same numbers and the result does not depend on G.
+// The members meet at two device-wide barriers per atom update (after the
+// per-angle contractions and after the per-angle energy changes); a barrier
+// wait is bounded in time.  The solve and the accept logic run redundantly in
+// every member; member 0 writes the progress fields back at the end.
+//
+// Pixel phases give one (atom, angle) pair to each wave.  A wave contracts
+// the moments of its angle as soon as it has reduced them (the FP64 work of
+// one pair spread over the wave's lanes, see prep1..prep3), so the latency of
+// that FP64 chain overlaps the pixel work of the other waves.
 #include "gt_pixel.h"
 
 namespace gt {
@@ -33,22 +42,44 @@ constexpr int FIT_THREADS = 1024;
 constexpr int FIT_WAVES = FIT_THREADS / 64;
 constexpr int NAMAX = GT_FIT_MAX_ANGLES;
 constexpr int DETMAX = GT_FIT_MAX_DETECTOR;
-constexpr int FCH = 64;  // pairs per contraction chunk
-constexpr double FIT_EPS = 1e-4;  // NewtonGaussian.py:21
+constexpr int NVS = 65;                          // stride of a per-angle gradient/Hessian record
+constexpr int FIT_UNR = 8;                       // detector rows in flight per lane in the pixel passes
+constexpr double FIT_EPS = 1e-4;                 // NewtonGaussian.py:21
+constexpr long long FIT_WAIT_TICKS = 200000000;  // barrier wait limit: 2 s of the 100 MHz clock
+
+// Slot scratch layout (bytes): per-angle records double[nA][NVS] | per-angle
+// energy changes double[nA] | barrier words.
+__host__ __device__ inline size_t fit_off_dE(int nA) { return align_up((size_t)nA * NVS * sizeof(double)); }
+__host__ __device__ inline size_t fit_off_bar(int nA) { return fit_off_dE(nA) + align_up((size_t)nA * sizeof(double)); }
+__host__ __device__ inline size_t fit_scratch_size(int nA) { return fit_off_bar(nA) + ALIGN; }
+
+// Progress of a fit, held in LDS by every member during a launch.
+struct FitCtl {
+    int it, pos, inner, jj, brk0, brk1, done, pad;
+    double E;
+    long long updates, accepted;
+};
+
+// Per-wave LDS of the fused moments + contraction pass: one pair at a time.
+template <int D> struct WaveC {
+    PairFin<D> F;
+    PairAlg<D> a;
+    PairGeom<D> g;
+    double S1[15], S2[15];
+};
 
 template <int D> struct FitLDS {
-    static constexpr int P = Dims<D>::P;
-    PairRec rec_old[NAMAX];
+    PairRec rec_old[NAMAX];  // indexed by the member's own angle number
     PairRec rec_new[NAMAX];
-    PairFin<D> fin[FCH];
-    PairAlg<D> alg[FCH];
+    WaveC<D> wc[FIT_WAVES];
     float p0[DETMAX], p1[DETMAX];
-    double dE[FIT_WAVES];
     double th_old[10], th_new[10];
     double g[10], H[100];
     double A[100], b[10], x[10];
     double dxnorm;
-    int accept, j, stop, solved;
+    FitCtl c;
+    unsigned gen;
+    int accept, j, stop, abort_;
 };
 
 // theta (FP64 values of the FP32 atom) -> PairGeom atom part
@@ -114,9 +145,10 @@ __device__ inline float pair_value(const PairRec &rec, float p0v, float p1v)
 }
 
 // Energy change (apply == 0) or residual update (apply == 1) of replacing
-// rec_old by rec_new over the union box; returns the lane's FP64 partial.
-// Both passes evaluate the same FP32 expressions, so the committed residual
-// is exactly the one whose energy was tested.
+// rec_old by rec_new over the union box, both values evaluated per pixel;
+// returns the lane's FP64 partial.  Both passes evaluate the same FP32
+// expressions, so the committed residual is exactly the one whose energy was
+// tested.
 template <int K>
 __device__ inline double pair_delta4(const PairRec &ro, const PairRec &rn, const float *p0, const float *p1,
                                      float *__restrict__ Ra, int n1, int lane, int apply)
@@ -142,12 +174,12 @@ __device__ inline double pair_delta4(const PairRec &ro, const PairRec &rn, const
             const float clo = fmaf(fmaf(ro.q[2], d1o, ro.q[4]), d1o, ro.q[5]);
             const float cln = fmaf(fmaf(rn.q[2], d1n, rn.q[4]), d1n, rn.q[5]);
             float *col = Ra + i1;
-            for (int r0 = cc.ro; r0 < H; r0 += 4 * cc.R) {
-                float old[4], pv0[4];
-                bool ok[4];
-                size_t off[4];
+            for (int r0 = cc.ro; r0 < H; r0 += FIT_UNR * cc.R) {
+                float old[FIT_UNR], pv0[FIT_UNR];
+                bool ok[FIT_UNR];
+                size_t off[FIT_UNR];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < FIT_UNR; u++) {
                     const int r = r0 + u * cc.R;
                     ok[u] = r < H;
                     const int i0 = lo[0] + (ok[u] ? r : 0);
@@ -156,7 +188,7 @@ __device__ inline double pair_delta4(const PairRec &ro, const PairRec &rn, const
                     pv0[u] = p0[i0];
                 }
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < FIT_UNR; u++) {
                     const float d0o = (pv0[u] - ro.c_hi[0]) - ro.c_lo[0];
                     const float d0n = (pv0[u] - rn.c_hi[0]) - rn.c_lo[0];
                     const float qo = fmaf(fmaf(ro.q[0], d0o, rlo), d0o, clo);
@@ -208,9 +240,67 @@ __device__ inline double pair_delta4(const PairRec &ro, const PairRec &rn, const
     return acc;
 }
 
+// LDS written by some lanes of a wave, then read by others: order the wave's
+// LDS traffic (the lanes of a wave execute together).
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// pair_alg with detector direction b on lane b (b = 0..K; other lanes idle),
+// the same expressions as pair_alg.  B is not needed by the contraction.
+template <int D>
+__device__ inline void wave_pair_alg(const PairGeom<D> &g, const double *pstar, PairAlg<D> &a, int lane)
+{
+    constexpr int K = Dims<D>::K;
+    if (lane > K) return;
+    double t[3] = {0.0, 0.0, 0.0}, s = 0.0;
+    matvec_upper<D>(g.U, g.T, t);
+    for (int i = 0; i < D; i++) s += t[i] * t[i];
+    const int b = lane;
+    double h[3] = {0.0, 0.0, 0.0}, u[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < D; i++) {
+        if (b < K) {
+            h[i] = g.G[b][i];
+        } else {
+            double y = 0.0;
+            for (int k = 0; k < K; k++) y += pstar[k] * g.G[k][i];
+            h[i] = y - g.x[i];
+        }
+    }
+    matvec_upper<D>(g.U, h, u);
+    double ip = 0.0;
+    for (int i = 0; i < D; i++) ip += u[i] * t[i];
+    const double mu = ip / s;
+    for (int i = 0; i < 3; i++) {
+        a.h[b][i] = h[i];
+        a.u[b][i] = u[i];
+        a.ut[b][i] = (i < D) ? u[i] - mu * t[i] : 0.0;
+        a.ht[b][i] = (i < D) ? h[i] - mu * g.T[i] : 0.0;
+    }
+    a.mu[b] = mu;
+    if (lane == 0) {
+        for (int i = 0; i < 3; i++) a.t[i] = t[i];
+        a.s = s;
+        a.alpha = g.I / sqrt(s);
+    }
+}
+
+// Value of v in lane l (l the same in every lane): two v_readlane, no LDS.
+__device__ inline double readlane_f64(double v, int l)
+{
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)bits, l);
+    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Solve A x = b (n x n, row-major in LDS) with partial pivoting on one wave,
-// lane i holding row i.  Pivot = first row of largest magnitude (LAPACK
-// i*amax).  Returns false if a pivot is zero.
+// lane i holding row i; rows are broadcast with lane reads.  Pivot = first row
+// of largest magnitude (LAPACK i*amax), rows swapped as in LAPACK getrf.
+// Returns false if a pivot is zero (or not a number).
 template <int n> __device__ inline bool wave_solve(const double *A, const double *b, double *x, int lane)
 {
     const bool own = lane < n;
@@ -220,34 +310,33 @@ template <int n> __device__ inline bool wave_solve(const double *A, const double
     bool ok = true;
 #pragma unroll
     for (int k = 0; k < n; k++) {
-        double val = (own && lane >= k) ? fabs(a[k]) : -1.0;
-        int idx = lane;
+        const double colk = fabs(a[k]);
+        double best = readlane_f64(colk, k);
+        int p = k;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const double ov = __shfl_xor(val, o);
-            const int oi = __shfl_xor(idx, o);
-            if (ov > val || (ov == val && oi < idx)) {
-                val = ov;
-                idx = oi;
+        for (int r = k + 1; r < n; r++) {
+            const double v = readlane_f64(colk, r);
+            if (v > best) {
+                best = v;
+                p = r;
             }
         }
-        if (!(val > 0.0)) ok = false;
-        const int p = idx;
+        if (!(best > 0.0)) ok = false;
         if (p != k) {
 #pragma unroll
             for (int c = 0; c < n; c++) {
-                const double ak = __shfl(a[c], k), ap = __shfl(a[c], p);
+                const double ak = readlane_f64(a[c], k), ap = readlane_f64(a[c], p);
                 if (lane == k) a[c] = ap;
                 if (lane == p) a[c] = ak;
             }
-            const double bk = __shfl(bb, k), bp = __shfl(bb, p);
+            const double bk = readlane_f64(bb, k), bp = readlane_f64(bb, p);
             if (lane == k) bb = bp;
             if (lane == p) bb = bk;
         }
         double pk[n];
 #pragma unroll
-        for (int c = 0; c < n; c++) pk[c] = __shfl(a[c], k);
-        const double pb = __shfl(bb, k);
+        for (int c = 0; c < n; c++) pk[c] = readlane_f64(a[c], k);
+        const double pb = readlane_f64(bb, k);
         if (own && lane > k) {
             const double f = a[k] / pk[k];
 #pragma unroll
@@ -263,8 +352,7 @@ template <int n> __device__ inline bool wave_solve(const double *A, const double
 #pragma unroll
         for (int c = 0; c < n; c++)
             if (c > i) s -= a[c] * xs[c];
-        const double xi = (lane == i) ? s / a[i] : 0.0;
-        xs[i] = __shfl(xi, i);
+        xs[i] = readlane_f64((lane == i) ? s / a[i] : 0.0, i);
     }
     if (lane == 0)
 #pragma unroll
@@ -293,48 +381,121 @@ template <int D> __device__ inline double collapse_H(const double *H, int P, int
     return s;
 }
 
+// Device-wide barrier of the G workgroups of one slot, called by thread 0 of
+// each after its workgroup has published its writes.  bar[0] counts arrivals,
+// bar[1] is the generation, bar[2] the abort flag.  The wait is bounded:
+// after FIT_WAIT_TICKS the slot is aborted.  Returns false if aborted.
+__device__ inline bool group_barrier(unsigned *bar, int G, unsigned &gen)
+{
+    if (__hip_atomic_load(bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return false;
+    const unsigned target = gen + 1u;
+    const unsigned old = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == (unsigned)G - 1u) {
+        __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(bar + 1, target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        const long long t0 = wall_clock64();
+        while (__hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != target) {
+            if (__hip_atomic_load(bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return false;
+            if (wall_clock64() - t0 > FIT_WAIT_TICKS) {
+                __hip_atomic_store(bar + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return false;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+    }
+    gen = target;
+    return true;
+}
+
 template <int D>
-__global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restrict__ slots, int nslots, int budget)
+__global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restrict__ slots, int nslots,
+                                                         const gt_fit_block *__restrict__ blocks, int budget)
 {
     using Dm = Dims<D>;
-    constexpr int K = Dm::K, P = Dm::P, NR = Dm::NR, NM = Dm::NM;
+    constexpr int K = Dm::K, P = Dm::P, NR = Dm::NR;
     constexpr int NV = P + P * (P + 1) / 2;
+    static_assert(NV <= NVS, "per-angle record stride too small");
     __shared__ FitLDS<D> L;
-    const int sid = blockIdx.x;
-    if (sid >= nslots) return;
+    int sid = blockIdx.x, member = 0, G = 1, quota = 0;
+    if (blocks) {
+        const gt_fit_block e = blocks[blockIdx.x];
+        sid = e.slot;
+        member = e.member;
+        G = e.group;
+        quota = e.quota;
+    }
+    if (sid < 0 || sid >= nslots || G < 1 || member < 0 || member >= G) return;
     gt_fit_slot &S = slots[sid];
     if (!S.active || S.done) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int iso = S.iso;
     const int PI = iso ? 2 + D : P;  // parameters of the solve
     const int NRs = iso ? 1 : NR;
-    const gt_geom G = slot_geom(S, D);
+    const gt_geom Gm = slot_geom(S, D);
     const int N = S.N, nA = S.nA, n1 = S.n1;
     const size_t slab = (size_t)S.n0 * S.n1;
     const float *det[2] = {S.p0, S.p1};
     const int nd[2] = {S.n0, S.n1};
+    char *scr = (char *)S.scratch;
+    double *prec = (double *)scr;
+    double *dEa = (double *)(scr + fit_off_dE(nA));
+    unsigned *bar = (unsigned *)(scr + fit_off_bar(nA));
+    const int nOwn = member < nA ? (nA - member + G - 1) / G : 0;
     for (int i = tid; i < S.n0; i += FIT_THREADS) L.p0[i] = S.p0[i];
     if (K == 2)
         for (int i = tid; i < S.n1; i += FIT_THREADS) L.p1[i] = S.p1[i];
+    if (tid == 0) {
+        FitCtl &c = L.c;
+        c.it = S.it;
+        c.pos = S.pos;
+        c.inner = S.inner;
+        c.jj = S.jj;
+        c.brk0 = S.brk0;
+        c.brk1 = S.brk1;
+        c.done = S.done;
+        c.E = S.E;
+        c.updates = S.updates;
+        c.accepted = S.accepted;
+        L.gen = (G > 1) ? __hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+        L.abort_ = 0;
+    }
+    __syncthreads();
 
     long long ticks[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     long long last = wall_clock64();
+    const bool timing = member == 0 && S.phase_ticks != nullptr;
 #define GT_FIT_TICK(k)                             \
-    if (tid == 0 && S.phase_ticks) {               \
+    if (tid == 0 && timing) {                      \
         const long long now_ = wall_clock64();     \
         ticks[k] += now_ - last;                   \
         last = now_;                               \
     }
+    // Publish this workgroup's global writes and meet the other members.
+#define GT_FIT_MEET()                                                          \
+    if (G > 1) {                                                               \
+        __threadfence();                                                       \
+        __syncthreads();                                                       \
+        if (tid == 0 && !group_barrier(bar, G, L.gen)) L.abort_ = 1;           \
+        __syncthreads();                                                       \
+        if (L.abort_) break;                                                   \
+        __threadfence();                                                       \
+    } else {                                                                   \
+        __syncthreads();                                                       \
+    }
 
-    const int my_budget = S.quota > 0 ? min(budget, S.quota) : budget;
+    const int my_budget = quota > 0 ? min(budget, quota) : (S.quota > 0 ? min(budget, S.quota) : budget);
     for (int step = 0; step < my_budget; step++) {
         // ---- choose the atom, load its parameters -----------------------
         if (tid == 0) {
-            L.stop = S.done != 0 || S.it >= S.max_outer || S.pos >= N;
+            FitCtl &c = L.c;
+            L.stop = c.done != 0 || c.it >= S.max_outer || c.pos >= N;
             if (!L.stop) {
-                const int j = S.perm[(size_t)S.it * N + S.pos];
-                L.stop = (j < 0 || j >= N);
-                if (L.stop) S.done = 2;
+                const int j = S.perm[(size_t)c.it * N + c.pos];
+                if (j < 0 || j >= N) {
+                    L.stop = 1;
+                    c.done = 2;
+                }
                 L.j = j;
                 if (!L.stop) {
                     L.th_old[0] = S.I[j];
@@ -343,88 +504,101 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                 }
             }
         }
-        if (tid < NV) {
-            if (tid < P)
-                L.g[tid] = 0.0;
-            else {
-                int i, l;
-                upper_ij(tid - P, P, i, l);
-                L.H[i * P + l] = 0.0;
-                L.H[l * P + i] = 0.0;
-            }
-        }
         __syncthreads();
         if (L.stop) break;
         const int j = L.j;
-        // ---- phase A: detector Gaussians of the current atom -------------
-        for (int a = tid; a < nA; a += FIT_THREADS) {
+        // ---- phase A: detector Gaussians of the current atom (own angles) --
+        for (int k = tid; k < nOwn; k += FIT_THREADS) {
             PairGeom<D> g;
             atom_from_theta<D>(L.th_old, iso, g);
-            load_angle<D>(G, a, g);
-            pair_setup<D>(g, det, nd, S.sorted, L.rec_old[a]);
+            load_angle<D>(Gm, member + G * k, g);
+            pair_setup<D>(g, det, nd, S.sorted, L.rec_old[k]);
         }
         __syncthreads();
         GT_FIT_TICK(0);
-        // ---- phase B: pixel moments against the residual ------------------
-        for (int a = wave; a < nA; a += FIT_WAVES) {
-            float acc[MOMW];
+        // ---- phase B+C: per-angle pixel moments, then that angle's FP64
+        //      gradient/Hessian record, both by the wave that owns the angle --
+        {
+            WaveC<D> &W = L.wc[wave];
+            for (int k = wave; k < nOwn; k += FIT_WAVES) {
+                const int a = member + G * k;
+                const PairRec &rec = L.rec_old[k];
+                float acc[MOMW];
 #pragma unroll
-            for (int v = 0; v < MOMW; v++) acc[v] = 0.0f;
-            if (K == 2)
-                pair_moments_rows<GT_MODE_RESID>(L.rec_old[a], L.p0, L.p1, S.resid + (size_t)a * slab, n1, lane, acc);
-            else
-                pair_moments4<K, GT_MODE_RESID>(L.rec_old[a], L.p0, L.p1, S.resid + (size_t)a * slab, n1, lane, acc);
-            float mine = 0.0f;
+                for (int v = 0; v < MOMW; v++) acc[v] = 0.0f;
+                const float *Ra = S.resid + (size_t)a * slab;
+                if (K == 2)
+                    pair_moments_rows<GT_MODE_RESID, FIT_UNR>(rec, L.p0, L.p1, Ra, n1, lane, acc);
+                else
+                    pair_moments4<K, GT_MODE_RESID>(rec, L.p0, L.p1, Ra, n1, lane, acc);
+                float mine = 0.0f;
 #pragma unroll
-            for (int v = 0; v < 30; v++) {
-                const float t = wave_sum_f32(acc[v]);
-                if (lane == v) mine = t;
-            }
-            if (lane < MOMW) S.scratch[(size_t)a * MOMW + lane] = mine;
-        }
-        __syncthreads();
-        GT_FIT_TICK(1);
-        // ---- phase C: FP64 gradient and Hessian of atom j ----------------
-        for (int c0 = 0; c0 < nA; c0 += FCH) {
-            const int cnt = min(FCH, nA - c0);
-            // pair k of the chunk runs on lane k / 16 of wave k % 16: the FP64
-            // prepare work is spread over all SIMDs instead of one wave
-            const int k = lane * FIT_WAVES + wave;
-            if (k < cnt) {
-                const int a = c0 + k;
-                const PairRec &rec = L.rec_old[a];
-                const bool empty = rec.lo[0] > rec.hi[0] || (K == 2 && rec.lo[1] > rec.hi[1]);
-                PairGeom<D> g;
-                atom_from_theta<D>(L.th_old, iso, g);
-                load_angle<D>(G, a, g);
-                double S1[NM], S2[NM];
-                const float *m = S.scratch + (size_t)a * MOMW;
-                for (int v = 0; v < NM; v++) {
-                    S1[v] = empty ? 0.0 : (double)m[v];
-                    S2[v] = empty ? 0.0 : (double)m[15 + v];
+                for (int v = 0; v < 30; v++) {
+                    const float t = wave_sum_f32(acc[v]);
+                    if (lane == v) mine = t;
                 }
+                const bool empty = rec.lo[0] > rec.hi[0] || (K == 2 && rec.lo[1] > rec.hi[1]);
+                if (lane < 15)
+                    W.S1[lane] = empty ? 0.0 : (double)mine;
+                else if (lane < 30)
+                    W.S2[lane - 15] = empty ? 0.0 : (double)mine;
+                if (lane == 0) {
+                    atom_from_theta<D>(L.th_old, iso, W.g);
+                    load_angle<D>(Gm, a, W.g);
+                }
+                wave_sync();
                 const double pstar[2] = {(double)rec.c_hi[0] + (double)rec.c_lo[0],
                                          (double)rec.c_hi[1] + (double)rec.c_lo[1]};
-                pair_prepare<D>(g, pstar, S1, S2, 1, L.fin[k], L.alg[k]);
+                wave_pair_alg<D>(W.g, pstar, W.a, lane);
+                wave_sync();
+                for (int i = lane; i < PrepCount<D>::N1; i += 64) prep1<D>(W.g, W.a, W.S1, W.S2, 1, i, W.F);
+                wave_sync();
+                if (lane < PrepCount<D>::N2) prep2<D>(W.g, W.a, lane, W.F);
+                wave_sync();
+                if (lane < PrepCount<D>::N3) prep3<D>(W.g, W.a, lane, W.F);
+                wave_sync();
+                for (int e = lane; e < NV; e += 64) {
+                    int i = e, l = e;
+                    if (e >= P) upper_ij(e - P, P, i, l);
+                    prec[(size_t)a * NVS + e] = (e < P) ? fin_grad<D>(W.F, e) : fin_hess<D>(W.F, i, l);
+                }
+                wave_sync();  // W is reused by the wave's next angle
             }
-            __syncthreads();
-            for (int e = wave; e < NV; e += FIT_WAVES) {
-                int i = e, l = e;
-                if (e >= P) upper_ij(e - P, P, i, l);
-                double v = 0.0;
-                if (lane < cnt) v = (e < P) ? fin_grad<D>(L.fin[lane], e) : fin_hess<D>(L.fin[lane], i, l);
-                v = wave_sum_f64(v);
-                if (lane == 0) {
+        }
+        GT_FIT_TICK(1);
+        GT_FIT_MEET();
+        // sum the per-angle records over all angles in one fixed order (angle
+        // a and a + 64 on lane a, then the wave butterfly); all loads first
+        {
+            constexpr int NE = (NV + FIT_WAVES - 1) / FIT_WAVES;
+            double v[NE];
+#pragma unroll
+            for (int q = 0; q < NE; q++) {
+                const int e = wave + q * FIT_WAVES;
+                double t = 0.0;
+                if (e < NV) {
+                    if (lane < nA) t = prec[(size_t)lane * NVS + e];
+                    if (lane + 64 < nA) t += prec[(size_t)(lane + 64) * NVS + e];
+                }
+                v[q] = t;
+            }
+#pragma unroll
+            for (int q = 0; q < NE; q++) {
+                const int e = wave + q * FIT_WAVES;
+                const double t = wave_sum_f64(v[q]);
+                if (lane == 0 && e < NV) {
                     if (e < P) {
-                        L.g[e] += v;
+                        L.g[e] = t;
                     } else {
-                        L.H[i * P + l] += v;
-                        if (l != i) L.H[l * P + i] += v;
+                        int i, l;
+                        upper_ij(e - P, P, i, l);
+                        L.H[i * P + l] = t;
+                        L.H[l * P + i] = t;
                     }
                 }
             }
-            __syncthreads();
         }
+        __syncthreads();
         GT_FIT_TICK(2);
         // ---- phase D: damped Newton system, solve, projections ------------
         if (tid < PI * PI) {
@@ -445,143 +619,206 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         }
         __syncthreads();
         if (wave == 0) {
-            bool ok;
-            if (PI == P)
-                ok = wave_solve<P>(L.A, L.b, L.x, lane);
-            else
-                ok = wave_solve<2 + D>(L.A, L.b, L.x, lane);
-            if (lane == 0) L.solved = ok;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            if (!L.solved) {
-                S.done = 2;
-                L.stop = 1;
-            } else {
-                const double *b = L.x;
-                double n2 = 0.0;
-                for (int i = 0; i < PI; i++) n2 += b[i] * b[i];
-                L.dxnorm = sqrt(n2);
-                // GD_lib._step projections (GD_lib.py:502-522), result rounded to FP32
-                double t[10];
-                t[0] = fmax(0.0, L.th_old[0] + b[0]);
-                for (int i = 0; i < D; i++) t[1 + i] = L.th_old[1 + i] + b[1 + i];
-                if (iso) {
-                    t[1 + D] = fabs(L.th_old[1 + D] + b[1 + D]);
+            const bool ok = (PI == P) ? wave_solve<P>(L.A, L.b, L.x, lane) : wave_solve<2 + D>(L.A, L.b, L.x, lane);
+            if (lane == 0) {
+                if (!ok) {
+                    L.c.done = 2;
+                    L.stop = 1;
                 } else {
-                    for (int k = 0; k < NR; k++) t[1 + D + k] = L.th_old[1 + D + k] + b[1 + D + k];
-                    double *rr = t + 1 + D;
-                    if (D == 2) {
-                        if (rr[0] < 0) {
-                            rr[0] = -rr[0];
-                            rr[2] = -rr[2];
-                        }
-                        if (rr[1] < 0) rr[1] = -rr[1];
+                    const double *b = L.x;
+                    double n2 = 0.0;
+                    for (int i = 0; i < PI; i++) n2 += b[i] * b[i];
+                    L.dxnorm = sqrt(n2);
+                    // GD_lib._step projections (GD_lib.py:502-522), result rounded to FP32
+                    double t[10];
+                    t[0] = fmax(0.0, L.th_old[0] + b[0]);
+                    for (int i = 0; i < D; i++) t[1 + i] = L.th_old[1 + i] + b[1 + i];
+                    if (iso) {
+                        t[1 + D] = fabs(L.th_old[1 + D] + b[1 + D]);
                     } else {
-                        if (rr[0] < 0) {
-                            rr[0] = -rr[0];
-                            rr[3] = -rr[3];
-                            rr[5] = -rr[5];
+                        for (int k = 0; k < NR; k++) t[1 + D + k] = L.th_old[1 + D + k] + b[1 + D + k];
+                        double *rr = t + 1 + D;
+                        if (D == 2) {
+                            if (rr[0] < 0) {
+                                rr[0] = -rr[0];
+                                rr[2] = -rr[2];
+                            }
+                            if (rr[1] < 0) rr[1] = -rr[1];
+                        } else {
+                            if (rr[0] < 0) {
+                                rr[0] = -rr[0];
+                                rr[3] = -rr[3];
+                                rr[5] = -rr[5];
+                            }
+                            if (rr[1] < 0) {
+                                rr[1] = -rr[1];
+                                rr[4] = -rr[4];
+                            }
+                            if (rr[2] < 0) rr[2] = -rr[2];
                         }
-                        if (rr[1] < 0) {
-                            rr[1] = -rr[1];
-                            rr[4] = -rr[4];
-                        }
-                        if (rr[2] < 0) rr[2] = -rr[2];
                     }
+                    for (int i = 0; i < 1 + D + NRs; i++) L.th_new[i] = (double)(float)t[i];
                 }
-                for (int i = 0; i < 1 + D + NRs; i++) L.th_new[i] = (double)(float)t[i];
             }
         }
         __syncthreads();
         GT_FIT_TICK(3);
         if (L.stop) break;
         // ---- phase E: detector Gaussians of the proposal -------------------
-        for (int a = tid; a < nA; a += FIT_THREADS) {
+        for (int k = tid; k < nOwn; k += FIT_THREADS) {
             PairGeom<D> g;
             atom_from_theta<D>(L.th_new, iso, g);
-            load_angle<D>(G, a, g);
-            pair_setup<D>(g, det, nd, S.sorted, L.rec_new[a]);
+            load_angle<D>(Gm, member + G * k, g);
+            pair_setup<D>(g, det, nd, S.sorted, L.rec_new[k]);
         }
         __syncthreads();
         GT_FIT_TICK(4);
-        // ---- phase F: FP64 energy change ---------------------------------
-        {
-            double acc = 0.0;
-            for (int a = wave; a < nA; a += FIT_WAVES)
-                acc += pair_delta4<K>(L.rec_old[a], L.rec_new[a], L.p0, L.p1, S.resid + (size_t)a * slab, n1, lane, 0);
+        // ---- phase F: per-angle FP64 energy changes -----------------------
+        for (int k = wave; k < nOwn; k += FIT_WAVES) {
+            const int a = member + G * k;
+            float *Ra = S.resid + (size_t)a * slab;
+            double acc = pair_delta4<K>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, Ra, n1, lane, 0);
             acc = wave_sum_f64(acc);
-            if (lane == 0) L.dE[wave] = acc;
+            if (lane == 0) dEa[a] = acc;
         }
-        __syncthreads();
+        GT_FIT_MEET();
         GT_FIT_TICK(5);
-        // ---- phase G: accept / reject, step size, flags (one thread) -------
-        if (tid == 0) {
+        // ---- phase G: accept / reject, step size, flags -------------------
+        if (wave == 0) {
             double dE = 0.0;
-            for (int w = 0; w < FIT_WAVES; w++) dE += L.dE[w];
-            const bool accept = dE <= 0.0;  // false for NaN: a non-finite proposal is rejected
-            double h = S.h[j];
-            if (!accept) {
-                if (h > 2 * FIT_EPS) {
-                    S.brk0 = 0;
-                    S.brk1 = 0;
+            if (lane < nA) dE = dEa[lane];
+            if (lane + 64 < nA) dE += dEa[lane + 64];
+            dE = wave_sum_f64(dE);
+            if (lane == 0) {
+                FitCtl &c = L.c;
+                const bool accept = dE <= 0.0;  // false for NaN: a non-finite proposal is rejected
+                double h = S.h[j];
+                if (!accept) {
+                    if (h > 2 * FIT_EPS) {
+                        c.brk0 = 0;
+                        c.brk1 = 0;
+                    }
+                    h /= 10.0;
+                } else {
+                    c.E += dE;
+                    if (L.dxnorm > S.tol) {
+                        c.brk0 = 0;
+                        c.brk1 = 0;
+                    }
+                    h *= 2.0;
+                    // every member writes the same values: its own later reads
+                    // of atom j then never depend on another member's timing
+                    S.I[j] = (float)L.th_new[0];
+                    for (int i = 0; i < D; i++) S.x[(size_t)j * D + i] = (float)L.th_new[1 + i];
+                    for (int k = 0; k < NRs; k++) S.r[(size_t)j * NRs + k] = (float)L.th_new[1 + D + k];
+                    c.accepted++;
                 }
-                h /= 10.0;
-            } else {
-                S.E += dE;
-                if (L.dxnorm > S.tol) {
-                    S.brk0 = 0;
-                    S.brk1 = 0;
-                }
-                h *= 2.0;
-                S.I[j] = (float)L.th_new[0];
-                for (int i = 0; i < D; i++) S.x[(size_t)j * D + i] = (float)L.th_new[1 + i];
-                for (int k = 0; k < NRs; k++) S.r[(size_t)j * NRs + k] = (float)L.th_new[1 + D + k];
-                S.accepted++;
+                S.h[j] = fmin(fmax(h, FIT_EPS), 1.0 / FIT_EPS);
+                c.jj++;
+                if (S.Ehist && member == 0) S.Ehist[c.jj] = c.E;
+                c.updates++;
+                L.accept = accept;
             }
-            S.h[j] = fmin(fmax(h, FIT_EPS), 1.0 / FIT_EPS);
-            S.jj++;
-            if (S.Ehist) S.Ehist[S.jj] = S.E;
-            S.updates++;
-            L.accept = accept;
         }
         __syncthreads();
         GT_FIT_TICK(6);
-        // ---- phase H: commit the residual ---------------------------------
+        // ---- phase H: commit the residual of the own angles ----------------
         if (L.accept) {
-            for (int a = wave; a < nA; a += FIT_WAVES)
-                pair_delta4<K>(L.rec_old[a], L.rec_new[a], L.p0, L.p1, S.resid + (size_t)a * slab, n1, lane, 1);
+            for (int k = wave; k < nOwn; k += FIT_WAVES) {
+                const int a = member + G * k;
+                pair_delta4<K>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, S.resid + (size_t)a * slab, n1, lane, 1);
+            }
         }
         // ---- loop control (NewtonGaussian.py:50-98) ------------------------
         if (tid == 0) {
-            S.inner++;
-            if (S.brk1 || S.inner >= S.max_inner) {
-                S.inner = 0;
-                S.pos++;
+            FitCtl &c = L.c;
+            c.inner++;
+            if (c.brk1 || c.inner >= S.max_inner) {
+                c.inner = 0;
+                c.pos++;
             }
-            S.brk1 = 1;  // BREAK[1] = True at the start of every inner iteration
-            if (S.pos >= N) {
-                S.pos = 0;
-                if (S.brk0 && S.it > S.min_iter) {
-                    S.done = 1;
+            c.brk1 = 1;  // BREAK[1] = True at the start of every inner iteration
+            if (c.pos >= N) {
+                c.pos = 0;
+                if (c.brk0 && c.it > S.min_iter) {
+                    c.done = 1;
                 } else {
-                    S.it++;
-                    if (S.it >= S.max_outer) S.done = 1;
-                    S.brk0 = 1;
+                    c.it++;
+                    if (c.it >= S.max_outer) c.done = 1;
+                    c.brk0 = 1;
                 }
             }
         }
         __syncthreads();
         GT_FIT_TICK(7);
     }
+#undef GT_FIT_MEET
 #undef GT_FIT_TICK
-    if (tid == 0 && S.phase_ticks)
-        for (int k = 0; k < 8; k++) S.phase_ticks[k] += ticks[k];
+    if (tid == 0) {
+        const bool failed = L.abort_ || (G > 1 && __hip_atomic_load(bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (member == 0) {
+            const FitCtl &c = L.c;
+            S.it = c.it;
+            S.pos = c.pos;
+            S.inner = c.inner;
+            S.jj = c.jj;
+            S.brk0 = c.brk0;
+            S.brk1 = c.brk1;
+            S.E = c.E;
+            S.updates = c.updates;
+            S.accepted = c.accepted;
+            if (timing)
+                for (int k = 0; k < 8; k++) S.phase_ticks[k] += ticks[k];
+        }
+        // done only grows (0 -> 1/2 -> 3), whichever member writes last
+        if (member == 0 || failed) atomicMax(&S.done, failed ? 3 : L.c.done);
+    }
+}
+
+int max_blocks_cached()
+{
+    static int cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
+    if (cache[dev] > 0) return cache[dev];
+    int cus = 0, per3 = 0, per2 = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per3, fit_kernel<3>, FIT_THREADS, 0) != hipSuccess) return -1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, fit_kernel<2>, FIT_THREADS, 0) != hipSuccess) return -1;
+    cache[dev] = cus * min(per3, per2);
+    return cache[dev];
 }
 
 }  // namespace gt
 
 using namespace gt;
+
+extern "C" size_t gt_fit_scratch_bytes(int nA)
+{
+    if (nA < 0 || nA > NAMAX) return 0;
+    return fit_scratch_size(nA);
+}
+
+extern "C" int gt_fit_max_blocks(void)
+{
+    const int n = max_blocks_cached();
+    GT_CHECK_ARG(n > 0, "gt_fit_max_blocks: cannot query the device");
+    return n;
+}
+
+static int launch_fit(gt_fit_slot *slots, int nslots, const gt_fit_block *blocks, int nblocks, int dim, int budget,
+                      void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    prof_begin(st, GT_K_FIT);
+    if (dim == 3)
+        hipLaunchKernelGGL(fit_kernel<3>, dim3(nblocks), dim3(FIT_THREADS), 0, st, slots, nslots, blocks, budget);
+    else
+        hipLaunchKernelGGL(fit_kernel<2>, dim3(nblocks), dim3(FIT_THREADS), 0, st, slots, nslots, blocks, budget);
+    prof_end(st, GT_K_FIT);
+    GT_HIP(hipGetLastError());
+    return 0;
+}
 
 extern "C" int gt_fit_run(gt_fit_slot *slots, int nslots, int dim, int budget, void *stream)
 {
@@ -590,13 +827,21 @@ extern "C" int gt_fit_run(gt_fit_slot *slots, int nslots, int dim, int budget, v
     GT_CHECK_ARG(dim == 2 || dim == 3, "gt_fit_run: dim must be 2 or 3");
     GT_CHECK_ARG(budget >= 1, "gt_fit_run: budget must be >= 1");
     if (nslots == 0) return 0;
-    hipStream_t st = (hipStream_t)stream;
-    prof_begin(st, GT_K_FIT);
-    if (dim == 3)
-        hipLaunchKernelGGL(fit_kernel<3>, dim3(nslots), dim3(FIT_THREADS), 0, st, slots, nslots, budget);
-    else
-        hipLaunchKernelGGL(fit_kernel<2>, dim3(nslots), dim3(FIT_THREADS), 0, st, slots, nslots, budget);
-    prof_end(st, GT_K_FIT);
-    GT_HIP(hipGetLastError());
-    return 0;
+    return launch_fit(slots, nslots, nullptr, nslots, dim, budget, stream);
+}
+
+extern "C" int gt_fit_run_blocks(gt_fit_slot *slots, int nslots, const gt_fit_block *blocks, int nblocks, int dim,
+                                 int budget, void *stream)
+{
+    GT_CHECK_ARG(slots != nullptr || nslots == 0, "gt_fit_run_blocks: slots is NULL");
+    GT_CHECK_ARG(blocks != nullptr || nblocks == 0, "gt_fit_run_blocks: blocks is NULL");
+    GT_CHECK_ARG(nslots >= 0 && nslots < (1 << 24), "gt_fit_run_blocks: bad slot count %d", nslots);
+    GT_CHECK_ARG(dim == 2 || dim == 3, "gt_fit_run_blocks: dim must be 2 or 3");
+    GT_CHECK_ARG(budget >= 1, "gt_fit_run_blocks: budget must be >= 1");
+    if (nblocks == 0 || nslots == 0) return 0;
+    const int cap = max_blocks_cached();
+    GT_CHECK_ARG(cap > 0, "gt_fit_run_blocks: cannot query the device");
+    GT_CHECK_ARG(nblocks > 0 && nblocks <= cap,
+                 "gt_fit_run_blocks: %d blocks requested, the device holds %d fit workgroups at once", nblocks, cap);
+    return launch_fit(slots, nslots, blocks, nblocks, dim, budget, stream);
 }

@@ -493,124 +493,185 @@ template <int D> struct PairFin {
 static_assert((sizeof(PairFin<3>) / 8) % 2 == 1 && (sizeof(PairFin<2>) / 8) % 2 == 1,
               "PairFin must hold an odd number of doubles");
 
-// `a` is caller-provided scratch (e.g. LDS) so that a thread does not have to
-// keep the whole PairAlg in registers.
+// Per-entry pieces of pair_prepare, so that the ingredients of one pair can be
+// formed by many lanes at once.  prep1 (v, Kg, J, z) needs PairAlg and the
+// moments; prep2 (MUc, scalars) needs prep1; prep3 (Hh, Uh, Uu, Q, qt, uq)
+// needs prep2.  Entry idx of each stage is independent of the others.
+template <int D> struct PrepCount {
+    static constexpr int NG = Dims<D>::NG, P = Dims<D>::P;
+    static constexpr int N1 = NG + NG * NG + NG * P + 3;
+    static constexpr int N2 = 10;
+    static constexpr int N3 = 42;
+};
+
+// B-pair index of (b0, b1) in any order.
+template <int K> __host__ __device__ inline int bpair_index(int b0, int b1)
+{
+    const int a = b0 < b1 ? b0 : b1, b = b0 < b1 ? b1 : b0;
+    return a * (K + 1) - a * (a - 1) / 2 + (b - a);
+}
+
+// V[b0][b1] of Phi's gamma weights (symmetric; off-diagonal entries halved).
+template <int D> __host__ __device__ inline double vmat(const PairFin<D> &F, int b0, int b1)
+{
+    const double v = F.v[1 + bpair_index<Dims<D>::K>(b0, b1)];
+    return (b0 == b1) ? v : 0.5 * v;
+}
+
 template <int D>
-__host__ __device__ inline void pair_prepare(const PairGeom<D> &g, const double *pstar, const double *S1,
-                                             const double *S2, int gn, PairFin<D> &F, PairAlg<D> &a)
+__host__ __device__ inline void prep1(const PairGeom<D> &g, const PairAlg<D> &a, const double *S1, const double *S2,
+                                      int gn, int idx, PairFin<D> &F)
 {
     using Dm = Dims<D>;
-    constexpr int K = Dm::K, NB = Dm::NB, NG = Dm::NG, P = Dm::P, NR = Dm::NR;
-    pair_alg<D>(g, pstar, a);
-    const double al = a.alpha, s = a.s;
-    for (int p = 0; p < NB; p++) {
+    constexpr int K = Dm::K, NG = Dm::NG, P = Dm::P;
+    const double al = a.alpha;
+    if (idx < NG) {  // v
+        if (idx == 0) {
+            F.v[0] = S1[0];
+            return;
+        }
         int x0, x1, ea0, ea1, eb0, eb1;
-        bpair<K>(p, x0, x1);
+        bpair<K>(idx - 1, x0, x1);
         dexp<K>(x0, ea0, ea1);
         dexp<K>(x1, eb0, eb1);
         const double fp = (x0 == x1) ? 1.0 : 2.0;
-        const int m = mono_index<K>(ea0 + eb0, ea1 + eb1);
-        F.v[1 + p] = -30.0 * al * fp * S1[m];
-        const double val = -30.0 * fp * (S1[m] + (gn ? al * S2[m] : 0.0));
-        F.Kg[0][1 + p] = val;
-        F.Kg[1 + p][0] = val;
-        for (int q = p; q < NB; q++) {
-            int y0, y1, fa0, fa1, fb0, fb1;
-            bpair<K>(q, y0, y1);
-            dexp<K>(y0, fa0, fa1);
-            dexp<K>(y1, fb0, fb1);
-            const double fq = (y0 == y1) ? 1.0 : 2.0;
-            const int mm = mono_index<K>(ea0 + eb0 + fa0 + fb0, ea1 + eb1 + fa1 + fb1);
-            const double w = 900.0 * fp * fq * (al * S1[mm] + (gn ? al * al * S2[mm] : 0.0));
-            F.Kg[1 + p][1 + q] = w;
-            F.Kg[1 + q][1 + p] = w;
-        }
+        F.v[idx] = -30.0 * al * fp * S1[mono_index<K>(ea0 + eb0, ea1 + eb1)];
+        return;
     }
-    F.v[0] = S1[0];
-    F.Kg[0][0] = gn ? S2[0] : 0.0;
-
-    const double rs = 1.0 / sqrt(s), rs3 = rs / s;
-    for (int i = 0; i < NG; i++)
-        for (int j = 0; j < P; j++) F.J[i][j] = 0.0;
-    F.J[0][0] = rs;
-    for (int k = 0; k < NR; k++) F.J[0][1 + D + k] = -g.I * rs3 * a.t[slot_row(D, k)] * g.T[slot_col(D, k)];
-    for (int p = 0; p < NB; p++) {
-        int x0, x1;
-        bpair<K>(p, x0, x1);
-        for (int c = 0; c < D; c++) {
-            double val = 0.0;
-            if (x0 == K) {
-                double acc = 0.0;
-                for (int i = 0; i <= c; i++) acc += g.U[i][c] * a.ut[x1][i];
-                val -= acc;
+    idx -= NG;
+    if (idx < NG * NG) {  // Kg
+        const int l = idx / NG, m = idx - l * NG;
+        if (l == 0 && m == 0) {
+            F.Kg[0][0] = gn ? S2[0] : 0.0;
+            return;
+        }
+        int x0, x1, ea0, ea1, eb0, eb1;
+        bpair<K>((l == 0 ? m : l) - 1, x0, x1);
+        dexp<K>(x0, ea0, ea1);
+        dexp<K>(x1, eb0, eb1);
+        const double fp = (x0 == x1) ? 1.0 : 2.0;
+        if (l == 0 || m == 0) {
+            const int mi = mono_index<K>(ea0 + eb0, ea1 + eb1);
+            F.Kg[l][m] = -30.0 * fp * (S1[mi] + (gn ? al * S2[mi] : 0.0));
+            return;
+        }
+        int y0, y1, fa0, fa1, fb0, fb1;
+        bpair<K>(m - 1, y0, y1);
+        dexp<K>(y0, fa0, fa1);
+        dexp<K>(y1, fb0, fb1);
+        const double fq = (y0 == y1) ? 1.0 : 2.0;
+        const int mm = mono_index<K>(ea0 + eb0 + fa0 + fb0, ea1 + eb1 + fa1 + fb1);
+        F.Kg[l][m] = 900.0 * fp * fq * (al * S1[mm] + (gn ? al * al * S2[mm] : 0.0));
+        return;
+    }
+    idx -= NG * NG;
+    if (idx < NG * P) {  // J = d gamma / d theta
+        const int row = idx / P, j = idx - row * P;
+        double val = 0.0;
+        if (row == 0) {
+            const double rs = 1.0 / sqrt(a.s), rs3 = rs / a.s;
+            if (j == 0) {
+                val = rs;
+            } else if (j >= 1 + D) {
+                const int k = j - 1 - D;
+                val = -g.I * rs3 * a.t[slot_row(D, k)] * g.T[slot_col(D, k)];
             }
-            if (x1 == K) {
-                double acc = 0.0;
-                for (int i = 0; i <= c; i++) acc += g.U[i][c] * a.ut[x0][i];
-                val -= acc;
+        } else {
+            int x0, x1;
+            bpair<K>(row - 1, x0, x1);
+            if (j >= 1 && j <= D) {
+                const int c = j - 1;
+                if (x0 == K) {
+                    double acc = 0.0;
+                    for (int i = 0; i <= c; i++) acc += g.U[i][c] * a.ut[x1][i];
+                    val -= acc;
+                }
+                if (x1 == K) {
+                    double acc = 0.0;
+                    for (int i = 0; i <= c; i++) acc += g.U[i][c] * a.ut[x0][i];
+                    val -= acc;
+                }
+            } else if (j > D) {
+                const int i = slot_row(D, j - 1 - D), jj = slot_col(D, j - 1 - D);
+                val = a.ut[x0][i] * a.ht[x1][jj] + a.ut[x1][i] * a.ht[x0][jj];
             }
-            F.J[1 + p][1 + c] = val;
         }
-        for (int k = 0; k < NR; k++) {
-            const int i = slot_row(D, k), j = slot_col(D, k);
-            F.J[1 + p][1 + D + k] = a.ut[x0][i] * a.ht[x1][j] + a.ut[x1][i] * a.ht[x0][j];
-        }
+        F.J[row][j] = val;
+        return;
     }
-
-    double V[3][3];
-    for (int p = 0; p < NB; p++) {
-        int x0, x1;
-        bpair<K>(p, x0, x1);
-        const double w = (x0 == x1) ? F.v[1 + p] : 0.5 * F.v[1 + p];
-        V[x0][x1] = w;
-        V[x1][x0] = w;
-    }
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) {
-            double hh = 0.0, uh = 0.0, uu = 0.0;
-            if (i < D && j < D)
-                for (int b0 = 0; b0 <= K; b0++)
-                    for (int b1 = 0; b1 <= K; b1++) {
-                        hh += V[b0][b1] * a.ht[b0][i] * a.ht[b1][j];
-                        uh += V[b0][b1] * a.ut[b0][i] * a.ht[b1][j];
-                        uu += V[b0][b1] * a.ut[b0][i] * a.ut[b1][j];
-                    }
-            F.Hh[i][j] = hh;
-            F.Uh[i][j] = uh;
-            F.Uu[i][j] = uu;
-        }
-    F.va = F.v[0];
-    F.I = g.I;
-    F.s = s;
-    F.rs3 = rs3;
-    F.rs5 = rs3 / s;
-    F.VKK = V[K][K];
-    for (int c = 0; c < 3; c++) {
+    idx -= NG * P;
+    if (idx < 3) {  // z = U^T t (upper U), t, T
+        const int c = idx;
         double acc = 0.0;
         for (int i = 0; i <= c && c < D; i++) acc += g.U[i][c] * a.t[i];
         F.z[c] = acc;
         F.t[c] = a.t[c];
         F.T[c] = g.T[c];
     }
-    for (int c = 0; c < 3; c++)
-        for (int i = 0; i < 3; i++)
-            F.MUc[c][i] = (c < D && i < D) ? ((i <= c ? g.U[i][c] : 0.0) - F.z[c] / s * a.t[i]) : 0.0;
-    for (int c = 0; c < 3; c++)
-        for (int d = 0; d < 3; d++) {
-            double q = 0.0;
-            for (int i = 0; i <= c && c < D; i++) q += g.U[i][c] * F.MUc[d][i];
-            F.Q[c][d] = q;
-        }
-    for (int i = 0; i < 3; i++) {
-        double a1 = 0.0, a2 = 0.0;
-        if (i < D)
-            for (int b = 0; b <= K; b++) {
-                a1 += V[b][K] * a.ht[b][i];
-                a2 += V[b][K] * a.ut[b][i];
-            }
-        F.qt[i] = a1;
-        F.uq[i] = a2;
+}
+
+template <int D>
+__host__ __device__ inline void prep2(const PairGeom<D> &g, const PairAlg<D> &a, int idx, PairFin<D> &F)
+{
+    constexpr int K = Dims<D>::K;
+    if (idx < 9) {
+        const int c = idx / 3, i = idx - 3 * c;
+        F.MUc[c][i] = (c < D && i < D) ? ((i <= c ? g.U[i][c] : 0.0) - F.z[c] / a.s * a.t[i]) : 0.0;
+    } else if (idx == 9) {
+        const double s = a.s, rs = 1.0 / sqrt(s), rs3 = rs / s;
+        F.va = F.v[0];
+        F.I = g.I;
+        F.s = s;
+        F.rs3 = rs3;
+        F.rs5 = rs3 / s;
+        F.VKK = vmat<D>(F, K, K);
+        F.pad = 0.0;
     }
+}
+
+template <int D>
+__host__ __device__ inline void prep3(const PairGeom<D> &g, const PairAlg<D> &a, int idx, PairFin<D> &F)
+{
+    constexpr int K = Dims<D>::K;
+    if (idx < 27) {
+        const int which = idx / 9, i = (idx / 3) % 3, j = idx % 3;
+        double acc = 0.0;
+        if (i < D && j < D)
+            for (int b0 = 0; b0 <= K; b0++)
+                for (int b1 = 0; b1 <= K; b1++) {
+                    const double V = vmat<D>(F, b0, b1);
+                    if (which == 0)
+                        acc += V * a.ht[b0][i] * a.ht[b1][j];
+                    else if (which == 1)
+                        acc += V * a.ut[b0][i] * a.ht[b1][j];
+                    else
+                        acc += V * a.ut[b0][i] * a.ut[b1][j];
+                }
+        (which == 0 ? F.Hh : which == 1 ? F.Uh : F.Uu)[i][j] = acc;
+    } else if (idx < 36) {
+        const int c = (idx - 27) / 3, d = (idx - 27) % 3;
+        double q = 0.0;
+        for (int i = 0; i <= c && c < D; i++) q += g.U[i][c] * F.MUc[d][i];
+        F.Q[c][d] = q;
+    } else if (idx < 42) {
+        const int which = (idx - 36) / 3, i = (idx - 36) % 3;
+        double acc = 0.0;
+        if (i < D)
+            for (int b = 0; b <= K; b++) acc += vmat<D>(F, b, K) * (which == 0 ? a.ht[b][i] : a.ut[b][i]);
+        (which == 0 ? F.qt : F.uq)[i] = acc;
+    }
+}
+
+// `a` is caller-provided scratch (e.g. LDS) so that a thread does not have to
+// keep the whole PairAlg in registers.
+template <int D>
+__host__ __device__ inline void pair_prepare(const PairGeom<D> &g, const double *pstar, const double *S1,
+                                             const double *S2, int gn, PairFin<D> &F, PairAlg<D> &a)
+{
+    pair_alg<D>(g, pstar, a);
+    for (int i = 0; i < PrepCount<D>::N1; i++) prep1<D>(g, a, S1, S2, gn, i, F);
+    for (int i = 0; i < PrepCount<D>::N2; i++) prep2<D>(g, a, i, F);
+    for (int i = 0; i < PrepCount<D>::N3; i++) prep3<D>(g, a, i, F);
 }
 
 template <int D>

@@ -197,7 +197,9 @@ struct ColChunk {
 // Moments of one 2D-detector pair, separable form: per lane and column,
 // A_i = sum_rows w1 dp0^i and B_i = sum_rows E^2 dp0^i (i <= 4); then
 // S1[i, j] += A_i dp1^j, S2[i, j] += B_i dp1^j.  Same sums as pair_moments.
-template <int MODE>
+// UNR rows are loaded per step (more rows in flight hide load latency when
+// few waves share a SIMD).
+template <int MODE, int UNR = 4>
 __device__ inline void pair_moments_rows(const PairRec &rec, const float *p0, const float *p1,
                                          const float *__restrict__ Ca, int n1, int lane, float *acc)
 {
@@ -215,11 +217,11 @@ __device__ inline void pair_moments_rows(const PairRec &rec, const float *p0, co
         const float colt = fmaf(fmaf(rec.q[2], dp1, rec.q[4]), dp1, rec.q[5]);
         float A[5] = {0, 0, 0, 0, 0}, B[5] = {0, 0, 0, 0, 0};
         const float *col = Ca + rec.lo[1] + cb + cc.c;
-        for (int r0 = cc.ro; r0 < H; r0 += 4 * cc.R) {
-            float cv[4], dp0[4];
-            bool ok[4];
+        for (int r0 = cc.ro; r0 < H; r0 += UNR * cc.R) {
+            float cv[UNR], dp0[UNR];
+            bool ok[UNR];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < UNR; u++) {
                 const int r = r0 + u * cc.R;
                 ok[u] = r < H;
                 const int i0 = rec.lo[0] + (ok[u] ? r : 0);
@@ -227,7 +229,7 @@ __device__ inline void pair_moments_rows(const PairRec &rec, const float *p0, co
                 dp0[u] = (p0[i0] - rec.c_hi[0]) - rec.c_lo[0];
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < UNR; u++) {
                 const float q = fmaf(fmaf(rec.q[0], dp0[u], rowlin), dp0[u], colt);
                 if (ok[u] && q >= kap) {
                     const float E = __builtin_amdgcn_exp2f(q);

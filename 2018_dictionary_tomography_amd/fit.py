@@ -1,9 +1,10 @@
 """Fitters over the HIP library.
 
 newton_linesearch   NewtonGaussian.linesearch (NewtonGaussian.py:18-109) with
-                    GD_lib._step (GD_lib.py:492-536): device-resident, one
-                    workgroup per fit (gt_fit_run), FP64 block solves and FP64
-                    energy deltas.
+                    GD_lib._step (GD_lib.py:492-536): device-resident, a fit
+                    advanced by a group of workgroups that share its angles
+                    (gt_fit_run_blocks), FP64 block solves and FP64 energy
+                    deltas.
 gd_linesearch_block GD_lib.linesearch_block (GD_lib.py:82-178): block gradient
                     descent driven from the host over the batched forward and
                     derivative kernels.
@@ -44,13 +45,42 @@ MAX_ANGLES = 128
 MAX_DETECTOR = 1024
 
 
+class gt_fit_block(ctypes.Structure):
+    _fields_ = [("slot", ctypes.c_int), ("member", ctypes.c_int), ("group", ctypes.c_int), ("quota", ctypes.c_int)]
+
+
+BLOCK_BYTES = ctypes.sizeof(gt_fit_block)
+ANGLES_PER_MEMBER = 16  # one angle per wave of a 16-wave workgroup
+
+
 def _declare():
-    lib = load()
-    if not hasattr(lib.gt_fit_run, "_declared"):
-        lib.gt_fit_run.restype = ctypes.c_int
-        lib.gt_fit_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
-        lib.gt_fit_run._declared = True
-    return lib
+    return load()
+
+
+def default_group(nA):
+    """Workgroups that advance one fit together: enough that each holds about
+    one angle per wave, at most 8."""
+    return max(1, min(8, -(-int(nA) // ANGLES_PER_MEMBER)))
+
+
+def max_blocks():
+    """Fit workgroups the device holds at once (grouped launches may not exceed it)."""
+    n = _declare().gt_fit_max_blocks()
+    if n <= 0:
+        check(-1, "gt_fit_max_blocks")
+    return n
+
+
+def block_table(groups, quotas=None):
+    """gt_fit_block entries for slots 0.. with the given group sizes (None or 0
+    leaves a slot out): members of one slot are spread over consecutive blocks."""
+    ent = []
+    for s, g in enumerate(groups):
+        if not g:
+            continue
+        q = 0 if quotas is None else int(quotas[s])
+        ent += [(s, m, int(g), q) for m in range(int(g))]
+    return np.array(ent, dtype=np.int32).reshape(-1, 4)
 
 
 def permutations(n, max_outer, seed=1):
@@ -91,7 +121,8 @@ class FitRequest:
         self.Ehist = torch.zeros(nh, dtype=torch.float64, device=dev) if record else None
         if record:
             self.Ehist[0] = self.E0
-        self.scratch = torch.zeros(self.geom.nA * 32, dtype=torch.float32, device=dev)
+        lib = _declare()
+        self.scratch = torch.zeros(lib.gt_fit_scratch_bytes(self.geom.nA), dtype=torch.uint8, device=dev)
         g = self.geom
         self.slot = gt_fit_slot(
             active=1 if self.N > 0 else 0, iso=self.iso, N=self.N, nA=g.nA, n0=g.n0, n1=g.n1, lenT=g.lenT,
@@ -118,17 +149,35 @@ def run_slots(dev_table, nslots, dim, budget):
           "gt_fit_run")
 
 
-def newton_fit(atoms, space, data, max_iter, tol=1e-4, min_iter=1, seed=1, budget=None, record=True):
-    """Run one Newton fit to completion on the device; returns (request, final slot)."""
+def run_blocks(dev_table, nslots, dev_blocks, nblocks, dim, budget):
+    """Grouped launch: dev_blocks is a device int32 tensor of gt_fit_block rows."""
+    lib = _declare()
+    check(lib.gt_fit_run_blocks(ctypes.c_void_p(dev_table.data_ptr()), nslots, ctypes.c_void_p(dev_blocks.data_ptr()),
+                                nblocks, dim, budget, stream_handle(dev_table.device)), "gt_fit_run_blocks")
+
+
+def newton_fit(atoms, space, data, max_iter, tol=1e-4, min_iter=1, seed=1, budget=None, record=True, group=None):
+    """Run one Newton fit to completion on the device; returns (request, final
+    slot).  group: workgroups advancing the fit together (None: default_group
+    of the angle count; 1: a single workgroup).  The result does not depend on
+    the group size."""
     req = FitRequest(atoms, space, data, max_iter, tol, min_iter, seed, record)
     dev = req.I.device
     table = torch.as_tensor(np.frombuffer(req.slot_bytes(), dtype=np.uint8).copy(), device=dev)
     budget = budget or max(req.N * req.max_inner, 1)
+    group = default_group(req.geom.nA) if group is None else int(group)
+    group = max(1, min(group, max_blocks()))
+    blocks = torch.as_tensor(block_table([group]), device=dev) if group > 1 else None
     while True:
-        run_slots(table, 1, req.dim, budget)
+        if blocks is None:
+            run_slots(table, 1, req.dim, budget)
+        else:
+            run_blocks(table, 1, blocks, group, req.dim, budget)
         s = read_slots(table, 1)[0]
         if s.done or not s.active:
             break
+    if s.done == 3:
+        raise RuntimeError("the workgroups of the fit failed to meet at a barrier (device oversubscribed?)")
     req.final = s
     return req, s
 

@@ -2,17 +2,21 @@
 
 Requests (atoms, geometry, data) arrive over time; each one becomes a slot of
 a device-resident slot table (gt_fit_slot) as soon as a slot is free and its
-HBM footprint fits the budget.  One launch of gt_fit_run advances every
-active slot by a work quota (one workgroup per slot, all slots of the table
-in one grid); the launch is captured once in a HIP graph and replayed, and
-between replays the host admits arrivals into free slots and retires finished
-fits.  State (atoms, residual, step sizes) never leaves the device while a
-fit runs.  Because each slot is computed by its own workgroup with a fixed
-reduction order, a request's result does not depend on what else is in the
-table (tests/test_gpu_serving.py checks this bit for bit).
+HBM footprint fits the budget.  One grouped launch (gt_fit_run_blocks) over
+every fit workgroup the device holds advances every active slot by a work
+quota: the block table gives each active fit one workgroup, and spare
+workgroups go to the fits with the most remaining work (a group of G
+workgroups shares a fit's angles).  The launch is captured once in a HIP
+graph and replayed; between replays the host admits arrivals into free
+slots, retires finished fits and rewrites the block table.  State (atoms,
+residual, step sizes) never leaves the device while a fit runs.  A fit's
+gradients, Hessians and energy changes are reduced per angle in a fixed
+order, so its result depends neither on its group size nor on what else is
+in the table (tests/test_gpu_serving.py checks this bit for bit).
 
-`plain_fits` runs the same fits one request at a time (one slot, the same
-kernel), the baseline the served throughput is compared with.
+`plain_fits` runs the same fits one request at a time, each with the whole
+device (its default group), the baseline the served throughput is compared
+with.
 """
 from __future__ import annotations
 
@@ -44,10 +48,12 @@ class RequestSpec:
     max_iter: int = 30
 
     def footprint_bytes(self):
-        """HBM held while the request is in flight: data + residual sinograms,
-        atoms (twice), permutations, step sizes, moment scratch, geometry."""
+        """HBM held while the request is in flight: data, residual and per-pixel
+        value sinograms, atoms (twice), permutations, step sizes, the slot
+        scratch (moments, per-angle records and energy changes), geometry."""
         sino = self.nA * self.S * self.S * 4
-        return 2 * sino + self.N * (10 * 4 * 2 + 8) + self.max_iter * self.N * 4 + self.nA * (32 * 4 + 12 * 4) + 8 * self.S
+        scratch = self.nA * (32 * 4 + 65 * 8 + 8) + 1024
+        return 3 * sino + self.N * (10 * 4 * 2 + 8) + self.max_iter * self.N * 4 + scratch + self.nA * 12 * 4 + 8 * self.S
 
 
 def c5_stream(n_requests=2000, rate=200.0, seed=0, n_range=(50, 5000), angle_range=(30, 120),
@@ -85,9 +91,43 @@ def build_request(spec: RequestSpec, device):
     return start, space, data
 
 
+
 def work_per_update(spec):
-    """Relative cost of one atom update: pixels touched ~ angles x detector area."""
-    return spec.nA * spec.S * spec.S
+    """Relative cost of one atom update on one workgroup, in detector-pixel
+    units: the pixels touched grow with angles x detector area, plus a fixed
+    part (setup, FP64 contraction and solve) worth about 5.3e5 pixels."""
+    return spec.nA * spec.S * spec.S + 5.3e5
+
+
+# Fixed cost of one grouped atom update (barriers, redundant solve), pixel units.
+GROUP_UPDATE_OVERHEAD = 2.0e5
+
+
+def update_cost(spec, group):
+    """Relative cost of one atom update advanced by `group` workgroups."""
+    return work_per_update(spec) / group + (GROUP_UPDATE_OVERHEAD if group > 1 else 0.0)
+
+
+def allocate_groups(specs, remaining, capacity):
+    """Share `capacity` fit workgroups among the active requests: one each,
+    then one more at a time to the request whose remaining work per workgroup
+    is largest, up to its useful group size (fit.default_group)."""
+    import heapq
+
+    n = len(specs)
+    groups = [1] * n
+    spare = capacity - n
+    if spare <= 0:
+        return groups
+    heap = [(-remaining[i] * update_cost(specs[i], 1), i) for i in range(n) if F.default_group(specs[i].nA) > 1]
+    heapq.heapify(heap)
+    while spare > 0 and heap:
+        _, i = heapq.heappop(heap)
+        groups[i] += 1
+        spare -= 1
+        if groups[i] < F.default_group(specs[i].nA):
+            heapq.heappush(heap, (-remaining[i] * update_cost(specs[i], groups[i]), i))
+    return groups
 
 
 @dataclass
@@ -100,6 +140,7 @@ class ServeStats:
     admitted_bytes_peak: int = 0
     queued_peak: int = 0
     refused: int = 0
+    failed: int = 0
     results: dict = field(default_factory=dict)
 
     def summary(self):
@@ -114,37 +155,50 @@ class ServeStats:
             "peak_admitted_bytes": self.admitted_bytes_peak,
             "peak_queue": self.queued_peak,
             "refused": self.refused,
+            "failed": self.failed,
         }
 
 
 class FitServer:
-    def __init__(self, device=None, max_slots=512, hbm_budget=64 << 30, work_per_launch=2.0e7, use_graph=True,
-                 keep_results=False):
+    """Slot table + block table on the device; one graph-replayed grouped
+    launch advances every active fit by its quota.  max_slots defaults to the
+    number of fit workgroups the device holds at once, so every active fit
+    always has at least one workgroup."""
+
+    def __init__(self, device=None, max_slots=None, hbm_budget=64 << 30, work_per_launch=4.0e7, use_graph=True,
+                 keep_results=False, grouped=True):
         self.dev = device or torch.device("cuda", torch.cuda.current_device())
-        self.max_slots = max_slots
+        self.capacity = F.max_blocks()
+        self.max_slots = min(max_slots or self.capacity, self.capacity)
         self.hbm_budget = hbm_budget
         self.work_per_launch = work_per_launch
         self.use_graph = use_graph
         self.keep_results = keep_results
-        self.table = torch.zeros(max_slots * F.SLOT_BYTES, dtype=torch.uint8, device=self.dev)
-        self.host = torch.zeros(max_slots * F.SLOT_BYTES, dtype=torch.uint8).pin_memory()
-        self.slots = [None] * max_slots  # (spec, FitRequest, t_admit)
+        self.grouped = grouped
+        self.table = torch.zeros(self.max_slots * F.SLOT_BYTES, dtype=torch.uint8, device=self.dev)
+        self.host = torch.zeros(self.max_slots * F.SLOT_BYTES, dtype=torch.uint8).pin_memory()
+        self.blocks = torch.full((self.capacity, 4), -1, dtype=torch.int32, device=self.dev)
+        self.blocks_host = torch.full((self.capacity, 4), -1, dtype=torch.int32).pin_memory()
+        self.slots = [None] * self.max_slots  # (spec, FitRequest, t_admit)
         self.graph = None
+
+    def _raw_launch(self):
+        F.run_blocks(self.table, self.max_slots, self.blocks, self.capacity, 3, 1 << 30)
 
     def _launch(self):
         if not self.use_graph:
-            F.run_slots(self.table, self.max_slots, 3, 1 << 30)
+            self._raw_launch()
             return
         if self.graph is None:
             s = torch.cuda.Stream(self.dev)
             s.wait_stream(torch.cuda.current_stream(self.dev))
             with torch.cuda.stream(s):
-                F.run_slots(self.table, self.max_slots, 3, 1 << 30)  # warm-up on the side stream
+                self._raw_launch()  # warm-up on the side stream
             torch.cuda.current_stream(self.dev).wait_stream(s)
             torch.cuda.synchronize(self.dev)
             self.graph = torch.cuda.CUDAGraph()
             with torch.cuda.graph(self.graph):
-                F.run_slots(self.table, self.max_slots, 3, 1 << 30)
+                self._raw_launch()
         self.graph.replay()
 
     def _write_slot(self, i, slot):
@@ -156,6 +210,26 @@ class FitServer:
         raw = self.host.numpy().tobytes()
         return [F.gt_fit_slot.from_buffer_copy(raw, i * F.SLOT_BYTES) for i in range(self.max_slots)]
 
+    def _plan(self, progress):
+        """Rewrite the block table: groups and quotas of the active slots."""
+        act = [i for i in range(self.max_slots) if self.slots[i] is not None]
+        specs = [self.slots[i][0] for i in act]
+        remaining = []
+        for i, spec in zip(act, specs):
+            done = progress.get(i, 0)
+            remaining.append(max(1, spec.max_iter * spec.N - done))
+        groups = allocate_groups(specs, remaining, self.capacity) if self.grouped else [1] * len(act)
+        rows = np.full((self.capacity, 4), -1, dtype=np.int32)
+        b = 0
+        for i, spec, g in zip(act, specs, groups):
+            q = max(1, int(self.work_per_launch / update_cost(spec, g)))
+            for m in range(g):
+                rows[b] = (i, m, g, q)
+                b += 1
+        self.blocks_host.copy_(torch.from_numpy(rows))
+        self.blocks.copy_(self.blocks_host)
+        return dict(zip(act, groups))
+
     def run(self, specs, time_scale=1.0, log=None):
         """Serve `specs` (sorted by arrival).  time_scale stretches the arrival
         clock (1.0 = real time).  Returns ServeStats."""
@@ -166,9 +240,11 @@ class FitServer:
         t0 = time.perf_counter()
         nxt = 0
         active = 0
+        progress = {}
         # the empty table must not be left with stale slots
         self.table.zero_()
         torch.cuda.synchronize(self.dev)
+        dirty = True
         while nxt < len(pending) or queue or active:
             now = time.perf_counter() - t0
             while nxt < len(pending) and pending[nxt].arrival * time_scale <= now:
@@ -186,33 +262,43 @@ class FitServer:
                 i = free.pop(0)
                 start, space, data = build_request(spec, self.dev)
                 req = F.FitRequest(start, space, data, [spec.max_iter, 1], record=False)
-                req.slot.quota = max(1, int(self.work_per_launch / work_per_update(spec)))
                 self._write_slot(i, req.slot)
                 self.slots[i] = (spec, req, time.perf_counter() - t0)
+                progress[i] = 0
                 reserved += spec.footprint_bytes()
                 active += 1
+                dirty = True
             st.admitted_bytes_peak = max(st.admitted_bytes_peak, reserved)
             if active == 0:
                 if nxt < len(pending):
                     time.sleep(max(0.0, min(0.01, pending[nxt].arrival * time_scale - now)))
                 continue
+            if dirty:
+                self._plan(progress)
+                dirty = False
             self._launch()
             st.launches += 1
             torch.cuda.synchronize(self.dev)
             for i, s in enumerate(self._read_all()):
-                if self.slots[i] is None or not (s.done or not s.active):
+                if self.slots[i] is None:
+                    continue
+                progress[i] = s.updates
+                if not (s.done or not s.active):
                     continue
                 spec, req, t_admit = self.slots[i]
                 t_done = time.perf_counter() - t0
                 st.latencies.append(t_done - spec.arrival * time_scale)
                 st.service.append(t_done - t_admit)
                 st.completed += 1
+                st.failed += int(s.done == 3)
                 if self.keep_results:
                     st.results[spec.rid] = (req.I.clone(), req.x.clone(), req.r.clone(), s.E, s.done, s.updates)
                 reserved -= spec.footprint_bytes()
                 active -= 1
                 self.slots[i] = None
+                progress.pop(i, None)
                 self._write_slot(i, F.gt_fit_slot())  # inactive
+                dirty = True
                 if log:
                     log(f"done rid={spec.rid} N={spec.N} nA={spec.nA} S={spec.S} updates={s.updates} "
                         f"latency={st.latencies[-1]:.2f}s")
@@ -220,19 +306,25 @@ class FitServer:
         return st
 
 
-def plain_fits(specs, device=None, keep_results=False):
-    """The same fits one request at a time (one slot per launch sequence).
-    Returns (service times in s, results)."""
+def plain_fits(specs, device=None, keep_results=False, work_per_launch=4.0e7, group=None):
+    """The same fits one request at a time, each with the whole device to
+    itself: fit.newton_fit with its default group (group=1: one workgroup).
+    Returns (service times in s, results).  With keep_results, each result
+    also carries the drift of the incrementally updated residual:
+    max|resid - (forward(final atoms) - data)| / max|forward(final atoms)|."""
     dev = device or torch.device("cuda", torch.cuda.current_device())
     times, results = [], {}
     for spec in specs:
         torch.cuda.synchronize(dev)
         t0 = time.perf_counter()
         start, space, data = build_request(spec, dev)
-        req, s = F.newton_fit(start, space, data, [spec.max_iter, 1], record=False,
-                              budget=max(1, int(2.0e7 / work_per_update(spec))))
+        g = F.default_group(spec.nA) if group is None else group
+        req, s = F.newton_fit(start, space, data, [spec.max_iter, 1], record=False, group=g,
+                              budget=max(1, int(work_per_launch / update_cost(spec, g))))
         torch.cuda.synchronize(dev)
         times.append(time.perf_counter() - t0)
         if keep_results:
-            results[spec.rid] = (req.I.clone(), req.x.clone(), req.r.clone(), s.E, s.done, s.updates)
+            fresh = ops.forward(req, space)
+            drift = float((req.resid - (fresh - req.data)).abs().max() / fresh.abs().max())
+            results[spec.rid] = (req.I.clone(), req.x.clone(), req.r.clone(), s.E, s.done, s.updates, drift)
     return times, results

@@ -156,11 +156,12 @@ typedef struct gt_fit_slot {
     float *resid;     /* [nA][n0][n1] current projection minus data */
     double *h;        /* [N] per-atom step sizes, initialised to 1 */
     double *Ehist;    /* [max_outer*max_inner*N + 1] energy after each update, or NULL */
-    float *scratch;   /* [nA][32] pixel-moment scratch of this slot */
+    float *scratch;   /* gt_fit_scratch_bytes(nA) bytes, zeroed before the first run */
     /* progress (read-write) */
     int it, pos, inner, jj;  /* outer iteration, position in perm, inner step, history index */
     int brk0, brk1;          /* convergence flags of the current outer / inner loop */
-    int done;                /* 1: converged or out of iterations; 2: singular solve */
+    int done;                /* 1: converged or out of iterations; 2: singular solve;
+                                3: the workgroups of a group failed to meet (see gt_fit_run_blocks) */
     int pad1;
     double E;                /* current energy 1/2 ||resid||^2 */
     long long updates;       /* atom updates performed */
@@ -171,7 +172,29 @@ typedef struct gt_fit_slot {
 /* nA must be <= GT_FIT_MAX_ANGLES and n0, n1 <= GT_FIT_MAX_DETECTOR. dim 2 or 3. */
 #define GT_FIT_MAX_ANGLES 128
 #define GT_FIT_MAX_DETECTOR 1024
+size_t gt_fit_scratch_bytes(int nA);
 int gt_fit_run(gt_fit_slot *slots, int nslots, int dim, int budget, void *stream);
+
+/*
+ * Grouped launch: workgroup b of the grid advances slot blocks[b].slot as
+ * member blocks[b].member of a group of blocks[b].group workgroups, which
+ * share the slot's angles (member m owns angles m, m + group, ...) and meet
+ * at two device-wide barriers per atom update.  Gradients, Hessians and
+ * energy changes are reduced per angle in a fixed order, so a fit's result
+ * does not depend on its group size.  Every member of a slot must appear
+ * exactly once with the same group and quota (quota > 0: atom updates this
+ * launch, else the slot's quota / budget).  slot < 0 marks an idle block.
+ * nblocks must not exceed the number of workgroups the device holds at once
+ * (one per CU); a member that waits more than about two seconds for the
+ * others marks the slot done = 3 and returns.  gt_fit_run is the same
+ * launch with one workgroup per slot.  blocks is a device array.
+ */
+typedef struct gt_fit_block {
+    int slot, member, group, quota;
+} gt_fit_block;
+int gt_fit_max_blocks(void);
+int gt_fit_run_blocks(gt_fit_slot *slots, int nslots, const gt_fit_block *blocks, int nblocks, int dim, int budget,
+                      void *stream);
 
 #ifdef __cplusplus
 }

@@ -103,6 +103,53 @@ def test_newton_multi_slot_equals_single(gt, cuda):
         assert torch.equal(a.resid, m.resid)
 
 
+def test_newton_group_size_does_not_change_result(gt, oracle, cuda):
+    """A fit advanced by G = 1, 2, 3, 5 workgroups (angles shared between
+    them, per-angle reductions in a fixed order) is bit-identical to G = 1
+    and follows the oracle."""
+    F = importlib.import_module("2018_dictionary_tomography_amd.fit")
+    I, x, r, space, rng = _setup3(gt, 7, 23, 36, seed=12)
+    aspace = gt.AtomSpace(3, isotropic=False)
+    data = gt.ops.forward(gt.AtomElement(aspace, x, r, I, device=cuda), space)
+    data = data + torch.as_tensor(rng.normal(0, 0.01 * float(data.max()), data.shape), dtype=torch.float32,
+                                  device=cuda)
+    x0 = x + rng.normal(0, 0.02, x.shape)
+    start = gt.AtomElement(aspace, x0, r, I * 0.9, device=cuda)
+    runs = []
+    for G in (1, 2, 3, 5):
+        req = F.FitRequest(start, space, data, [3, 1], tol=1e-6)
+        table = torch.as_tensor(np.frombuffer(req.slot_bytes(), dtype=np.uint8).copy(), device=cuda)
+        blocks = torch.as_tensor(F.block_table([G], [4]), device=cuda)
+        for _ in range(100):
+            F.run_blocks(table, 1, blocks, G, 3, 1 << 20)
+            s = F.read_slots(table, 1)[0]
+            if s.done:
+                break
+        assert s.done == 1 and s.updates == 3 * 7
+        runs.append((req, s))
+    (r1, s1) = runs[0]
+    for req, s in runs[1:]:
+        assert torch.equal(req.x, r1.x) and torch.equal(req.r, r1.r) and torch.equal(req.I, r1.I)
+        assert torch.equal(req.resid, r1.resid) and torch.equal(req.Ehist, r1.Ehist)
+        assert s.E == s1.E and s.accepted == s1.accepted
+    Eg = r1.Ehist[: s1.jj + 1].cpu().numpy()
+    Io, xo, ro, Eo = oracle.newton_linesearch(f32(I * 0.9), f32(x0), f32(r), _oracle_geom(oracle, space),
+                                              data.double().cpu().numpy(), [3, 1], tol=1e-6)
+    assert np.abs(Eg - Eo).max() < 1e-5 * Eo[0]
+
+
+def test_grouped_launch_rejects_oversubscription(gt, cuda):
+    F = importlib.import_module("2018_dictionary_tomography_amd.fit")
+    cap = F.max_blocks()
+    assert cap >= 1
+    table = torch.zeros(F.SLOT_BYTES, dtype=torch.uint8, device=cuda)  # one inactive slot
+    blocks = torch.full((cap + 1, 4), -1, dtype=torch.int32, device=cuda)
+    with pytest.raises(gt.GTError, match="holds"):
+        F.run_blocks(table, 1, blocks, cap + 1, 3, 1)
+    F.run_blocks(table, 1, blocks, cap, 3, 1)  # idle blocks only: returns at once
+    torch.cuda.synchronize()
+
+
 def test_gd_block_2d_matches_oracle(gt, oracle, cuda):
     F = importlib.import_module("2018_dictionary_tomography_amd.fit")
     W = importlib.import_module("2018_dictionary_tomography_amd.workloads")

@@ -25,11 +25,28 @@ def test_server_alone_equals_mixed(gt, cuda):
     s = st.summary()
     assert s["p99_latency_s"] >= s["p50_latency_s"] > 0
     _, alone = Sv.plain_fits(specs, cuda, keep_results=True)
-    for rid, (I, x, r, E, done, upd) in alone.items():
+    for rid, (I, x, r, E, done, upd, drift) in alone.items():
         Im, xm, rm, Em, donem, updm = st.results[rid]
         assert done == donem == 1 and upd == updm
         assert torch.equal(I, Im) and torch.equal(x, xm) and torch.equal(r, rm)
         assert E == Em
+        # the residual kept by incremental updates stays that of the final atoms
+        assert drift < 1e-5
+
+
+def test_server_groups_equal_single_workgroup(gt, cuda):
+    """With angle counts that make the server give fits several workgroups,
+    every result still equals the one-workgroup fit bit for bit."""
+    Sv = importlib.import_module("2018_dictionary_tomography_amd.serving")
+    specs = Sv.c5_stream(5, rate=500.0, seed=8, n_range=(4, 12), angle_range=(33, 48), sides=(24, 32), max_iter=2)
+    srv = Sv.FitServer(cuda, max_slots=8, work_per_launch=2.0e5, keep_results=True)
+    st = srv.run(specs)
+    assert st.completed == len(specs) and st.failed == 0
+    _, alone = Sv.plain_fits(specs, cuda, keep_results=True, group=1)
+    for rid, (I, x, r, E, done, upd, drift) in alone.items():
+        Im, xm, rm, Em, donem, updm = st.results[rid]
+        assert done == donem == 1 and upd == updm
+        assert torch.equal(I, Im) and torch.equal(x, xm) and torch.equal(r, rm) and E == Em
 
 
 def test_server_memory_admission(gt, cuda):
