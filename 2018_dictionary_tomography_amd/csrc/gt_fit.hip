@@ -76,7 +76,7 @@ template <int D> struct FitLDS {
     double th_old[10], th_new[10];
     double g[10], H[100];
     double A[100], b[10], x[10];
-    double dxnorm;
+    double dxnorm, hj;
     FitCtl c;
     unsigned gen;
     int accept, j, stop, abort_;
@@ -601,10 +601,14 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         __syncthreads();
         GT_FIT_TICK(2);
         // ---- phase D: damped Newton system, solve, projections ------------
+        // h_j is read here, between the two barriers of this update: other
+        // members write its new value only after the second one
+        if (tid == 0) L.hj = S.h[j];
+        __syncthreads();
         if (tid < PI * PI) {
             const int i = tid / PI, l = tid - (tid / PI) * PI;
             double v = iso ? collapse_H<D>(L.H, P, i, l) : L.H[i * P + l];
-            if (i == l) v += 1.0 / S.h[j];
+            if (i == l) v += 1.0 / L.hj;
             L.A[tid] = v;
         }
         if (tid < PI) {
@@ -692,7 +696,7 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
             if (lane == 0) {
                 FitCtl &c = L.c;
                 const bool accept = dE <= 0.0;  // false for NaN: a non-finite proposal is rejected
-                double h = S.h[j];
+                double h = L.hj;
                 if (!accept) {
                     if (h > 2 * FIT_EPS) {
                         c.brk0 = 0;

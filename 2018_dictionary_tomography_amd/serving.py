@@ -48,12 +48,12 @@ class RequestSpec:
     max_iter: int = 30
 
     def footprint_bytes(self):
-        """HBM held while the request is in flight: data, residual and per-pixel
-        value sinograms, atoms (twice), permutations, step sizes, the slot
-        scratch (moments, per-angle records and energy changes), geometry."""
+        """HBM held while the request is in flight: data and residual sinograms,
+        atoms (twice), permutations, step sizes, the slot scratch (per-angle
+        gradient/Hessian records and energy changes), geometry."""
         sino = self.nA * self.S * self.S * 4
-        scratch = self.nA * (32 * 4 + 65 * 8 + 8) + 1024
-        return 3 * sino + self.N * (10 * 4 * 2 + 8) + self.max_iter * self.N * 4 + scratch + self.nA * 12 * 4 + 8 * self.S
+        scratch = self.nA * (65 * 8 + 8) + 1024
+        return 2 * sino + self.N * (10 * 4 * 2 + 8) + self.max_iter * self.N * 4 + scratch + self.nA * 12 * 4 + 8 * self.S
 
 
 def c5_stream(n_requests=2000, rate=200.0, seed=0, n_range=(50, 5000), angle_range=(30, 120),
@@ -90,6 +90,13 @@ def build_request(spec: RequestSpec, device):
     start = AtomElement(aspace, x + rng.normal(0, 0.01, x.shape), r, I, device=device)
     return start, space, data
 
+
+
+def prepare_payloads(specs, device):
+    """Client side: every request's (start atoms, geometry, data), built before
+    the stream is served so that neither the served nor the plain timing
+    includes generating the synthetic data."""
+    return {spec.rid: build_request(spec, device) for spec in specs}
 
 
 def work_per_update(spec):
@@ -205,10 +212,12 @@ class FitServer:
         b = torch.frombuffer(bytearray(bytes(slot)), dtype=torch.uint8)
         self.table[i * F.SLOT_BYTES:(i + 1) * F.SLOT_BYTES].copy_(b.to(self.dev))
 
-    def _read_all(self):
+    def _read_active(self):
+        """(index, gt_fit_slot) of every occupied slot, read back after a launch."""
         self.host.copy_(self.table)
-        raw = self.host.numpy().tobytes()
-        return [F.gt_fit_slot.from_buffer_copy(raw, i * F.SLOT_BYTES) for i in range(self.max_slots)]
+        raw = self.host.numpy()
+        return [(i, F.gt_fit_slot.from_buffer_copy(raw, i * F.SLOT_BYTES))
+                for i in range(self.max_slots) if self.slots[i] is not None]
 
     def _plan(self, progress):
         """Rewrite the block table: groups and quotas of the active slots."""
@@ -230,9 +239,10 @@ class FitServer:
         self.blocks.copy_(self.blocks_host)
         return dict(zip(act, groups))
 
-    def run(self, specs, time_scale=1.0, log=None):
+    def run(self, specs, time_scale=1.0, log=None, payloads=None):
         """Serve `specs` (sorted by arrival).  time_scale stretches the arrival
-        clock (1.0 = real time).  Returns ServeStats."""
+        clock (1.0 = real time).  payloads: prepare_payloads() output (else each
+        request's data is generated at admission).  Returns ServeStats."""
         st = ServeStats()
         pending = sorted(specs, key=lambda s: s.arrival)
         queue = []
@@ -260,7 +270,7 @@ class FitServer:
             while queue and free and reserved + queue[0].footprint_bytes() <= self.hbm_budget:
                 spec = queue.pop(0)
                 i = free.pop(0)
-                start, space, data = build_request(spec, self.dev)
+                start, space, data = payloads[spec.rid] if payloads else build_request(spec, self.dev)
                 req = F.FitRequest(start, space, data, [spec.max_iter, 1], record=False)
                 self._write_slot(i, req.slot)
                 self.slots[i] = (spec, req, time.perf_counter() - t0)
@@ -279,9 +289,7 @@ class FitServer:
             self._launch()
             st.launches += 1
             torch.cuda.synchronize(self.dev)
-            for i, s in enumerate(self._read_all()):
-                if self.slots[i] is None:
-                    continue
+            for i, s in self._read_active():
                 progress[i] = s.updates
                 if not (s.done or not s.active):
                     continue
@@ -306,7 +314,7 @@ class FitServer:
         return st
 
 
-def plain_fits(specs, device=None, keep_results=False, work_per_launch=4.0e7, group=None):
+def plain_fits(specs, device=None, keep_results=False, work_per_launch=4.0e7, group=None, payloads=None):
     """The same fits one request at a time, each with the whole device to
     itself: fit.newton_fit with its default group (group=1: one workgroup).
     Returns (service times in s, results).  With keep_results, each result
@@ -315,9 +323,9 @@ def plain_fits(specs, device=None, keep_results=False, work_per_launch=4.0e7, gr
     dev = device or torch.device("cuda", torch.cuda.current_device())
     times, results = [], {}
     for spec in specs:
+        start, space, data = payloads[spec.rid] if payloads else build_request(spec, dev)
         torch.cuda.synchronize(dev)
         t0 = time.perf_counter()
-        start, space, data = build_request(spec, dev)
         g = F.default_group(spec.nA) if group is None else group
         req, s = F.newton_fit(start, space, data, [spec.max_iter, 1], record=False, group=g,
                               budget=max(1, int(work_per_launch / update_cost(spec, g))))

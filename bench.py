@@ -46,9 +46,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--atoms", type=int, default=1000)
     ap.add_argument("--fit-iters", type=int, default=20, help="Newton outer iterations of the fit leg (0 skips it)")
-    ap.add_argument("--serve-requests", type=int, default=64, help="requests of the serving leg (0 skips it)")
+    ap.add_argument("--serve-requests", type=int, default=256, help="requests of the serving leg (0 skips it)")
     ap.add_argument("--serve-nmax", type=int, default=500, help="upper end of the log-uniform atom count")
-    ap.add_argument("--plain-requests", type=int, default=4, help="requests timed one at a time")
+    ap.add_argument("--plain-requests", type=int, default=6, help="requests timed one at a time")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg and print its JSON")
     ap.add_argument("--skip-checks", action="store_true",
                     help="skip the FP64 parity spot check and the in-support pair count (counter runs)")
@@ -64,7 +64,8 @@ def fit_leg(torch, gtp, space, data, start, iters):
     F = importlib.import_module("2018_dictionary_tomography_amd.fit")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    req, s = F.newton_fit(start, space, data, [iters, 1], record=True, budget=4000)
+    group = F.default_group(space.shape[0])
+    req, s = F.newton_fit(start, space, data, [iters, 1], record=True, budget=4000, group=group)
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     E = req.Ehist[: s.jj + 1].cpu().numpy()
@@ -73,8 +74,9 @@ def fit_leg(torch, gtp, space, data, start, iters):
     return {
         "outer_iterations": int(s.it), "atom_updates": int(s.updates), "accepted": int(s.accepted),
         "seconds": t, "updates_per_s": s.updates / t, "E0": float(E[0]), "E_final": float(s.E),
-        "monotone": bool(np.all(np.diff(E) <= 0)), "residual_drift": drift,
-        "note": "one workgroup advances the fit; FP64 solves and energy deltas",
+        "monotone": bool(np.all(np.diff(E) <= 0)), "residual_drift": drift, "workgroups": group,
+        "note": "device-resident fit advanced by a group of workgroups sharing the angles; FP64 solves and "
+                "energy deltas",
     }
 
 
@@ -82,19 +84,27 @@ def serving_leg(torch, args, dev):
     """Poisson stream at 200 req/s with the serving workload's distributions
     (angles U{30..120} over +-60 deg, detector side from {64,96,128,192,256},
     30-iteration Newton), atom count log-uniform in [50, serve_nmax].  Served
-    = continuous batching into the slot table with graph-replayed launches;
-    plain = the same fits one request at a time, on a seeded random subset."""
+    = continuous batching into the slot table with graph-replayed grouped
+    launches; plain = the same fits one request at a time, each with the whole
+    device (its default workgroup group), on a seeded random subset.  Request
+    payloads (start atoms, data) are generated before either is timed."""
     Sv = importlib.import_module("2018_dictionary_tomography_amd.serving")
     specs = Sv.c5_stream(args.serve_requests, rate=200.0, seed=args.seed, n_range=(50, args.serve_nmax))
     log = (lambda m: print(m, file=sys.stderr, flush=True)) if args.serve_only else None
-    server = Sv.FitServer(dev, max_slots=256, keep_results=True)
-    st = server.run(specs, log=log)
+    t0 = time.perf_counter()
+    payloads = Sv.prepare_payloads(specs, dev)
+    torch.cuda.synchronize()
+    prep_s = time.perf_counter() - t0
+    if log:
+        log(f"payloads of {len(specs)} requests built in {prep_s:.1f}s")
+    server = Sv.FitServer(dev, keep_results=True)
+    st = server.run(specs, log=log, payloads=payloads)
     served = st.summary()
     pick = sorted(np.random.default_rng(args.seed + 1).choice(len(specs), min(args.plain_requests, len(specs)),
                                                               replace=False).tolist())
     times, alone = [], {}
     for i in pick:
-        t, a = Sv.plain_fits([specs[i]], dev, keep_results=True)
+        t, a = Sv.plain_fits([specs[i]], dev, keep_results=True, payloads=payloads)
         times += t
         alone.update(a)
         if log:
@@ -104,9 +114,9 @@ def serving_leg(torch, args, dev):
     plain_fps = 1.0 / float(np.mean(times))
     return {
         "requests": len(specs), "rate_per_s": 200.0, "atoms_range": [50, args.serve_nmax],
-        "iterations": 30, "served": served,
+        "iterations": 30, "served": served, "client_payload_s": prep_s,
         "plain": {"requests_timed": pick, "service_s": times, "fits_per_s": plain_fps,
-                  "note": "same device fitter, one request at a time"},
+                  "note": "same device fitter, one request at a time, whole device (default group)"},
         "speedup_fits_per_s": served["fits_per_s"] / plain_fps,
         "alone_equals_mixed": bool(same),
         "max_residual_drift": max(a[6] for a in alone.values()),

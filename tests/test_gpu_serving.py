@@ -49,6 +49,25 @@ def test_server_groups_equal_single_workgroup(gt, cuda):
         assert torch.equal(I, Im) and torch.equal(x, xm) and torch.equal(r, rm) and E == Em
 
 
+def test_server_grouped_mix_of_large_fits(gt, cuda):
+    """Several fits of 60-100 angles in one launch, each advanced by 4-7
+    workgroups whose timing drifts apart: every result still equals the
+    one-workgroup fit bit for bit (members must not read state that another
+    member of the same update may already have rewritten)."""
+    Sv = importlib.import_module("2018_dictionary_tomography_amd.serving")
+    specs = Sv.c5_stream(6, rate=1000.0, seed=21, n_range=(40, 90), angle_range=(60, 100), sides=(64, 96, 128),
+                         max_iter=3)
+    pay = Sv.prepare_payloads(specs, cuda)
+    srv = Sv.FitServer(cuda, keep_results=True)
+    st = srv.run(specs, payloads=pay)
+    assert st.completed == len(specs) and st.failed == 0
+    _, alone = Sv.plain_fits(specs, cuda, keep_results=True, group=1, payloads=pay)
+    for rid, (I, x, r, E, done, upd, drift) in alone.items():
+        Im, xm, rm, Em, donem, updm = st.results[rid]
+        assert done == donem == 1 and upd == updm
+        assert torch.equal(I, Im) and torch.equal(x, xm) and torch.equal(r, rm) and E == Em
+
+
 def test_server_memory_admission(gt, cuda):
     Sv, specs = _small_stream(6, seed=4)
     big = max(s.footprint_bytes() for s in specs)
