@@ -381,24 +381,41 @@ template <int D> __device__ inline double collapse_H(const double *H, int P, int
     return s;
 }
 
+// Cross-workgroup traffic of a group (per-angle records, energy changes,
+// barrier words) uses device-scope relaxed accesses: they are performed at the
+// device's point of coherence (sc1), so no L2 write-back or invalidation is
+// needed -- those would also evict the cached residuals of every other fit on
+// the same XCD.  Ordering comes from waiting for completion (s_waitcnt) before
+// a barrier is signalled, and from issuing the reads after it is observed.
+__device__ inline void xput(double *p, double v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline double xget(const double *p)
+{
+    return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline unsigned xgetu(unsigned *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline void xputu(unsigned *p, unsigned v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline void wait_all_memory() { __builtin_amdgcn_s_waitcnt(0); }
+
 // Device-wide barrier of the G workgroups of one slot, called by thread 0 of
-// each after its workgroup has published its writes.  bar[0] counts arrivals,
-// bar[1] is the generation, bar[2] the abort flag.  The wait is bounded:
-// after FIT_WAIT_TICKS the slot is aborted.  Returns false if aborted.
+// each after every thread of its workgroup has completed its xput stores.
+// bar[0] counts arrivals, bar[1] is the generation, bar[2] the abort flag.
+// The wait is bounded: after FIT_WAIT_TICKS the slot is aborted.  Returns
+// false if aborted.
 __device__ inline bool group_barrier(unsigned *bar, int G, unsigned &gen)
 {
-    if (__hip_atomic_load(bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return false;
+    if (xgetu(bar + 2) != 0u) return false;
     const unsigned target = gen + 1u;
-    const unsigned old = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned old = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (old == (unsigned)G - 1u) {
-        __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(bar + 1, target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        xputu(bar, 0u);
+        wait_all_memory();  // the reset lands before anyone can pass and arrive again
+        xputu(bar + 1, target);
     } else {
         const long long t0 = wall_clock64();
-        while (__hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != target) {
-            if (__hip_atomic_load(bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return false;
+        while (xgetu(bar + 1) != target) {
+            if (xgetu(bar + 2) != 0u) return false;
             if (wall_clock64() - t0 > FIT_WAIT_TICKS) {
-                __hip_atomic_store(bar + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                xputu(bar + 2, 1u);
                 return false;
             }
             __builtin_amdgcn_s_sleep(2);
@@ -457,7 +474,7 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         c.E = S.E;
         c.updates = S.updates;
         c.accepted = S.accepted;
-        L.gen = (G > 1) ? __hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+        L.gen = (G > 1) ? xgetu(bar + 1) : 0u;
         L.abort_ = 0;
     }
     __syncthreads();
@@ -473,15 +490,12 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
     }
     // Publish this workgroup's global writes and meet the other members.
 #define GT_FIT_MEET()                                                          \
+    wait_all_memory();                                                         \
+    __syncthreads();                                                           \
     if (G > 1) {                                                               \
-        __threadfence();                                                       \
-        __syncthreads();                                                       \
         if (tid == 0 && !group_barrier(bar, G, L.gen)) L.abort_ = 1;           \
         __syncthreads();                                                       \
         if (L.abort_) break;                                                   \
-        __threadfence();                                                       \
-    } else {                                                                   \
-        __syncthreads();                                                       \
     }
 
     const int my_budget = quota > 0 ? min(budget, quota) : (S.quota > 0 ? min(budget, S.quota) : budget);
@@ -560,7 +574,7 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                 for (int e = lane; e < NV; e += 64) {
                     int i = e, l = e;
                     if (e >= P) upper_ij(e - P, P, i, l);
-                    prec[(size_t)a * NVS + e] = (e < P) ? fin_grad<D>(W.F, e) : fin_hess<D>(W.F, i, l);
+                    xput(prec + (size_t)a * NVS + e, (e < P) ? fin_grad<D>(W.F, e) : fin_hess<D>(W.F, i, l));
                 }
                 wave_sync();  // W is reused by the wave's next angle
             }
@@ -577,8 +591,8 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                 const int e = wave + q * FIT_WAVES;
                 double t = 0.0;
                 if (e < NV) {
-                    if (lane < nA) t = prec[(size_t)lane * NVS + e];
-                    if (lane + 64 < nA) t += prec[(size_t)(lane + 64) * NVS + e];
+                    if (lane < nA) t = xget(prec + (size_t)lane * NVS + e);
+                    if (lane + 64 < nA) t += xget(prec + (size_t)(lane + 64) * NVS + e);
                 }
                 v[q] = t;
             }
@@ -683,15 +697,15 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
             float *Ra = S.resid + (size_t)a * slab;
             double acc = pair_delta4<K>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, Ra, n1, lane, 0);
             acc = wave_sum_f64(acc);
-            if (lane == 0) dEa[a] = acc;
+            if (lane == 0) xput(dEa + a, acc);
         }
         GT_FIT_MEET();
         GT_FIT_TICK(5);
         // ---- phase G: accept / reject, step size, flags -------------------
         if (wave == 0) {
             double dE = 0.0;
-            if (lane < nA) dE = dEa[lane];
-            if (lane + 64 < nA) dE += dEa[lane + 64];
+            if (lane < nA) dE = xget(dEa + lane);
+            if (lane + 64 < nA) dE += xget(dEa + lane + 64);
             dE = wave_sum_f64(dE);
             if (lane == 0) {
                 FitCtl &c = L.c;
@@ -759,7 +773,7 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
 #undef GT_FIT_MEET
 #undef GT_FIT_TICK
     if (tid == 0) {
-        const bool failed = L.abort_ || (G > 1 && __hip_atomic_load(bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        const bool failed = L.abort_ || (G > 1 && xgetu(bar + 2) != 0u);
         if (member == 0) {
             const FitCtl &c = L.c;
             S.it = c.it;

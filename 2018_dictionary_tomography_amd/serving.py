@@ -106,13 +106,17 @@ def work_per_update(spec):
     return spec.nA * spec.S * spec.S + 5.3e5
 
 
-# Fixed cost of one grouped atom update (barriers, redundant solve), pixel units.
-GROUP_UPDATE_OVERHEAD = 2.0e5
+# Fixed cost of one atom update advanced by a group (setup, contraction,
+# barriers, the solve repeated in every member), pixel units.
+GROUP_UPDATE_OVERHEAD = 4.1e5
 
 
 def update_cost(spec, group):
-    """Relative cost of one atom update advanced by `group` workgroups."""
-    return work_per_update(spec) / group + (GROUP_UPDATE_OVERHEAD if group > 1 else 0.0)
+    """Relative cost (time) of one atom update advanced by `group` workgroups:
+    the pixel work is shared, the fixed part is not."""
+    if group <= 1:
+        return work_per_update(spec)
+    return spec.nA * spec.S * spec.S / group + GROUP_UPDATE_OVERHEAD
 
 
 def allocate_groups(specs, remaining, capacity):
@@ -148,6 +152,9 @@ class ServeStats:
     queued_peak: int = 0
     refused: int = 0
     failed: int = 0
+    admit_s: float = 0.0   # host + device time spent admitting requests
+    launch_s: float = 0.0  # time from each launch to its completion
+    trace: list = field(default_factory=list)  # per launch: (start s, active fits, workgroups used, s, updates)
     results: dict = field(default_factory=dict)
 
     def summary(self):
@@ -163,6 +170,8 @@ class ServeStats:
             "peak_queue": self.queued_peak,
             "refused": self.refused,
             "failed": self.failed,
+            "admit_s": self.admit_s,
+            "launch_s": self.launch_s,
         }
 
 
@@ -237,6 +246,7 @@ class FitServer:
                 b += 1
         self.blocks_host.copy_(torch.from_numpy(rows))
         self.blocks.copy_(self.blocks_host)
+        self.used_blocks = b
         return dict(zip(act, groups))
 
     def run(self, specs, time_scale=1.0, log=None, payloads=None):
@@ -267,6 +277,7 @@ class FitServer:
             st.queued_peak = max(st.queued_peak, len(queue))
             # admit FIFO while a slot and memory are free
             free = [i for i in range(self.max_slots) if self.slots[i] is None]
+            ta = time.perf_counter()
             while queue and free and reserved + queue[0].footprint_bytes() <= self.hbm_budget:
                 spec = queue.pop(0)
                 i = free.pop(0)
@@ -278,6 +289,7 @@ class FitServer:
                 reserved += spec.footprint_bytes()
                 active += 1
                 dirty = True
+            st.admit_s += time.perf_counter() - ta
             st.admitted_bytes_peak = max(st.admitted_bytes_peak, reserved)
             if active == 0:
                 if nxt < len(pending):
@@ -286,11 +298,18 @@ class FitServer:
             if dirty:
                 self._plan(progress)
                 dirty = False
+            tl = time.perf_counter()
             self._launch()
             st.launches += 1
             torch.cuda.synchronize(self.dev)
-            for i, s in self._read_active():
+            dt = time.perf_counter() - tl
+            st.launch_s += dt
+            before = sum(progress.values())
+            rows = self._read_active()
+            for i, s in rows:
                 progress[i] = s.updates
+            st.trace.append((tl - t0, active, self.used_blocks, dt, sum(progress.values()) - before))
+            for i, s in rows:
                 if not (s.done or not s.active):
                     continue
                 spec, req, t_admit = self.slots[i]
