@@ -35,6 +35,8 @@ PEAK_FP32_TFLOPS = 157.3      # MI355X published FP32 (vector, packed / matrix)
 PEAK_FP32_NONPACKED = 78.65   # 256 CU x 4 SIMD x 16 lanes x 2 flop x 2.4 GHz
 PEAK_TRANS_PER_S = 256 * 4 * 16 * 2.4e9 / 4  # quarter-rate transcendental
 PEAK_HBM_GBS = 8000.0
+# moments kernel, 1000 atoms x 61 angles x 128^2: 34.3 MiB read + 7.4 MiB written per launch
+MOMENTS_HBM_BYTES = int((34.3 + 7.4) * 2**20)
 
 
 def parse():
@@ -271,7 +273,11 @@ def main():
         "avg_kernel_ms": round(mom_ms, 5),
         "in_support_pairs": P_in,
         "flops_per_pair": FLOPS_L2,
-        "traffic": None,
+        # HBM bytes per launch of the moments kernel from the two rocprofv3 counter
+        # passes (FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, both scaled against a
+        # 256 MiB copy), recorded with the command in the profiles directory
+        "traffic": MOMENTS_HBM_BYTES,
+        "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, profiles/step_1000atoms_61x128x128_traffic.md",
         "forward_kernel": {"avg_ms": round(fwd_ms, 5), "achieved_tflops": round(P_in * FLOPS_FWD / (fwd_ms * 1e-3) / 1e12, 3),
                            "trans_frac": round(P_in / (fwd_ms * 1e-3) / PEAK_TRANS_PER_S, 4)},
         "kernel_ms": {k: round(v[0] / max(v[1], 1), 5) for k, v in kt.items()},
