@@ -445,7 +445,7 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
     if (sid < 0 || sid >= nslots || G < 1 || member < 0 || member >= G) return;
     gt_fit_slot &S = slots[sid];
     if (!S.active || S.done) return;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int iso = S.iso;
     const int PI = iso ? 2 + D : P;  // parameters of the solve
     const int NRs = iso ? 1 : NR;

@@ -199,6 +199,36 @@ struct ColChunk {
 // S1[i, j] += A_i dp1^j, S2[i, j] += B_i dp1^j.  Same sums as pair_moments.
 // UNR rows are loaded per step (more rows in flight hide load latency when
 // few waves share a SIMD).
+// One pixel of pair_moments_rows: branch-free cutoff (outside the support
+// E = 0 and every accumulation adds an exact zero, so the sums equal those of
+// skipping the pixel).
+template <int MODE>
+__device__ inline void moment_pixel(const PairRec &rec, float cv, float x, float rowlin, float colt, float *A,
+                                    float *B, float *acc)
+{
+    const float q = fmaf(fmaf(rec.q[0], x, rowlin), x, colt);
+    const bool in = q >= (float)KAPPA;
+    const float E = in ? __builtin_amdgcn_exp2f(q) : 0.0f;
+    float w1;
+    if (MODE == GT_MODE_L2) {
+        const float res = fmaf(rec.alpha, E, -cv);
+        w1 = res * E;
+        acc[30] = fmaf(in ? res : 0.0f, res, acc[30]);
+        acc[31] = fmaf(in ? cv : 0.0f, cv, acc[31]);
+    } else {
+        w1 = cv * E;
+    }
+    const float x2 = x * x;
+    const float xp[5] = {1.0f, x, x2, x2 * x, x2 * x2};
+#pragma unroll
+    for (int i = 0; i < 5; i++) A[i] = fmaf(w1, xp[i], A[i]);
+    if (MODE != GT_MODE_LINEAR) {
+        const float w2 = E * E;
+#pragma unroll
+        for (int i = 0; i < 5; i++) B[i] = fmaf(w2, xp[i], B[i]);
+    }
+}
+
 template <int MODE, int UNR = 4>
 __device__ inline void pair_moments_rows(const PairRec &rec, const float *p0, const float *p1,
                                          const float *__restrict__ Ca, int n1, int lane, float *acc)
@@ -206,7 +236,11 @@ __device__ inline void pair_moments_rows(const PairRec &rec, const float *p0, co
     const int W = rec.hi[1] - rec.lo[1] + 1;
     const int H = rec.hi[0] - rec.lo[0] + 1;
     if (W <= 0 || H <= 0) return;
-    const float kap = (float)KAPPA;
+    // 32-bit byte offsets from the wave-uniform bases (scalar-base loads, one
+    // multiply-add per pixel); detector slabs are far below 4 GiB
+    const char *cbase = reinterpret_cast<const char *>(Ca);
+    const char *pbase = reinterpret_cast<const char *>(p0);
+    const unsigned rowb = 4u * (unsigned)n1;
     for (int cb = 0; cb < W; cb += 64) {
         ColChunk cc;
         cc.init(lane, W, cb);
@@ -216,43 +250,27 @@ __device__ inline void pair_moments_rows(const PairRec &rec, const float *p0, co
         const float rowlin = fmaf(rec.q[1], dp1, rec.q[3]);
         const float colt = fmaf(fmaf(rec.q[2], dp1, rec.q[4]), dp1, rec.q[5]);
         float A[5] = {0, 0, 0, 0, 0}, B[5] = {0, 0, 0, 0, 0};
-        const float *col = Ca + rec.lo[1] + cb + cc.c;
-        for (int r0 = cc.ro; r0 < H; r0 += UNR * cc.R) {
+        const unsigned cb0 = 4u * ((unsigned)rec.lo[0] * (unsigned)n1 + (unsigned)i1);
+        const unsigned pb0 = 4u * (unsigned)rec.lo[0];
+        int r0 = cc.ro;
+        // full groups of UNR rows, all inside the box
+        for (; r0 + (UNR - 1) * cc.R < H; r0 += UNR * cc.R) {
             float cv[UNR], dp0[UNR];
-            bool ok[UNR];
 #pragma unroll
             for (int u = 0; u < UNR; u++) {
-                const int r = r0 + u * cc.R;
-                ok[u] = r < H;
-                const int i0 = rec.lo[0] + (ok[u] ? r : 0);
-                cv[u] = col[(size_t)i0 * n1];
-                dp0[u] = (p0[i0] - rec.c_hi[0]) - rec.c_lo[0];
+                const unsigned r = (unsigned)(r0 + u * cc.R);
+                cv[u] = *reinterpret_cast<const float *>(cbase + (cb0 + r * rowb));
+                dp0[u] = (*reinterpret_cast<const float *>(pbase + (pb0 + 4u * r)) - rec.c_hi[0]) - rec.c_lo[0];
             }
 #pragma unroll
-            for (int u = 0; u < UNR; u++) {
-                const float q = fmaf(fmaf(rec.q[0], dp0[u], rowlin), dp0[u], colt);
-                if (ok[u] && q >= kap) {
-                    const float E = __builtin_amdgcn_exp2f(q);
-                    float w1;
-                    if (MODE == GT_MODE_L2) {
-                        const float res = fmaf(rec.alpha, E, -cv[u]);
-                        w1 = res * E;
-                        acc[30] = fmaf(res, res, acc[30]);
-                        acc[31] = fmaf(cv[u], cv[u], acc[31]);
-                    } else {
-                        w1 = cv[u] * E;
-                    }
-                    const float x = dp0[u], x2 = x * x;
-                    const float xp[5] = {1.0f, x, x2, x2 * x, x2 * x2};
-#pragma unroll
-                    for (int i = 0; i < 5; i++) A[i] = fmaf(w1, xp[i], A[i]);
-                    if (MODE != GT_MODE_LINEAR) {
-                        const float w2 = E * E;
-#pragma unroll
-                        for (int i = 0; i < 5; i++) B[i] = fmaf(w2, xp[i], B[i]);
-                    }
-                }
-            }
+            for (int u = 0; u < UNR; u++) moment_pixel<MODE>(rec, cv[u], dp0[u], rowlin, colt, A, B, acc);
+        }
+        // remaining rows one at a time (same row order)
+        for (; r0 < H; r0 += cc.R) {
+            const unsigned r = (unsigned)r0;
+            const float cv = *reinterpret_cast<const float *>(cbase + (cb0 + r * rowb));
+            const float x = (*reinterpret_cast<const float *>(pbase + (pb0 + 4u * r)) - rec.c_hi[0]) - rec.c_lo[0];
+            moment_pixel<MODE>(rec, cv, x, rowlin, colt, A, B, acc);
         }
         const float y2 = dp1 * dp1;
         const float yp[5] = {1.0f, dp1, y2, y2 * dp1, y2 * y2};

@@ -188,7 +188,8 @@ __global__ __launch_bounds__(256) void moments_kernel(const PairRec *__restrict_
                                                       float *__restrict__ mom)
 {
     __shared__ float red[4][64][MOM_STRIDE + 1];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wave index is uniform: scalar pair index, record and slab base
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const long pair = (long)blockIdx.x * 4 + wave;
     const bool valid = pair < (long)N * nA;
     float acc[MOM_STRIDE];
