@@ -157,7 +157,7 @@ __host__ __device__ inline void pair_alg(const PairGeom<D> &g, const double *pst
 }
 
 // FP32 record of one (atom, angle) pair, consumed by the pixel kernels.
-struct PairRec {
+struct alignas(16) PairRec {
     float alpha;
     float c_hi[2], c_lo[2];  // expansion point p* = c_hi + c_lo
     float q[6];              // kappa * m coefficients (see pair_q)

@@ -3,8 +3,8 @@
 // Pipeline (all stream-ordered, no host synchronisation):
 //   setup    one thread per (atom, angle): FP64 detector Gaussian -> PairRec
 //   forward  one block per (angle, 16x64 pixel tile): stages the angle's
-//            records that touch the tile in LDS, 4 rows per wave, 4 pixels
-//            per lane, Kahan sums
+//            records that touch the tile in LDS; each of 8 waves owns an
+//            8x16 rectangle (2 pixels per lane), Kahan sums
 //   moments  one wave per (atom, angle): FP32 separable pixel moments over
 //            the footprint box, LDS-transposed wave reduction, no atomics
 //   finish   one wave per atom, one lane per angle: FP64 moment -> (f, g, H)
@@ -18,7 +18,7 @@
 
 namespace gt {
 
-constexpr int TILE_ROWS = 16, TILE_COLS = 64, ROWS_PER_LANE = 4;
+constexpr int TILE_ROWS = 16, TILE_COLS = 64;
 constexpr int CHUNK = 256;
 constexpr int MOM_STRIDE = 32;  // floats per (atom, angle) moment record
 constexpr int MODE_LINEAR = GT_MODE_LINEAR, MODE_L2 = GT_MODE_L2, MODE_RESID = GT_MODE_RESID;
@@ -53,6 +53,7 @@ template <int D> __global__ __launch_bounds__(256) void setup_kernel(gt_atoms A,
 // Deterministic block-wide compaction of the records that touch a box
 // [r_lo, r_hi] x [c_lo, c_hi]; returns the list length.  The list keeps atom
 // order, so the per-pixel summation order does not depend on timing.
+template <int NW>
 __device__ inline int stage_chunk(const PairRec *__restrict__ ra, int base, int N, int r_lo, int r_hi, int c_lo,
                                   int c_hi, PairRec *list, int *wcnt)
 {
@@ -74,7 +75,7 @@ __device__ inline int stage_chunk(const PairRec *__restrict__ ra, int base, int 
     __syncthreads();
     int off = 0, total = 0;
 #pragma unroll
-    for (int w = 0; w < 4; w++) {
+    for (int w = 0; w < NW; w++) {
         if (w < wave) off += wcnt[w];
         total += wcnt[w];
     }
@@ -92,56 +93,151 @@ __device__ inline int stage_chunk(const PairRec *__restrict__ ra, int base, int 
 // ---------------------------------------------------------------------------
 // forward, 2D detectors (3D and 2.5D geometries)
 // ---------------------------------------------------------------------------
-// Block tile 16 x 64 pixels; wave w owns rows 4w .. 4w+3 of it and lane l
-// column l.  Each of a lane's 4 rows is shared by the whole wave, so the
-// exp/accumulate branch below is taken row-coherently (square wave tiles were
-// measured slower for that reason).
-__global__ __launch_bounds__(256) void forward3_kernel(const PairRec *__restrict__ recs, int N, int n0, int n1,
-                                                       const float *__restrict__ p0, const float *__restrict__ p1,
-                                                       float *__restrict__ R)
+// Block tile 16 x 64 pixels, 8 waves; wave w owns the 8 x 16 rectangle
+// rows 8 (w / 4) .. +7, columns 16 (w % 4) .. +15, and lane l column l % 16
+// of it, rows l / 16 and l / 16 + 4 (2 pixels per lane).  Staging keeps the
+// records that touch the tile in LDS and, for every wave, the list of those
+// that touch its rectangle (both in atom order, so each pixel's summation
+// order is fixed): a wave then walks only records that reach its pixels.
+constexpr int F3_WAVES = 8, F3_CHUNK = F3_WAVES * 64;
+
+__device__ inline int stage_rects(const PairRec *__restrict__ ra, int base, int N, int row0, int col0, int thi0,
+                                  int thi1, PairRec *list, unsigned short (*wl)[F3_CHUNK], int (*cnt)[F3_WAVES],
+                                  int *wlen)
 {
-    __shared__ PairRec list[CHUNK];
-    __shared__ int wcnt[4];
+    const int lane = threadIdx.x & 63, hw = threadIdx.x >> 6;
+    const int j = base + (int)threadIdx.x;
+    int4 v0 = make_int4(0, 0, 0, 0), v1 = v0, v2 = v0, v3 = v0;
+    bool hit = false;
+    unsigned mask = 0;
+    if (j < N) {
+        const int4 *src = reinterpret_cast<const int4 *>(ra + j);
+        v0 = src[0];
+        v1 = src[1];
+        v2 = src[2];
+        v3 = src[3];
+        // v3 = (lo[0], lo[1], hi[0], hi[1])
+        hit = v3.x <= thi0 && v3.z >= row0 && v3.y <= thi1 && v3.w >= col0;
+        if (hit)
+#pragma unroll
+            for (int w = 0; w < F3_WAVES; w++) {
+                const int r = row0 + 8 * (w >> 2), c = col0 + 16 * (w & 3);
+                if (v3.x <= r + 7 && v3.z >= r && v3.y <= c + 15 && v3.w >= c) mask |= 1u << w;
+            }
+    }
+    // cnt[k][hw]: hits of this hardware wave, k = 0 the tile list, k = 1 + w wave w's list
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const unsigned long long mt = __ballot(hit);
+    if (lane == 0) cnt[0][hw] = __popcll(mt);
+    unsigned long long mw[F3_WAVES];
+#pragma unroll
+    for (int w = 0; w < F3_WAVES; w++) {
+        mw[w] = __ballot((mask >> w) & 1u);
+        if (lane == 0) cnt[1 + w][hw] = __popcll(mw[w]);
+    }
+    __syncthreads();
+    int off = 0, total = 0;
+#pragma unroll
+    for (int h = 0; h < F3_WAVES; h++) {
+        if (h < hw) off += cnt[0][h];
+        total += cnt[0][h];
+    }
+    const int pos = off + __popcll(mt & below);
+    if (hit) {
+        int4 *dst = reinterpret_cast<int4 *>(list + pos);
+        dst[0] = v0;
+        dst[1] = v1;
+        dst[2] = v2;
+        dst[3] = v3;
+    }
+#pragma unroll
+    for (int w = 0; w < F3_WAVES; w++) {
+        int o = 0, t = 0;
+#pragma unroll
+        for (int h = 0; h < F3_WAVES; h++) {
+            if (h < hw) o += cnt[1 + w][h];
+            t += cnt[1 + w][h];
+        }
+        if ((mask >> w) & 1u) wl[w][o + __popcll(mw[w] & below)] = (unsigned short)pos;
+        if (threadIdx.x == 0) wlen[w] = t;
+    }
+    __syncthreads();
+    return total;
+}
+
+// One record's contribution to a lane's two pixels.  The record arrives as
+// its first 48 bytes (alpha, c_hi, c_lo, q): f0 = (alpha, c_hi0, c_hi1,
+// c_lo0), f1 = (c_lo1, q0, q1, q2), f2 = (q3, q4, q5, pad).  Branch-free
+// cutoff: every record in a wave's list reaches its rectangle.
+__device__ inline void f3_pixels(const float4 &f0, const float4 &f1, const float4 &f2, float pc, float prA,
+                                 float prB, float &sA, float &cA, float &sB, float &cB)
+{
+    const float kap = (float)KAPPA;
+    const float dp1 = (pc - f0.z) - f1.x;
+    const float rowlin = fmaf(f1.z, dp1, f2.x);
+    const float col = fmaf(fmaf(f1.w, dp1, f2.y), dp1, f2.z);
+    const float dA = (prA - f0.y) - f0.w;
+    const float dB = (prB - f0.y) - f0.w;
+    const float qA = fmaf(fmaf(f1.y, dA, rowlin), dA, col);
+    const float qB = fmaf(fmaf(f1.y, dB, rowlin), dB, col);
+    const float eA = __builtin_amdgcn_exp2f(qA), eB = __builtin_amdgcn_exp2f(qB);
+    kahan_add(sA, cA, qA >= kap ? f0.x * eA : 0.0f);
+    kahan_add(sB, cB, qB >= kap ? f0.x * eB : 0.0f);
+}
+
+__global__ __launch_bounds__(F3_WAVES * 64) void forward3_kernel(const PairRec *__restrict__ recs, int N, int n0,
+                                                                 int n1, const float *__restrict__ p0,
+                                                                 const float *__restrict__ p1, float *__restrict__ R)
+{
+    __shared__ PairRec list[F3_CHUNK];
+    __shared__ alignas(16) unsigned short wl[F3_WAVES][F3_CHUNK];  // 8-byte reads of 4 indices
+    __shared__ int cnt[F3_WAVES + 1][F3_WAVES];
+    __shared__ int wlen[F3_WAVES];
     const int a = blockIdx.y;
     const int tilesC = (n1 + TILE_COLS - 1) / TILE_COLS;
     const int tr = blockIdx.x / tilesC, tc = blockIdx.x - tr * tilesC;
     const int row0 = tr * TILE_ROWS, col0 = tc * TILE_COLS;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int i1 = col0 + lane;
-    const int r0 = row0 + wave * ROWS_PER_LANE;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wr0 = row0 + 8 * (wave >> 2), wc0 = col0 + 16 * (wave & 3);
+    const int i1 = wc0 + (lane & 15);
+    const int rA = wr0 + (lane >> 4), rB = rA + 4;
     const int thi0 = min(row0 + TILE_ROWS - 1, n0 - 1), thi1 = min(col0 + TILE_COLS - 1, n1 - 1);
     const float pc = p1[min(i1, n1 - 1)];
-    float pr[ROWS_PER_LANE], s[ROWS_PER_LANE], c[ROWS_PER_LANE];
-#pragma unroll
-    for (int k = 0; k < ROWS_PER_LANE; k++) {
-        pr[k] = p0[min(r0 + k, n0 - 1)];
-        s[k] = 0.0f;
-        c[k] = 0.0f;
-    }
+    const float prA = p0[min(rA, n0 - 1)], prB = p0[min(rB, n0 - 1)];
+    float sA = 0.0f, cA = 0.0f, sB = 0.0f, cB = 0.0f;
     const PairRec *ra = recs + (size_t)a * N;
-    const float kap = (float)KAPPA;
-    for (int base = 0; base < N; base += CHUNK) {
-        const int total = stage_chunk(ra, base, N, row0, thi0, col0, thi1, list, wcnt);
-        for (int e = 0; e < total; e++) {
-            const PairRec &q = list[e];
-            // wave-uniform cull against the wave's 4 rows
-            if (q.lo[0] > r0 + ROWS_PER_LANE - 1 || q.hi[0] < r0) continue;
-            const float dp1 = (pc - q.c_hi[1]) - q.c_lo[1];
-            const float rowlin = fmaf(q.q[1], dp1, q.q[3]);
-            const float col = fmaf(fmaf(q.q[2], dp1, q.q[4]), dp1, q.q[5]);
+    for (int base = 0; base < N; base += F3_CHUNK) {
+        stage_rects(ra, base, N, row0, col0, thi0, thi1, list, wl, cnt, wlen);
+        const int len = wlen[wave];
+        const unsigned short *wlw = wl[wave];
+        int e = 0;
+        // 4 records per step: their list indices (one 8-byte read) and records
+        // (three 16-byte reads each) are all read before any is used; the
+        // pixel sums still take the records in list order
+        for (; e + 4 <= len; e += 4) {
+            const uint2 iv = *reinterpret_cast<const uint2 *>(wlw + e);
+            const int idx[4] = {(int)(iv.x & 0xffffu), (int)(iv.x >> 16), (int)(iv.y & 0xffffu), (int)(iv.y >> 16)};
+            float4 f0[4], f1[4], f2[4];
 #pragma unroll
-            for (int k = 0; k < ROWS_PER_LANE; k++) {
-                const float dp0 = (pr[k] - q.c_hi[0]) - q.c_lo[0];
-                const float qq = fmaf(fmaf(q.q[0], dp0, rowlin), dp0, col);
-                if (qq >= kap) kahan_add(s[k], c[k], q.alpha * __builtin_amdgcn_exp2f(qq));
+            for (int u = 0; u < 4; u++) {
+                const float4 *rp = reinterpret_cast<const float4 *>(list + idx[u]);
+                f0[u] = rp[0];
+                f1[u] = rp[1];
+                f2[u] = rp[2];
             }
+#pragma unroll
+            for (int u = 0; u < 4; u++) f3_pixels(f0[u], f1[u], f2[u], pc, prA, prB, sA, cA, sB, cB);
+        }
+        for (; e < len; e++) {
+            const float4 *rp = reinterpret_cast<const float4 *>(list + wlw[e]);
+            f3_pixels(rp[0], rp[1], rp[2], pc, prA, prB, sA, cA, sB, cB);
         }
         __syncthreads();
     }
-    if (i1 < n1)
-#pragma unroll
-        for (int k = 0; k < ROWS_PER_LANE; k++)
-            if (r0 + k < n0) R[((size_t)a * n0 + r0 + k) * n1 + i1] = s[k];
+    if (i1 < n1) {
+        if (rA < n0) R[((size_t)a * n0 + rA) * n1 + i1] = sA;
+        if (rB < n0) R[((size_t)a * n0 + rB) * n1 + i1] = sB;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -163,7 +259,7 @@ __global__ __launch_bounds__(256) void forward2_kernel(const PairRec *__restrict
     const PairRec *ra = recs + (size_t)a * N;
     const float kap = (float)KAPPA;
     for (int base = 0; base < N; base += CHUNK) {
-        const int total = stage_chunk(ra, base, N, row0, thi, 0, 0, list, wcnt);
+        const int total = stage_chunk<4>(ra, base, N, row0, thi, 0, 0, list, wcnt);
         for (int e = 0; e < total; e++) {
             const PairRec &q = list[e];
             if (q.lo[0] > w1 || q.hi[0] < w0) continue;
@@ -393,7 +489,7 @@ int launch_forward(const gt_atoms *A, const gt_geom *G, const PairRec *recs, flo
     prof_begin(st, GT_K_FORWARD);
     if (G->dim == 3) {
         const int tiles = ((G->n0 + TILE_ROWS - 1) / TILE_ROWS) * ((G->n1 + TILE_COLS - 1) / TILE_COLS);
-        hipLaunchKernelGGL(forward3_kernel, dim3(tiles, G->nA), dim3(256), 0, st, recs, A->n, G->n0, G->n1, G->p0,
+        hipLaunchKernelGGL(forward3_kernel, dim3(tiles, G->nA), dim3(F3_WAVES * 64), 0, st, recs, A->n, G->n0, G->n1, G->p0,
                            G->p1, R);
     } else {
         const int tiles = (G->n0 + 255) / 256;
