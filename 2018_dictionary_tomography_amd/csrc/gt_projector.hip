@@ -187,7 +187,8 @@ __device__ inline void f3_pixels(const float4 &f0, const float4 &f1, const float
 
 __global__ __launch_bounds__(F3_WAVES * 64) void forward3_kernel(const PairRec *__restrict__ recs, int N, int n0,
                                                                  int n1, const float *__restrict__ p0,
-                                                                 const float *__restrict__ p1, float *__restrict__ R)
+                                                                 const float *__restrict__ p1, float *__restrict__ R,
+                                                                 const float *__restrict__ sub)
 {
     __shared__ PairRec list[F3_CHUNK];
     __shared__ alignas(16) unsigned short wl[F3_WAVES][F3_CHUNK];  // 8-byte reads of 4 indices
@@ -234,9 +235,11 @@ __global__ __launch_bounds__(F3_WAVES * 64) void forward3_kernel(const PairRec *
         }
         __syncthreads();
     }
+    // R = projection, or projection - sub (the residual) when sub is given
     if (i1 < n1) {
-        if (rA < n0) R[((size_t)a * n0 + rA) * n1 + i1] = sA;
-        if (rB < n0) R[((size_t)a * n0 + rB) * n1 + i1] = sB;
+        const size_t oA = ((size_t)a * n0 + rA) * n1 + i1, oB = ((size_t)a * n0 + rB) * n1 + i1;
+        if (rA < n0) R[oA] = sub ? sA - sub[oA] : sA;
+        if (rB < n0) R[oB] = sub ? sB - sub[oB] : sB;
     }
 }
 
@@ -244,7 +247,8 @@ __global__ __launch_bounds__(F3_WAVES * 64) void forward3_kernel(const PairRec *
 // forward, 1D detectors (2D geometry)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void forward2_kernel(const PairRec *__restrict__ recs, int N, int n0,
-                                                       const float *__restrict__ p0, float *__restrict__ R)
+                                                       const float *__restrict__ p0, float *__restrict__ R,
+                                                       const float *__restrict__ sub)
 {
     __shared__ PairRec list[CHUNK];
     __shared__ int wcnt[4];
@@ -269,7 +273,7 @@ __global__ __launch_bounds__(256) void forward2_kernel(const PairRec *__restrict
         }
         __syncthreads();
     }
-    if (i0 < n0) R[(size_t)a * n0 + i0] = s;
+    if (i0 < n0) R[(size_t)a * n0 + i0] = sub ? s - sub[(size_t)a * n0 + i0] : s;
 }
 
 // ---------------------------------------------------------------------------
@@ -309,6 +313,14 @@ __global__ __launch_bounds__(256) void moments_kernel(const PairRec *__restrict_
         s += __shfl_xor(s, 32);
         if (lane < 32) mom[pair * MOM_STRIDE + lane] = s;
     }
+}
+
+// f = 1/2 sum over angles of the per-angle FP64 sums of squares (one thread).
+__global__ void half_total_kernel(const double *__restrict__ per, int n, double *__restrict__ f)
+{
+    double s = 0.0;
+    for (int a = 0; a < n; a++) s += per[a];
+    f[0] = 0.5 * s;
 }
 
 // FP64 sum of C^2 per angle (the L2 value of the pixels outside every footprint).
@@ -482,18 +494,34 @@ int launch_fill(float *p, size_t n, float v, hipStream_t st)
 }
 
 // Forward projection from precomputed records (used by the fitters too).
-int launch_forward(const gt_atoms *A, const gt_geom *G, const PairRec *recs, float *R, hipStream_t st)
+__global__ void negate_kernel(float *__restrict__ R, const float *__restrict__ sub, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        R[i] = 0.0f - sub[i];
+}
+
+// Forward projection from precomputed records (used by the fitters too);
+// with sub != NULL the result is the residual projection - sub.
+int launch_forward(const gt_atoms *A, const gt_geom *G, const PairRec *recs, float *R, hipStream_t st,
+                   const float *sub)
 {
     const size_t nout = (size_t)G->nA * G->n0 * G->n1;
-    if (A->n == 0) return launch_fill(R, nout, 0.0f, st);
+    if (A->n == 0) {
+        if (!sub) return launch_fill(R, nout, 0.0f, st);
+        if (nout == 0) return 0;
+        hipLaunchKernelGGL(negate_kernel, dim3((unsigned)std::min<size_t>((nout + 255) / 256, 65535)), dim3(256), 0,
+                           st, R, sub, nout);
+        GT_HIP(hipGetLastError());
+        return 0;
+    }
     prof_begin(st, GT_K_FORWARD);
     if (G->dim == 3) {
         const int tiles = ((G->n0 + TILE_ROWS - 1) / TILE_ROWS) * ((G->n1 + TILE_COLS - 1) / TILE_COLS);
-        hipLaunchKernelGGL(forward3_kernel, dim3(tiles, G->nA), dim3(F3_WAVES * 64), 0, st, recs, A->n, G->n0, G->n1, G->p0,
-                           G->p1, R);
+        hipLaunchKernelGGL(forward3_kernel, dim3(tiles, G->nA), dim3(F3_WAVES * 64), 0, st, recs, A->n, G->n0, G->n1,
+                           G->p0, G->p1, R, sub);
     } else {
         const int tiles = (G->n0 + 255) / 256;
-        hipLaunchKernelGGL(forward2_kernel, dim3(tiles, G->nA), dim3(256), 0, st, recs, A->n, G->n0, G->p0, R);
+        hipLaunchKernelGGL(forward2_kernel, dim3(tiles, G->nA), dim3(256), 0, st, recs, A->n, G->n0, G->p0, R, sub);
     }
     prof_end(st, GT_K_FORWARD);
     GT_HIP(hipGetLastError());
@@ -566,7 +594,7 @@ extern "C" int gt_forward(const gt_atoms *atoms, const gt_geom *geom, float *R, 
     hipStream_t st = (hipStream_t)stream;
     PairRec *recs = (PairRec *)ws;
     if ((rc = launch_setup(atoms, geom, recs, st))) return rc;
-    return launch_forward(atoms, geom, recs, R, st);
+    return launch_forward(atoms, geom, recs, R, st, nullptr);
 }
 
 extern "C" size_t gt_derivs_workspace(const gt_atoms *atoms, const gt_geom *geom, int mode)
@@ -599,4 +627,35 @@ extern "C" int gt_derivs(const gt_atoms *atoms, const gt_geom *geom, const float
     double *sumC2 = (double *)(w + rec_bytes(atoms, geom) + mom_bytes(atoms, geom));
     if ((rc = launch_setup(atoms, geom, recs, st))) return rc;
     return launch_derivs(atoms, geom, recs, mom, sumC2, C, mode, order > 1, f, g, H, st);
+}
+
+extern "C" int gt_forward_derivs(const gt_atoms *atoms, const gt_geom *geom, const float *data, int order, float *R,
+                                 double *f, double *g, double *H, void *ws, size_t ws_bytes, void *stream)
+{
+    int rc;
+    if ((rc = check_atoms(atoms)) || (rc = check_geom(geom, atoms->dim))) return rc;
+    GT_CHECK_ARG(order == 1 || order == 2, "gt_forward_derivs: order must be 1 or 2, got %d", order);
+    GT_CHECK_ARG(data != nullptr, "gt_forward_derivs: data is NULL");
+    GT_CHECK_ARG(R != nullptr, "gt_forward_derivs: R is NULL");
+    GT_CHECK_ARG(g != nullptr || atoms->n == 0, "gt_forward_derivs: g is NULL");
+    GT_CHECK_ARG(R != data, "gt_forward_derivs: R must not alias data");
+    const size_t need = gt_derivs_workspace(atoms, geom, GT_MODE_RESID);
+    if (need > ws_bytes || (need > 0 && ws == nullptr)) {
+        set_error("gt_forward_derivs: workspace of %zu bytes given, %zu needed", ws_bytes, need);
+        return GT_EWORKSPACE;
+    }
+    GT_CHECK_ARG(((uintptr_t)ws & 63) == 0, "gt_forward_derivs: workspace must be 64-byte aligned");
+    hipStream_t st = (hipStream_t)stream;
+    char *w = (char *)ws;
+    PairRec *recs = (PairRec *)w;
+    float *mom = (float *)(w + rec_bytes(atoms, geom));
+    double *sumC2 = (double *)(w + rec_bytes(atoms, geom) + mom_bytes(atoms, geom));
+    if ((rc = launch_setup(atoms, geom, recs, st))) return rc;
+    if ((rc = launch_forward(atoms, geom, recs, R, st, data))) return rc;
+    if (f) {
+        hipLaunchKernelGGL(sumsq_kernel, dim3(geom->nA), dim3(256), 0, st, R, (long)geom->n0 * geom->n1, sumC2);
+        hipLaunchKernelGGL(half_total_kernel, dim3(1), dim3(1), 0, st, sumC2, geom->nA, f);
+        GT_HIP(hipGetLastError());
+    }
+    return launch_derivs(atoms, geom, recs, mom, sumC2, R, GT_MODE_RESID, order > 1, nullptr, g, H, st);
 }

@@ -155,6 +155,33 @@ def derivs(atoms, space, C, mode="L2", order=2, want_f=True):
     return f, g, H
 
 
+def forward_derivs(atoms, space, data, order=2, out=None, want_f=True):
+    """One sweep of the l2 fit: resid = forward(atoms) - data (written into
+    out), and per atom the gradient / Hessian block of 1/2||resid||^2 (FP64);
+    f = 1/2||resid||^2 as a 0-d FP64 tensor.  Returns (resid, f, g, H)."""
+    lib = load()
+    dev = atoms.I.device
+    geom = device_geometry(space, dev)
+    a = atoms_struct(atoms)
+    data = data.data if hasattr(data, "data") and not isinstance(data, torch.Tensor) else data
+    data = torch.as_tensor(data, device=dev).to(torch.float32).contiguous()
+    if tuple(data.shape) != tuple(geom.shape):
+        raise ValueError(f"data must have shape {geom.shape}, got {tuple(data.shape)}")
+    if out is None:
+        out = torch.empty(geom.shape, dtype=torch.float32, device=dev)
+    if tuple(out.shape) != tuple(geom.shape) or out.dtype != torch.float32 or not out.is_contiguous():
+        raise ValueError(f"out must be a contiguous float32 tensor of shape {geom.shape}")
+    P = n_params(a.dim, a.iso)
+    f = torch.zeros((), dtype=torch.float64, device=dev) if want_f else None
+    g = torch.empty((a.n, P), dtype=torch.float64, device=dev)
+    H = torch.empty((a.n, P, P), dtype=torch.float64, device=dev) if order > 1 else None
+    need = lib.gt_derivs_workspace(ctypes.byref(a), ctypes.byref(geom.struct), GT_MODE_RESID)
+    ws = _ws.get(need, dev)
+    check(lib.gt_forward_derivs(ctypes.byref(a), ctypes.byref(geom.struct), _ptr(data), order, _ptr(out), _ptr(f),
+                                _ptr(g), _ptr(H), _ptr(ws), ws.numel(), stream_handle(dev)), "gt_forward_derivs")
+    return out, f, g, H
+
+
 class DeviceVolume:
     def __init__(self, space, device):
         cv = space.coord_vectors

@@ -217,9 +217,8 @@ def main():
     R = torch.empty_like(data)
 
     def step():
-        gtp.ops.forward(cur, space, out=R)
-        R.sub_(data)
-        return gtp.ops.derivs(cur, space, R, "resid", 2)
+        # forward, residual and every atom's gradient / Hessian block in one call
+        return gtp.ops.forward_derivs(cur, space, data, 2, out=R, want_f=False)
 
     if args.calib_copy_mib > 0:
         src = torch.ones(args.calib_copy_mib << 18, dtype=torch.float32, device=dev)

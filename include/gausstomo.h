@@ -116,6 +116,17 @@ size_t gt_derivs_workspace(const gt_atoms *atoms, const gt_geom *geom, int mode)
 int gt_derivs(const gt_atoms *atoms, const gt_geom *geom, const float *C, int mode, int order, double *f,
               double *g, double *H, void *ws, size_t ws_bytes, void *stream);
 
+/* One sweep of the l2 fit in one call: R = RadProj(atoms) - data, then for
+ * every atom the gradient g[n][P] and (order 2) Hessian block H[n][P][P] of
+ * 1/2 ||R||^2 with respect to that atom (what NewtonGaussian.linesearch and
+ * GD_lib.linesearch_block request atom by atom through L2derivs_RadProj with
+ * C = data - R + R_j, NewtonGaussian.py:59-60, GD_lib.py:121-123), and, if f is
+ * not NULL, f[0] = 1/2 ||R||^2 in FP64.  The records are set up once for the
+ * projection and the derivatives.  Workspace: gt_derivs_workspace(.., GT_MODE_RESID).
+ * R and data are [nA][n0][n1] device arrays and must not overlap. */
+int gt_forward_derivs(const gt_atoms *atoms, const gt_geom *geom, const float *data, int order, float *R,
+                      double *f, double *g, double *H, void *ws, size_t ws_bytes, void *stream);
+
 /* VolProj: u[j0][j1][j2] = sum over atoms of I sqrt(30/pi) exp(-30|U(y-x)|^2), cut at 1. */
 size_t gt_volume_workspace(const gt_atoms *atoms, const gt_vol *vol);
 int gt_volume(const gt_atoms *atoms, const gt_vol *vol, float *u, void *ws, size_t ws_bytes, void *stream);

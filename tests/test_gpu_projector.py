@@ -131,6 +131,43 @@ def test_derivs_3d(gt, oracle, cuda, mode):
     _derivs_case(gt, oracle, cuda, space, I, x, r, mode, 3)
 
 
+@pytest.mark.parametrize("dim", [3, 2])
+def test_forward_derivs_fused(gt, oracle, cuda, dim):
+    """gt_forward_derivs = forward, minus data, residual-mode derivatives: bit for
+    bit the separate calls, f = 1/2||resid||^2, and the gradient equals the
+    oracle's L2 derivative against C = data - R + R_j (atom by atom)."""
+    W = __import__("importlib").import_module("2018_dictionary_tomography_amd.workloads")
+    rng = np.random.default_rng(31)
+    if dim == 3:
+        I, x, r, space, _ = _c2_case(gt, 12, 9, 40, seed=31)
+    else:
+        I, x, r = W.draw_iso2(7, rng)
+        T, ortho, det = W.fan_2d(40, 64)
+        space = gt.ProjSpace(orientations=T, ortho=ortho, detector=det)
+    atoms = _atoms(gt, I, x, r, dim, cuda)
+    data = gt.ops.forward(_atoms(gt, I * 1.1, x + 0.01, r, dim, cuda), space)
+    resid, f, g, H = gt.ops.forward_derivs(atoms, space, data)
+    R = gt.ops.forward(atoms, space)
+    assert torch.equal(resid, R - data)
+    _, g2, H2 = gt.ops.derivs(atoms, space, R - data, "resid", 2)
+    assert torch.equal(g, g2) and torch.equal(H, H2)
+    r64 = resid.double()
+    assert abs(float(f) - 0.5 * float((r64 * r64).sum())) <= 1e-12 * float(f)
+    # full-loss derivative of atom j = L2 derivative against data - R + R_j (NewtonGaussian.py:59-60)
+    og = _ogeom(oracle, space)
+    R64 = R.double().cpu().numpy()
+    d64 = data.double().cpu().numpy()
+    for j in (0, len(I) - 1):
+        Rj = oracle.radproj(f32(I[j:j + 1]), f32(x[j:j + 1]), f32(np.asarray(r)[j:j + 1]), og)
+        _, go, Ho = oracle.derivs(f32(I[j:j + 1]), f32(x[j:j + 1]), f32(np.asarray(r)[j:j + 1]), og,
+                                  d64 - R64 + Rj, "L2", 2)
+        assert relerr(g[j].cpu().numpy(), go[0]) < 1e-4 and relerr(H[j].cpu().numpy(), Ho[0]) < 1e-4
+    # empty atom set: resid = -data, nothing else
+    empty = _atoms(gt, I[:0], x[:0], np.asarray(r)[:0], dim, cuda)
+    resid0, f0, g0, _ = gt.ops.forward_derivs(empty, space, data)
+    assert torch.equal(resid0, -data) and g0.numel() == 0
+
+
 @pytest.mark.parametrize("mode", ["L2", "linear"])
 def test_derivs_2p5d(gt, oracle, cuda, mode):
     W = __import__("importlib").import_module("2018_dictionary_tomography_amd.workloads")
