@@ -144,14 +144,17 @@ __device__ inline float pair_value(const PairRec &rec, float p0v, float p1v)
     return (q >= (float)KAPPA) ? rec.alpha * __builtin_amdgcn_exp2f(q) : 0.0f;
 }
 
-// Energy change (apply == 0) or residual update (apply == 1) of replacing
-// rec_old by rec_new over the union box, both values evaluated per pixel;
-// returns the lane's FP64 partial.  Both passes evaluate the same FP32
-// expressions, so the committed residual is exactly the one whose energy was
-// tested.
+// Replacing rec_old by rec_new over the union box, both values evaluated per
+// pixel with the same FP32 expressions in every mode; returns the lane's FP64
+// energy-change partial (modes 0 and 2).
+//   mode 0  energy change only (residual untouched)
+//   mode 1  commit: residual += new - old
+//   mode 2  speculative commit: energy change, residual updated in place, and
+//           the old residual of every union-box pixel saved in Sa (same layout)
+//           so that pair_restore can undo a rejected proposal exactly
 template <int K>
 __device__ inline double pair_delta4(const PairRec &ro, const PairRec &rn, const float *p0, const float *p1,
-                                     float *__restrict__ Ra, int n1, int lane, int apply)
+                                     float *__restrict__ Ra, float *__restrict__ Sa, int n1, int lane, int mode)
 {
     int lo[2], hi[2];
     box_union(ro, rn, K, lo, hi);
@@ -161,7 +164,8 @@ __device__ inline double pair_delta4(const PairRec &ro, const PairRec &rn, const
     double acc = 0.0;
     if (total == 0) return acc;
     if (K == 2) {
-        // column-owning walk (ColChunk): per-column terms of both Gaussians once per lane
+        // column-owning walk (ColChunk): per-column terms of both Gaussians once
+        // per lane; 32-bit element offsets within the angle's slab
         const float kap = (float)KAPPA;
         for (int cb = 0; cb < W; cb += 64) {
             ColChunk cc;
@@ -173,19 +177,19 @@ __device__ inline double pair_delta4(const PairRec &ro, const PairRec &rn, const
             const float rlo = fmaf(ro.q[1], d1o, ro.q[3]), rln = fmaf(rn.q[1], d1n, rn.q[3]);
             const float clo = fmaf(fmaf(ro.q[2], d1o, ro.q[4]), d1o, ro.q[5]);
             const float cln = fmaf(fmaf(rn.q[2], d1n, rn.q[4]), d1n, rn.q[5]);
-            float *col = Ra + i1;
+            const unsigned off0 = (unsigned)lo[0] * (unsigned)n1 + (unsigned)i1;
             for (int r0 = cc.ro; r0 < H; r0 += FIT_UNR * cc.R) {
                 float old[FIT_UNR], pv0[FIT_UNR];
                 bool ok[FIT_UNR];
-                size_t off[FIT_UNR];
+                unsigned off[FIT_UNR];
 #pragma unroll
                 for (int u = 0; u < FIT_UNR; u++) {
                     const int r = r0 + u * cc.R;
                     ok[u] = r < H;
-                    const int i0 = lo[0] + (ok[u] ? r : 0);
-                    off[u] = (size_t)i0 * n1;
-                    old[u] = col[off[u]];
-                    pv0[u] = p0[i0];
+                    const unsigned rr = ok[u] ? (unsigned)r : 0u;
+                    off[u] = off0 + rr * (unsigned)n1;
+                    old[u] = Ra[off[u]];
+                    pv0[u] = p0[lo[0] + (int)rr];
                 }
 #pragma unroll
                 for (int u = 0; u < FIT_UNR; u++) {
@@ -196,12 +200,11 @@ __device__ inline double pair_delta4(const PairRec &ro, const PairRec &rn, const
                     const float vo = (qo >= kap) ? ro.alpha * __builtin_amdgcn_exp2f(qo) : 0.0f;
                     const float vn = (qn >= kap) ? rn.alpha * __builtin_amdgcn_exp2f(qn) : 0.0f;
                     const float dR = vn - vo;
+                    if (mode == 2 && ok[u]) Sa[off[u]] = old[u];
                     if (ok[u] && dR != 0.0f) {
                         const float nw = old[u] + dR;
-                        if (apply)
-                            col[off[u]] = nw;
-                        else
-                            acc += 0.5 * ((double)nw - (double)old[u]) * ((double)nw + (double)old[u]);
+                        if (mode != 0) Ra[off[u]] = nw;
+                        if (mode != 1) acc += 0.5 * ((double)nw - (double)old[u]) * ((double)nw + (double)old[u]);
                     }
                 }
             }
@@ -227,17 +230,33 @@ __device__ inline double pair_delta4(const PairRec &ro, const PairRec &rn, const
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const float dR = pair_value(rn, pv0[u], pv1[u]) - pair_value(ro, pv0[u], pv1[u]);
+            if (mode == 2 && ok[u]) Sa[idx[u]] = old[u];
             if (ok[u] && dR != 0.0f) {
                 const float nw = old[u] + dR;
-                if (apply)
-                    Ra[idx[u]] = nw;
-                else
-                    acc += 0.5 * ((double)nw - (double)old[u]) * ((double)nw + (double)old[u]);
+                if (mode != 0) Ra[idx[u]] = nw;
+                if (mode != 1) acc += 0.5 * ((double)nw - (double)old[u]) * ((double)nw + (double)old[u]);
             }
         }
         tr.step();
     }
     return acc;
+}
+
+// Undo a speculative commit: the union box of the residual back from Sa.
+template <int K>
+__device__ inline void pair_restore(const PairRec &ro, const PairRec &rn, float *__restrict__ Ra,
+                                    const float *__restrict__ Sa, int n1, int lane)
+{
+    int lo[2], hi[2];
+    box_union(ro, rn, K, lo, hi);
+    const int W = (K == 2) ? hi[1] - lo[1] + 1 : 1;
+    const int H = hi[0] - lo[0] + 1;
+    if (W <= 0 || H <= 0) return;
+    for (int k = lane; k < W * H; k += 64) {
+        const int r = k / W, c = k - (k / W) * W;
+        const unsigned off = (unsigned)(lo[0] + r) * (unsigned)n1 + (unsigned)(K == 2 ? lo[1] + c : 0);
+        Ra[off] = Sa[off];
+    }
 }
 
 // LDS written by some lanes of a wave, then read by others: order the wave's
@@ -456,6 +475,7 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
     const int nd[2] = {S.n0, S.n1};
     char *scr = (char *)S.scratch;
     double *prec = (double *)scr;
+    float *stage = S.stage;
     double *dEa = (double *)(scr + fit_off_dE(nA));
     unsigned *bar = (unsigned *)(scr + fit_off_bar(nA));
     const int nOwn = member < nA ? (nA - member + G - 1) / G : 0;
@@ -695,7 +715,8 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         for (int k = wave; k < nOwn; k += FIT_WAVES) {
             const int a = member + G * k;
             float *Ra = S.resid + (size_t)a * slab;
-            double acc = pair_delta4<K>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, Ra, n1, lane, 0);
+            double acc = pair_delta4<K>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, Ra, stage ? stage + (size_t)a * slab : nullptr,
+                                        n1, lane, stage ? 2 : 0);
             acc = wave_sum_f64(acc);
             if (lane == 0) xput(dEa + a, acc);
         }
@@ -741,10 +762,16 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         __syncthreads();
         GT_FIT_TICK(6);
         // ---- phase H: commit the residual of the own angles ----------------
-        if (L.accept) {
+        // with a staging slab the energy pass already committed: undo on
+        // reject; without one, commit on accept
+        if (stage ? !L.accept : L.accept) {
             for (int k = wave; k < nOwn; k += FIT_WAVES) {
                 const int a = member + G * k;
-                pair_delta4<K>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, S.resid + (size_t)a * slab, n1, lane, 1);
+                float *Ra = S.resid + (size_t)a * slab;
+                if (stage)
+                    pair_restore<K>(L.rec_old[k], L.rec_new[k], Ra, stage + (size_t)a * slab, n1, lane);
+                else
+                    pair_delta4<K>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, Ra, nullptr, n1, lane, 1);
             }
         }
         // ---- loop control (NewtonGaussian.py:50-98) ------------------------

@@ -32,7 +32,7 @@ class gt_fit_slot(ctypes.Structure):
         ("T", ctypes.c_void_p), ("w0", ctypes.c_void_p), ("w1", ctypes.c_void_p), ("p0", ctypes.c_void_p),
         ("p1", ctypes.c_void_p), ("perm", ctypes.c_void_p),
         ("I", ctypes.c_void_p), ("x", ctypes.c_void_p), ("r", ctypes.c_void_p), ("resid", ctypes.c_void_p),
-        ("h", ctypes.c_void_p), ("Ehist", ctypes.c_void_p), ("scratch", ctypes.c_void_p),
+        ("h", ctypes.c_void_p), ("Ehist", ctypes.c_void_p), ("scratch", ctypes.c_void_p), ("stage", ctypes.c_void_p),
         ("it", ctypes.c_int), ("pos", ctypes.c_int), ("inner", ctypes.c_int), ("jj", ctypes.c_int),
         ("brk0", ctypes.c_int), ("brk1", ctypes.c_int), ("done", ctypes.c_int), ("pad1", ctypes.c_int),
         ("E", ctypes.c_double), ("updates", ctypes.c_longlong), ("accepted", ctypes.c_longlong),
@@ -123,6 +123,8 @@ class FitRequest:
             self.Ehist[0] = self.E0
         lib = _declare()
         self.scratch = torch.zeros(lib.gt_fit_scratch_bytes(self.geom.nA), dtype=torch.uint8, device=dev)
+        # old residual of the update in flight: lets the energy pass commit at once
+        self.stage = torch.empty(self.geom.shape, dtype=torch.float32, device=dev)
         g = self.geom
         self.slot = gt_fit_slot(
             active=1 if self.N > 0 else 0, iso=self.iso, N=self.N, nA=g.nA, n0=g.n0, n1=g.n1, lenT=g.lenT,
@@ -131,6 +133,7 @@ class FitRequest:
             p0=g.p0.data_ptr(), p1=g.p1.data_ptr() if g.p1 is not None else None, perm=self.perm.data_ptr(),
             I=self.I.data_ptr(), x=self.x.data_ptr(), r=self.r.data_ptr(), resid=self.resid.data_ptr(),
             h=self.h.data_ptr(), Ehist=self.Ehist.data_ptr() if record else None, scratch=self.scratch.data_ptr(),
+            stage=self.stage.data_ptr(),
             it=0, pos=0, inner=0, jj=0, brk0=1, brk1=1, done=0 if self.N > 0 else 1, pad1=0, E=self.E0,
             updates=0, accepted=0)
 

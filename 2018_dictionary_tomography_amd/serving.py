@@ -48,12 +48,12 @@ class RequestSpec:
     max_iter: int = 30
 
     def footprint_bytes(self):
-        """HBM held while the request is in flight: data and residual sinograms,
-        atoms (twice), permutations, step sizes, the slot scratch (per-angle
-        gradient/Hessian records and energy changes), geometry."""
+        """HBM held while the request is in flight: data, residual and staging
+        sinograms, atoms (twice), permutations, step sizes, the slot scratch
+        (per-angle gradient/Hessian records and energy changes), geometry."""
         sino = self.nA * self.S * self.S * 4
         scratch = self.nA * (65 * 8 + 8) + 1024
-        return 2 * sino + self.N * (10 * 4 * 2 + 8) + self.max_iter * self.N * 4 + scratch + self.nA * 12 * 4 + 8 * self.S
+        return 3 * sino + self.N * (10 * 4 * 2 + 8) + self.max_iter * self.N * 4 + scratch + self.nA * 12 * 4 + 8 * self.S
 
 
 def c5_stream(n_requests=2000, rate=200.0, seed=0, n_range=(50, 5000), angle_range=(30, 120),

@@ -143,7 +143,8 @@ int gt_ft_eval(const gt_atoms *atoms, const gt_kgrid *grid, float *out, void *ws
  * Device-resident damped-Newton fitter (NewtonGaussian.linesearch with
  * GD_lib._step, NewtonGaussian.py:18-109, GD_lib.py:492-536; null
  * regulariser, l2 fidelity).  One gt_fit_slot per fit request, in device
- * memory; one workgroup advances one slot.  The host fills the problem and
+ * memory; one workgroup (gt_fit_run) or a group of workgroups
+ * (gt_fit_run_blocks) advances one slot.  The host fills the problem and
  * state pointers, the initial residual (forward - data) and energy, and the
  * per-outer-iteration atom permutations; gt_fit_run then performs up to
  * `budget` atom updates on every active slot and returns (stream-ordered),
@@ -168,6 +169,9 @@ typedef struct gt_fit_slot {
     double *h;        /* [N] per-atom step sizes, initialised to 1 */
     double *Ehist;    /* [max_outer*max_inner*N + 1] energy after each update, or NULL */
     float *scratch;   /* gt_fit_scratch_bytes(nA) bytes, zeroed before the first run */
+    float *stage;     /* [nA][n0][n1] old residual of the update in flight, or NULL: with it
+                         the energy pass commits the proposal at once and a rejected one is
+                         restored from here (same results, one pixel pass less) */
     /* progress (read-write) */
     int it, pos, inner, jj;  /* outer iteration, position in perm, inner step, history index */
     int brk0, brk1;          /* convergence flags of the current outer / inner loop */

@@ -105,8 +105,8 @@ def test_newton_multi_slot_equals_single(gt, cuda):
 
 def test_newton_group_size_does_not_change_result(gt, oracle, cuda):
     """A fit advanced by G = 1, 2, 3, 5 workgroups (angles shared between
-    them, per-angle reductions in a fixed order) is bit-identical to G = 1
-    and follows the oracle."""
+    them, per-angle reductions in a fixed order), with the speculative commit
+    or without it, is bit-identical to G = 1 and follows the oracle."""
     F = importlib.import_module("2018_dictionary_tomography_amd.fit")
     I, x, r, space, rng = _setup3(gt, 7, 23, 36, seed=12)
     aspace = gt.AtomSpace(3, isotropic=False)
@@ -116,8 +116,10 @@ def test_newton_group_size_does_not_change_result(gt, oracle, cuda):
     x0 = x + rng.normal(0, 0.02, x.shape)
     start = gt.AtomElement(aspace, x0, r, I * 0.9, device=cuda)
     runs = []
-    for G in (1, 2, 3, 5):
+    for G, staged in ((1, True), (2, True), (3, True), (5, True), (1, False), (3, False)):
         req = F.FitRequest(start, space, data, [3, 1], tol=1e-6)
+        if not staged:  # commit after the accept decision instead of restoring on reject
+            req.slot.stage = None
         table = torch.as_tensor(np.frombuffer(req.slot_bytes(), dtype=np.uint8).copy(), device=cuda)
         blocks = torch.as_tensor(F.block_table([G], [4]), device=cuda)
         for _ in range(100):
