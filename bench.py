@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=256, help="requests of the serving leg (0 skips it)")
     ap.add_argument("--serve-nmax", type=int, default=500, help="upper end of the log-uniform atom count")
     ap.add_argument("--plain-requests", type=int, default=6, help="requests timed one at a time")
+    ap.add_argument("--autograd-reps", type=int, default=3,
+                    help="timed forward+backward passes of the autograd leg (0 skips it)")
+    ap.add_argument("--fourier-reps", type=int, default=5, help="timed evaluations of the Fourier leg (0 skips it)")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg and print its JSON")
     ap.add_argument("--skip-checks", action="store_true",
                     help="skip the FP64 parity spot check and the in-support pair count (counter runs)")
@@ -123,6 +126,86 @@ def serving_leg(torch, args, dev):
         "alone_equals_mixed": bool(same),
         "max_residual_drift": max(a[6] for a in alone.values()),
     }
+
+
+def autograd_leg(torch, gtp, dev, reps):
+    """Differentiation workload: 5000 atoms drawn as the 3D workload, 90 angles
+    over [0, pi) about a seeded random unit axis, 256x256 detector; loss
+    1/2||A(theta) - b||^2 with b = ground truth + 1% noise, evaluated at a
+    perturbed ground truth; one timed pass = forward + hand-written backward
+    (the adjoint kernel), no stored residual x Jacobian intermediates."""
+    AG = importlib.import_module("2018_dictionary_tomography_amd.autograd")
+    W = importlib.import_module("2018_dictionary_tomography_amd.workloads")
+    rng = np.random.default_rng(3)
+    N, nA, S = 5000, 90, 256
+    I, x, r, _ = W.draw_aniso3(N, rng)
+    T, ortho, det = W.half_turn_series(nA, W.random_axis(rng), det=S)
+    space = gtp.ProjSpace(orientations=T, ortho=ortho, detector=det)
+    clean = gtp.ops.forward(gtp.AtomElement(gtp.AtomSpace(3, isotropic=False), x, r, I, device=dev), space)
+    b = clean + torch.as_tensor(rng.normal(0, 0.01 * float(clean.max()), clean.shape), dtype=torch.float32,
+                                device=dev)
+    t = [torch.tensor(v, dtype=torch.float32, device=dev, requires_grad=True)
+         for v in (I * rng.uniform(0.9, 1.1, N), x + rng.normal(0, 0.005, x.shape), r)]
+
+    def one():
+        for v in t:
+            v.grad = None
+        loss = AG.l2_loss(*t, space, b)
+        loss.backward()
+        return loss
+
+    one()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loss = one()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    return {"workload": "5000 atoms, 90 angles over [0, pi) about a random axis, 256x256 detector",
+            "ms_per_forward_backward": ms, "evals_per_s": 2.0 * N * nA * S * S / (ms * 1e-3),
+            "loss": float(loss.detach()), "reps": reps}
+
+
+def fourier_leg(torch, gtp, dev, reps):
+    """Fourier workload: analytic FT of 2000 atoms drawn as the 3D workload on
+    the integer grid [-64, 64)^3 in complex64; error against an FP64
+    restatement of the same closed form (torch, independent of the library) on
+    4096 sampled grid points."""
+    Fo = importlib.import_module("2018_dictionary_tomography_amd.fourier")
+    W = importlib.import_module("2018_dictionary_tomography_amd.workloads")
+    rng = np.random.default_rng(44)
+    I, x, r, _ = W.draw_aniso3(2000, rng)
+    nI, nx, nr = Fo.analytic_gauss_fourier(I, x, r * np.sqrt(60.0))
+    k = [np.arange(-64, 64, dtype=np.float64)] * 3
+    atoms = Fo.FTAtoms(nI, nx, nr, dev)
+    grid = Fo.DeviceKGrid(k, dev)
+    out = Fo.evaluate(atoms, grid)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        Fo.evaluate(atoms, grid, out=out)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    # FP64 restatement: sum I exp(-|U^T k|^2 / 2) [|U^T k|^2 < 30] exp(-i x.k), U upper triangular
+    idx = np.random.default_rng(0).integers(0, 128, (4096, 3))
+    K = torch.as_tensor(np.stack([k[0][idx[:, 0]], k[1][idx[:, 1]], k[2][idx[:, 2]]], 1), device=dev)
+    d = lambda a: torch.as_tensor(np.float32(a).astype(np.float64), device=dev)
+    I64, x64, r64 = d(nI), d(nx), d(nr)
+    U = torch.zeros((len(nI), 3, 3), dtype=torch.float64, device=dev)
+    for s_, (i, j) in enumerate([(0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (0, 2)]):
+        U[:, i, j] = r64[:, s_]
+    ref = torch.zeros(4096, dtype=torch.complex128, device=dev)
+    for c in range(0, len(nI), 250):
+        Ut = torch.einsum("nli,pl->npi", U[c:c + 250], K)
+        norm = (Ut * Ut).sum(-1)
+        amp = I64[c:c + 250, None] * torch.exp(-0.5 * norm) * (norm < 30.0)
+        ph = x64[c:c + 250] @ K.T
+        ref += (amp * torch.exp(-1j * ph)).sum(0)
+    got = out[idx[:, 0], idx[:, 1], idx[:, 2]].to(torch.complex128)
+    err = float((got - ref).abs().max() / ref.abs().max())
+    pairs = 2000.0 * 128 ** 3
+    return {"workload": "2000 atoms on the integer grid [-64, 64)^3, complex64", "ms_per_evaluation": ms,
+            "atom_k_pairs_per_s": pairs / (ms * 1e-3), "rel_err_vs_fp64_sampled": err, "reps": reps}
 
 
 def support_pairs(torch, I, x, r, T, w0, w1, p0, p1):
@@ -283,6 +366,8 @@ def main():
     }
     fit = fit_leg(torch, gtp, space, data, cur, args.fit_iters) if args.fit_iters > 0 else None
     serving = serving_leg(torch, args, dev) if args.serve_requests > 0 else None
+    autograd = autograd_leg(torch, gtp, dev, args.autograd_reps) if args.autograd_reps > 0 else None
+    fourier = fourier_leg(torch, gtp, dev, args.fourier_reps) if args.fourier_reps > 0 else None
     cpu = None
     if args.cpu_atoms > 0:
         cpu = cpu_baseline(np.float32(I).astype(float), np.float32(x_cur).astype(float),
@@ -311,6 +396,8 @@ def main():
         "cpu_baseline": cpu,
         "newton_fit": fit,
         "serving": serving,
+        "autograd": autograd,
+        "fourier": fourier,
     }
     print(json.dumps(out))
     return 0
