@@ -15,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgausstomo.so")
 
 GT_MODE_LINEAR, GT_MODE_L2, GT_MODE_RESID = 0, 1, 2
+GT_REG_NULL, GT_REG_JOUBERT, GT_REG_MASS, GT_REG_VOLUME, GT_REG_RADIUS, GT_REG_ISO = 0, 1, 2, 3, 4, 5
+GT_REG_NPARAM = 5
 
 
 class GTError(RuntimeError):
@@ -58,6 +60,7 @@ SIGNATURES = {
     "gt_volume": (_ci, [_P(gt_atoms), _P(gt_vol), _vp, _vp, _sz, _vp]),
     "gt_ft_workspace": (_sz, [_P(gt_atoms), _P(gt_kgrid)]),
     "gt_ft_eval": (_ci, [_P(gt_atoms), _P(gt_kgrid), _vp, _vp, _sz, _vp]),
+    "gt_reg_eval": (_ci, [_P(gt_atoms), _ci, _P(ctypes.c_double), _vp, _vp, _vp, _vp]),
     "gt_fit_run": (_ci, [_vp, _ci, _ci, _ci, _vp]),
     "gt_fit_scratch_bytes": (_sz, [_ci]),
     "gt_fit_max_blocks": (_ci, []),

@@ -1,22 +1,31 @@
 // gt_fit.hip -- device-resident damped-Newton fitter.
 //
 // Restates NewtonGaussian.linesearch (NewtonGaussian.py:18-109) with
-// GD_lib._step (GD_lib.py:492-536) for the null regulariser and the l2
-// fidelity.  For each atom j of the outer iteration's permutation:
+// GD_lib._step (GD_lib.py:492-536), and NewtonGaussian.linesearch_block
+// (NewtonGaussian.py:112-237), for the l2 fidelity and any per-atom
+// regulariser of gt_reg.h.  For each atom j of the outer iteration's
+// permutation:
 //
 //   1. C = data + R_j - R is never formed: the L2 derivatives of atom j
 //      against C equal the derivatives of 1/2||R - data||^2 w.r.t. atom j,
 //      so the pixel moments are taken against the stored residual R - data.
-//   2. (g, H) of atom j: per-angle FP32 pixel moments + FP64 contraction.
+//   2. (g, H) of atom j: per-angle FP32 pixel moments + FP64 contraction,
+//      plus the regulariser's gradient and Hessian (NewtonGaussian.py:61-66).
 //   3. dx = solve(H + I/h_j, -g) in FP64 with partial pivoting (numpy.linalg.
 //      solve, NewtonGaussian.py:73-76); singular -> the fit stops (done = 2).
-//   4. _step projections: I >= 0, rows of r with a negative diagonal negated.
-//   5. Energy change of the committed FP32 residual, accumulated in FP64 over
-//      the union of the old and new footprints; reject iff it is > 0 (or not
-//      finite), the reference's E[jj] > E[jj-1] test on the full energies.
+//      Block variant: the I, x and r blocks in turn, each solved on its own
+//      diagonal block of H + I/h_{t,j}; a block whose system is singular (or
+//      whose I pivot is <= 1e-8) is skipped (NewtonGaussian.py:154-176).
+//   4. Projections: _step's (I >= 0, rows of r with a negative diagonal
+//      negated); the block variant also clamps x to [-0.99, 0.99] (:167).
+//   5. Energy change: the change of the committed FP32 residual's
+//      1/2||.||^2, accumulated in FP64 over the union of the old and new
+//      footprints, plus reg(new_j) - reg(old_j); reject iff the new total
+//      exceeds the old (the reference's E[jj] > E[jj-1] test), and reject a
+//      proposal whose fidelity change is not finite.
 //   6. Accepted: atom written back, residual updated over the same pixels.
-//      Step size h_j x2 / /10 clamped to [1e-4, 1e4], convergence flags as in
-//      NewtonGaussian.py:78-98.
+//      Step size h x2 (block: x1.3) / /10 clamped to [1e-4, 1e4], convergence
+//      flags as in NewtonGaussian.py:78-98 and :205-221.
 //
 // A slot is advanced by a group of G workgroups of 16 waves (G = 1 for
 // gt_fit_run).  Member m owns angles m, m + G, ...: their detector Gaussians
@@ -35,6 +44,7 @@
 // one pair spread over the wave's lanes, see prep1..prep3), so the latency of
 // that FP64 chain overlaps the pixel work of the other waves.
 #include "gt_pixel.h"
+#include "gt_reg.h"
 
 namespace gt {
 
@@ -55,8 +65,8 @@ __host__ __device__ inline size_t fit_scratch_size(int nA) { return fit_off_bar(
 
 // Progress of a fit, held in LDS by every member during a launch.
 struct FitCtl {
-    int it, pos, inner, jj, brk0, brk1, done, pad;
-    double E;
+    int it, pos, inner, jj, brk0, brk1, done, blk, ninf, pad;
+    double E, F, Rsum;
     long long updates, accepted;
 };
 
@@ -76,10 +86,13 @@ template <int D> struct FitLDS {
     double th_old[10], th_new[10];
     double g[10], H[100];
     double A[100], b[10], x[10];
+    double A2[36], b2[6];     // diagonal block of the block variant
+    double rg[10], rH[100];   // regulariser gradient / Hessian of the current atom
+    double rv_old, rv_new;    // regulariser terms of the current atom and of the proposal
     double dxnorm, hj;
     FitCtl c;
     unsigned gen;
-    int accept, j, stop, abort_;
+    int accept, j, stop, abort_, skip, moved;
 };
 
 // theta (FP64 values of the FP32 atom) -> PairGeom atom part
@@ -152,9 +165,9 @@ __device__ inline float pair_value(const PairRec &rec, float p0v, float p1v)
 //   mode 2  speculative commit: energy change, residual updated in place, and
 //           the old residual of every union-box pixel saved in Sa (same layout)
 //           so that pair_restore can undo a rejected proposal exactly
-template <int K>
+template <int K, int mode>
 __device__ inline double pair_delta4(const PairRec &ro, const PairRec &rn, const float *p0, const float *p1,
-                                     float *__restrict__ Ra, float *__restrict__ Sa, int n1, int lane, int mode)
+                                     float *__restrict__ Ra, float *__restrict__ Sa, int n1, int lane)
 {
     int lo[2], hi[2];
     box_union(ro, rn, K, lo, hi);
@@ -379,6 +392,26 @@ template <int n> __device__ inline bool wave_solve(const double *A, const double
     return ok;
 }
 
+// wave_solve for the system sizes a fit of dimension D can ask for: all
+// parameters (aniso P, iso 2 + D), or one block (x: D, r: NR or 1).
+template <int D> __device__ inline bool solve_n(int n, const double *A, const double *b, double *x, int lane)
+{
+    switch (n) {
+    case 1:
+        return wave_solve<1>(A, b, x, lane);
+    case D:
+        return wave_solve<D>(A, b, x, lane);
+    case Dims<D>::NR:
+        return wave_solve<Dims<D>::NR>(A, b, x, lane);
+    case 2 + D:
+        return wave_solve<2 + D>(A, b, x, lane);
+    case Dims<D>::P:
+        return wave_solve<Dims<D>::P>(A, b, x, lane);
+    default:
+        return false;
+    }
+}
+
 // Upper-triangle entry index e (0-based, row-major over i <= j) -> (i, j).
 __device__ inline void upper_ij(int e, int P, int &i, int &j)
 {
@@ -444,6 +477,39 @@ __device__ inline bool group_barrier(unsigned *bar, int G, unsigned &gen)
     return true;
 }
 
+// Advance the loop position after an update (or after a skipped block):
+// inner iterations, the blocks of the block variant, atoms of the
+// permutation, outer iterations and the convergence exit
+// (NewtonGaussian.py:50-98, :131-222).
+__device__ inline void fit_advance(FitCtl &c, const gt_fit_slot &S, bool skipped)
+{
+    bool next = skipped;
+    if (!skipped) {
+        c.inner++;
+        next = c.brk1 || c.inner >= S.max_inner;
+    }
+    if (next) {
+        c.inner = 0;
+        if (S.block && ++c.blk < 3) {
+            // next block of the same atom
+        } else {
+            c.blk = 0;
+            c.pos++;
+        }
+    }
+    c.brk1 = 1;  // BREAK[1] = True at the start of every inner iteration
+    if (c.pos >= S.N) {
+        c.pos = 0;
+        if (c.brk0 && c.it > S.min_iter) {
+            c.done = 1;
+        } else {
+            c.it++;
+            if (c.it >= S.max_outer) c.done = 1;
+            c.brk0 = 1;
+        }
+    }
+}
+
 template <int D>
 __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restrict__ slots, int nslots,
                                                          const gt_fit_block *__restrict__ blocks, int budget)
@@ -494,6 +560,10 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         c.E = S.E;
         c.updates = S.updates;
         c.accepted = S.accepted;
+        c.blk = S.block ? S.blk : 0;
+        c.ninf = S.ninf;
+        c.F = S.F;
+        c.Rsum = S.Rsum;
         L.gen = (G > 1) ? xgetu(bar + 1) : 0u;
         L.abort_ = 0;
     }
@@ -523,6 +593,7 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         // ---- choose the atom, load its parameters -----------------------
         if (tid == 0) {
             FitCtl &c = L.c;
+            L.skip = 0;
             L.stop = c.done != 0 || c.it >= S.max_outer || c.pos >= N;
             if (!L.stop) {
                 const int j = S.perm[(size_t)c.it * N + c.pos];
@@ -635,13 +706,20 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         __syncthreads();
         GT_FIT_TICK(2);
         // ---- phase D: damped Newton system, solve, projections ------------
-        // h_j is read here, between the two barriers of this update: other
+        // h is read here, between the two barriers of this update: other
         // members write its new value only after the second one
-        if (tid == 0) L.hj = S.h[j];
+        const int rk = S.reg_kind;
+        if (tid == 0) {
+            L.hj = S.h[(size_t)L.c.blk * N + j];
+            L.rv_old = rk != GT_REG_NULL ? reg_eval<D>(rk, S.reg_p, iso, L.th_old, L.rg, L.rH) : 0.0;
+        }
         __syncthreads();
+        // (H + reg'' + I/h) dx = -(g + reg') over the solve's parameters
+        // (NewtonGaussian.py:61-74 and :147-152)
         if (tid < PI * PI) {
             const int i = tid / PI, l = tid - (tid / PI) * PI;
             double v = iso ? collapse_H<D>(L.H, P, i, l) : L.H[i * P + l];
+            if (rk != GT_REG_NULL) v += L.rH[tid];
             if (i == l) v += 1.0 / L.hj;
             L.A[tid] = v;
         }
@@ -653,55 +731,104 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
             } else {
                 gv = L.g[tid];
             }
+            if (rk != GT_REG_NULL) gv += L.rg[tid];
             L.b[tid] = -gv;
         }
         __syncthreads();
         if (wave == 0) {
-            const bool ok = (PI == P) ? wave_solve<P>(L.A, L.b, L.x, lane) : wave_solve<2 + D>(L.A, L.b, L.x, lane);
+            // the parameters this update moves: all, or block blk = I | x | r
+            int o = 0, n = PI;
+            if (S.block) {
+                o = L.c.blk == 0 ? 0 : (L.c.blk == 1 ? 1 : 1 + D);
+                n = L.c.blk == 0 ? 1 : (L.c.blk == 1 ? D : PI - 1 - D);
+                for (int e = lane; e < n * n; e += 64) L.A2[e] = L.A[(o + e / n) * PI + o + e % n];
+                if (lane < n) L.b2[lane] = L.b[o + lane];
+                wave_sync();
+            }
+            bool ok;
+            if (!S.block) {
+                ok = solve_n<D>(n, L.A, L.b, L.x, lane);
+            } else if (n == 1 && L.c.blk == 0) {
+                // dx = -df[0] / ddf[0, 0] if ddf[0, 0] > 1e-8 (NewtonGaussian.py:154-158)
+                ok = L.A2[0] > 1e-8;
+                if (lane == 0 && ok) L.x[0] = L.b2[0] / L.A2[0];
+            } else {
+                ok = solve_n<D>(n, L.A2, L.b2, L.x, lane);
+            }
             if (lane == 0) {
                 if (!ok) {
-                    L.c.done = 2;
-                    L.stop = 1;
+                    if (S.block) {
+                        L.skip = 1;  // `break` to the next block
+                    } else {
+                        L.c.done = 2;
+                        L.stop = 1;
+                    }
                 } else {
                     const double *b = L.x;
-                    double n2 = 0.0;
-                    for (int i = 0; i < PI; i++) n2 += b[i] * b[i];
-                    L.dxnorm = sqrt(n2);
-                    // GD_lib._step projections (GD_lib.py:502-522), result rounded to FP32
-                    double t[10];
-                    t[0] = fmax(0.0, L.th_old[0] + b[0]);
-                    for (int i = 0; i < D; i++) t[1 + i] = L.th_old[1 + i] + b[1 + i];
-                    if (iso) {
-                        t[1 + D] = fabs(L.th_old[1 + D] + b[1 + D]);
-                    } else {
-                        for (int k = 0; k < NR; k++) t[1 + D + k] = L.th_old[1 + D + k] + b[1 + D + k];
-                        double *rr = t + 1 + D;
-                        if (D == 2) {
-                            if (rr[0] < 0) {
-                                rr[0] = -rr[0];
-                                rr[2] = -rr[2];
-                            }
-                            if (rr[1] < 0) rr[1] = -rr[1];
+                    double *t = L.th_new;  // LDS: indexed by the block's runtime offset
+                    for (int i = 0; i < 1 + D + NRs; i++) t[i] = L.th_old[i];
+                    for (int i = 0; i < n; i++) t[o + i] += b[i];
+                    if (!S.block) {
+                        double n2 = 0.0;
+                        for (int i = 0; i < PI; i++) n2 += b[i] * b[i];
+                        L.dxnorm = sqrt(n2);
+                    }
+                    // projections: GD_lib.py:502-522; block variant :159-187
+                    if (!S.block || L.c.blk == 0) t[0] = fmax(0.0, t[0]);
+                    if (S.block && L.c.blk == 1)
+                        for (int i = 0; i < D; i++) {
+                            const double v = t[1 + i];
+                            t[1 + i] = v < -0.99 ? -0.99 : (v > 0.99 ? 0.99 : v);
+                        }
+                    if (!S.block || L.c.blk == 2) {
+                        if (iso) {
+                            t[1 + D] = fabs(t[1 + D]);
                         } else {
-                            if (rr[0] < 0) {
-                                rr[0] = -rr[0];
-                                rr[3] = -rr[3];
-                                rr[5] = -rr[5];
+                            double *rr = t + 1 + D;
+                            if (D == 2) {
+                                if (rr[0] < 0) {
+                                    rr[0] = -rr[0];
+                                    rr[2] = -rr[2];
+                                }
+                                if (rr[1] < 0) rr[1] = -rr[1];
+                            } else {
+                                if (rr[0] < 0) {
+                                    rr[0] = -rr[0];
+                                    rr[3] = -rr[3];
+                                    rr[5] = -rr[5];
+                                }
+                                if (rr[1] < 0) {
+                                    rr[1] = -rr[1];
+                                    rr[4] = -rr[4];
+                                }
+                                if (rr[2] < 0) rr[2] = -rr[2];
                             }
-                            if (rr[1] < 0) {
-                                rr[1] = -rr[1];
-                                rr[4] = -rr[4];
-                            }
-                            if (rr[2] < 0) rr[2] = -rr[2];
                         }
                     }
-                    for (int i = 0; i < 1 + D + NRs; i++) L.th_new[i] = (double)(float)t[i];
+                    // rounded to the FP32 atom storage
+                    for (int i = 0; i < 1 + D + NRs; i++) t[i] = (double)(float)t[i];
+                    if (S.block) {
+                        // norm(old - T) > tol * norm(old) over the block (NewtonGaussian.py:214)
+                        double dn = 0.0, on = 0.0;
+                        for (int i = o; i < o + n; i++) {
+                            const double d = L.th_new[i] - L.th_old[i];
+                            dn += d * d;
+                            on += L.th_old[i] * L.th_old[i];
+                        }
+                        L.moved = sqrt(dn) > S.tol * sqrt(on);
+                    }
+                    L.rv_new = rk != GT_REG_NULL ? reg_eval<D>(rk, S.reg_p, iso, L.th_new, nullptr, nullptr) : 0.0;
                 }
             }
         }
         __syncthreads();
         GT_FIT_TICK(3);
         if (L.stop) break;
+        if (L.skip) {  // block variant: nothing to try for this block
+            if (tid == 0) fit_advance(L.c, S, true);
+            __syncthreads();
+            continue;
+        }
         // ---- phase E: detector Gaussians of the proposal -------------------
         for (int k = tid; k < nOwn; k += FIT_THREADS) {
             PairGeom<D> g;
@@ -715,8 +842,9 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         for (int k = wave; k < nOwn; k += FIT_WAVES) {
             const int a = member + G * k;
             float *Ra = S.resid + (size_t)a * slab;
-            double acc = pair_delta4<K>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, Ra, stage ? stage + (size_t)a * slab : nullptr,
-                                        n1, lane, stage ? 2 : 0);
+            double acc = stage ? pair_delta4<K, 2>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, Ra, stage + (size_t)a * slab,
+                                                   n1, lane)
+                               : pair_delta4<K, 0>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, Ra, nullptr, n1, lane);
             acc = wave_sum_f64(acc);
             if (lane == 0) xput(dEa + a, acc);
         }
@@ -730,7 +858,20 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
             dE = wave_sum_f64(dE);
             if (lane == 0) {
                 FitCtl &c = L.c;
-                const bool accept = dE <= 0.0;  // false for NaN: a non-finite proposal is rejected
+                // E_new > E_old with E = F + sum of the regulariser terms: with a
+                // non-finite term already present E_old = +inf and nothing
+                // exceeds it; a non-finite new term exceeds any finite E_old
+                const bool bad_old = !isfinite(L.rv_old), bad_new = !isfinite(L.rv_new);
+                bool accept;
+                if (!isfinite(dE))
+                    accept = false;  // a non-finite proposal is rejected
+                else if (c.ninf > 0)
+                    accept = true;
+                else if (bad_new)
+                    accept = false;
+                else
+                    accept = (rk != GT_REG_NULL ? dE + (L.rv_new - L.rv_old) : dE) <= 0.0;
+                if (S.Fhist && member == 0) S.Fhist[c.jj + 1] = c.F + dE;
                 double h = L.hj;
                 if (!accept) {
                     if (h > 2 * FIT_EPS) {
@@ -739,12 +880,19 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                     }
                     h /= 10.0;
                 } else {
-                    c.E += dE;
-                    if (L.dxnorm > S.tol) {
+                    c.F += dE;
+                    if (rk != GT_REG_NULL) {
+                        c.Rsum += (bad_new ? 0.0 : L.rv_new) - (bad_old ? 0.0 : L.rv_old);
+                        c.ninf += (int)bad_new - (int)bad_old;
+                        c.E = c.F + (c.ninf > 0 ? __builtin_huge_val() : c.Rsum);
+                    } else {
+                        c.E = c.F;
+                    }
+                    if (S.block ? L.moved : L.dxnorm > S.tol) {
                         c.brk0 = 0;
                         c.brk1 = 0;
                     }
-                    h *= 2.0;
+                    h *= S.block ? 1.3 : 2.0;
                     // every member writes the same values: its own later reads
                     // of atom j then never depend on another member's timing
                     S.I[j] = (float)L.th_new[0];
@@ -752,7 +900,7 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                     for (int k = 0; k < NRs; k++) S.r[(size_t)j * NRs + k] = (float)L.th_new[1 + D + k];
                     c.accepted++;
                 }
-                S.h[j] = fmin(fmax(h, FIT_EPS), 1.0 / FIT_EPS);
+                S.h[(size_t)c.blk * N + j] = fmin(fmax(h, FIT_EPS), 1.0 / FIT_EPS);
                 c.jj++;
                 if (S.Ehist && member == 0) S.Ehist[c.jj] = c.E;
                 c.updates++;
@@ -771,29 +919,11 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                 if (stage)
                     pair_restore<K>(L.rec_old[k], L.rec_new[k], Ra, stage + (size_t)a * slab, n1, lane);
                 else
-                    pair_delta4<K>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, Ra, nullptr, n1, lane, 1);
+                    pair_delta4<K, 1>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, Ra, nullptr, n1, lane);
             }
         }
-        // ---- loop control (NewtonGaussian.py:50-98) ------------------------
-        if (tid == 0) {
-            FitCtl &c = L.c;
-            c.inner++;
-            if (c.brk1 || c.inner >= S.max_inner) {
-                c.inner = 0;
-                c.pos++;
-            }
-            c.brk1 = 1;  // BREAK[1] = True at the start of every inner iteration
-            if (c.pos >= N) {
-                c.pos = 0;
-                if (c.brk0 && c.it > S.min_iter) {
-                    c.done = 1;
-                } else {
-                    c.it++;
-                    if (c.it >= S.max_outer) c.done = 1;
-                    c.brk0 = 1;
-                }
-            }
-        }
+        // ---- loop control (NewtonGaussian.py:50-98, :131-222) ---------------
+        if (tid == 0) fit_advance(L.c, S, false);
         __syncthreads();
         GT_FIT_TICK(7);
     }
@@ -812,6 +942,10 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
             S.E = c.E;
             S.updates = c.updates;
             S.accepted = c.accepted;
+            S.blk = c.blk;
+            S.ninf = c.ninf;
+            S.F = c.F;
+            S.Rsum = c.Rsum;
             if (timing)
                 for (int k = 0; k < 8; k++) S.phase_ticks[k] += ticks[k];
         }

@@ -1,16 +1,19 @@
 """Fitters over the HIP library.
 
-newton_linesearch   NewtonGaussian.linesearch (NewtonGaussian.py:18-109) with
-                    GD_lib._step (GD_lib.py:492-536): device-resident, a fit
-                    advanced by a group of workgroups that share its angles
-                    (gt_fit_run_blocks), FP64 block solves and FP64 energy
-                    deltas.
-gd_linesearch_block GD_lib.linesearch_block (GD_lib.py:82-178): block gradient
-                    descent driven from the host over the batched forward and
-                    derivative kernels.
+newton_linesearch        NewtonGaussian.linesearch (NewtonGaussian.py:18-109)
+                         with GD_lib._step (GD_lib.py:492-536): device-resident,
+                         a fit advanced by a group of workgroups that share its
+                         angles (gt_fit_run_blocks), FP64 solves and FP64 energy
+                         deltas.
+newton_linesearch_block  NewtonGaussian.linesearch_block (NewtonGaussian.py:
+                         112-237): the same kernel updating I, x and r of each
+                         atom in turn.
+gd_linesearch_block      GD_lib.linesearch_block (GD_lib.py:82-178): block
+                         gradient descent driven from the host over the batched
+                         forward and derivative kernels.
 
-Both take the null regulariser and the l2 fidelity, which is what every
-configuration uses (GaussDictCode/__init__.py:177-178, transport_loss.py:567-600).
+The fidelity is l2 (transport_loss.py:567-600); the Newton fitters take any
+regulariser of the regularisation module, evaluated inside the fit kernel.
 """
 from __future__ import annotations
 
@@ -21,6 +24,7 @@ import torch
 
 from . import ops
 from ._lib import check, load, stream_handle
+from .regularisation import as_regulariser
 
 
 class gt_fit_slot(ctypes.Structure):
@@ -37,6 +41,9 @@ class gt_fit_slot(ctypes.Structure):
         ("brk0", ctypes.c_int), ("brk1", ctypes.c_int), ("done", ctypes.c_int), ("pad1", ctypes.c_int),
         ("E", ctypes.c_double), ("updates", ctypes.c_longlong), ("accepted", ctypes.c_longlong),
         ("phase_ticks", ctypes.c_void_p),
+        ("block", ctypes.c_int), ("reg_kind", ctypes.c_int), ("reg_p", ctypes.c_double * 5),
+        ("Fhist", ctypes.c_void_p),
+        ("blk", ctypes.c_int), ("ninf", ctypes.c_int), ("F", ctypes.c_double), ("Rsum", ctypes.c_double),
     ]
 
 
@@ -91,10 +98,17 @@ def permutations(n, max_outer, seed=1):
 
 
 class FitRequest:
-    """Device state of one Newton fit: atoms, residual, step sizes, history."""
+    """Device state of one Newton fit: atoms, residual, step sizes, history.
 
-    def __init__(self, atoms, space, data, max_iter, tol=1e-4, min_iter=1, seed=1, record=True, device=None):
+    reg: a regularisation.Regulariser (None: null); block: update I, x and r
+    of each atom in turn (NewtonGaussian.linesearch_block)."""
+
+    def __init__(self, atoms, space, data, max_iter, tol=1e-4, min_iter=1, seed=1, record=True, device=None,
+                 reg=None, block=False):
         dev = device or atoms.I.device
+        self.reg = as_regulariser(reg)
+        self.block = bool(block)
+        nblk = 3 if self.block else 1
         if not hasattr(max_iter, "__iter__"):
             max_iter = [max_iter, 1]
         max_iter = list(max_iter) + [1] * (2 - len(max_iter))
@@ -114,13 +128,24 @@ class FitRequest:
         self.data = torch.as_tensor(data, device=dev).to(torch.float32).reshape(self.geom.shape).contiguous()
         self.resid = ops.forward(self, space).sub_(self.data)
         r64 = self.resid.double()
-        self.E0 = float(0.5 * (r64 * r64).sum())
+        self.F0 = float(0.5 * (r64 * r64).sum())
+        # E = F + reg(recon) (NewtonGaussian.py:28-30): the sum of the finite
+        # per-atom terms and the count of the others
+        self.ninf0, self.Rsum0 = 0, 0.0
+        if self.reg.kind != 0 and self.N > 0:
+            val = self.reg.terms(self, 0)[0]
+            fin = torch.isfinite(val)
+            self.ninf0 = int((~fin).sum())
+            self.Rsum0 = float(val[fin].sum())
+        self.E0 = self.F0 + (float("inf") if self.ninf0 else self.Rsum0)
         self.perm = torch.as_tensor(permutations(self.N, self.max_outer, seed), device=dev).contiguous()
-        self.h = torch.ones(self.N, dtype=torch.float64, device=dev)
-        nh = self.max_outer * self.max_inner * self.N + 1
+        self.h = torch.ones(nblk * self.N, dtype=torch.float64, device=dev)
+        nh = nblk * self.max_outer * self.max_inner * self.N + 1
         self.Ehist = torch.zeros(nh, dtype=torch.float64, device=dev) if record else None
+        self.Fhist = torch.zeros(nh, dtype=torch.float64, device=dev) if record else None
         if record:
             self.Ehist[0] = self.E0
+            self.Fhist[0] = self.F0
         lib = _declare()
         self.scratch = torch.zeros(lib.gt_fit_scratch_bytes(self.geom.nA), dtype=torch.uint8, device=dev)
         # old residual of the update in flight: lets the energy pass commit at once
@@ -135,7 +160,9 @@ class FitRequest:
             h=self.h.data_ptr(), Ehist=self.Ehist.data_ptr() if record else None, scratch=self.scratch.data_ptr(),
             stage=self.stage.data_ptr(),
             it=0, pos=0, inner=0, jj=0, brk0=1, brk1=1, done=0 if self.N > 0 else 1, pad1=0, E=self.E0,
-            updates=0, accepted=0)
+            updates=0, accepted=0, block=int(self.block), reg_kind=int(self.reg.kind),
+            Fhist=self.Fhist.data_ptr() if record else None, blk=0, ninf=self.ninf0, F=self.F0, Rsum=self.Rsum0)
+        self.slot.reg_p[:] = [float(v) for v in self.reg.params()]
 
     def slot_bytes(self):
         return bytes(self.slot)
@@ -159,15 +186,16 @@ def run_blocks(dev_table, nslots, dev_blocks, nblocks, dim, budget):
                                 nblocks, dim, budget, stream_handle(dev_table.device)), "gt_fit_run_blocks")
 
 
-def newton_fit(atoms, space, data, max_iter, tol=1e-4, min_iter=1, seed=1, budget=None, record=True, group=None):
+def newton_fit(atoms, space, data, max_iter, tol=1e-4, min_iter=1, seed=1, budget=None, record=True, group=None,
+               reg=None, block=False):
     """Run one Newton fit to completion on the device; returns (request, final
     slot).  group: workgroups advancing the fit together (None: default_group
     of the angle count; 1: a single workgroup).  The result does not depend on
     the group size."""
-    req = FitRequest(atoms, space, data, max_iter, tol, min_iter, seed, record)
+    req = FitRequest(atoms, space, data, max_iter, tol, min_iter, seed, record, reg=reg, block=block)
     dev = req.I.device
     table = torch.as_tensor(np.frombuffer(req.slot_bytes(), dtype=np.uint8).copy(), device=dev)
-    budget = budget or max(req.N * req.max_inner, 1)
+    budget = budget or max(req.N * req.max_inner * (3 if req.block else 1), 1)
     group = default_group(req.geom.nA) if group is None else int(group)
     group = max(1, min(group, max_blocks()))
     blocks = torch.as_tensor(block_table([group]), device=dev) if group > 1 else None
@@ -185,31 +213,59 @@ def newton_fit(atoms, space, data, max_iter, tol=1e-4, min_iter=1, seed=1, budge
     return req, s
 
 
-def newton_linesearch(recon, data, max_iter, fidelity=None, reg=None, Radon=None, view=None, guess=None, tol=1e-4,
-                      min_iter=1, **kwargs):
-    """Drop-in for NewtonGaussian.linesearch: fits `recon` in place and returns
-    (recon, E, F) with E == F the energy history (null regulariser)."""
+def _check_fidelity(fidelity):
+    if fidelity is not None and not getattr(fidelity, "is_l2_squared", False):
+        raise NotImplementedError("the fitters take the l2 fidelity (l2_squared_loss) only")
+
+
+def _newton_dropin(recon, data, max_iter, fidelity, reg, Radon, guess, tol, min_iter, block, kwargs):
     if guess is not None:
         raise NotImplementedError("the initial-guess path (KL_GaussRadon) is not part of this build")
+    _check_fidelity(fidelity)
     space = Radon.range if Radon is not None else kwargs["space"]
-    req, s = newton_fit(recon, space, data, max_iter, tol, min_iter)
+    req, s = newton_fit(recon, space, data, max_iter, tol, min_iter, reg=reg, block=block,
+                        group=kwargs.get("group"))
     if s.done == 2:
         raise np.linalg.LinAlgError("singular Newton system")
     recon.I.copy_(req.I)
     recon.x.copy_(req.x)
     recon.r.copy_(req.r)
     E = req.Ehist[: s.jj + 1].cpu().numpy()
-    return recon, E, E.copy()
+    F = req.Fhist[: s.jj + 1].cpu().numpy()
+    return recon, E, F
+
+
+def newton_linesearch(recon, data, max_iter, fidelity=None, reg=None, Radon=None, view=None, guess=None, tol=1e-4,
+                      min_iter=1, **kwargs):
+    """Drop-in for NewtonGaussian.linesearch: fits `recon` in place and returns
+    (recon, E, F): E the energy (fidelity + regulariser) after each update, F
+    the fidelity of each update's proposal."""
+    return _newton_dropin(recon, data, max_iter, fidelity, reg, Radon, guess, tol, min_iter, False, kwargs)
+
+
+def newton_linesearch_block(recon, data, max_iter, fidelity=None, reg=None, Radon=None, view=None, guess=None,
+                            tol=1e-4, min_iter=1, **kwargs):
+    """Drop-in for NewtonGaussian.linesearch_block (NewtonGaussian.py:112-237):
+    for each atom the blocks I, x and r in turn, each with its own step size
+    (x1.3 on accept, /10 on reject), damped Newton step on its diagonal block
+    and projection (I >= 0; x clamped to [-0.99, 0.99]; r as in _step).  The
+    reference slices the Hessian with the string 'Ixr' in place of the spatial
+    dimension (NewtonGaussian.py:135,163,172), which raises inside its try
+    blocks so that its x and r blocks never move; here they do."""
+    return _newton_dropin(recon, data, max_iter, fidelity, reg, Radon, guess, tol, min_iter, True, kwargs)
 
 
 # ---------------------------------------------------------------------------
 # Block gradient descent, GD_lib.linesearch_block (GD_lib.py:82-178)
 # ---------------------------------------------------------------------------
-def gd_linesearch_block(recon, data, max_iter, space, dim="xrI", tol=1e-4, min_iter=1, seed=1):
+def gd_linesearch_block(recon, data, max_iter, space, dim="xrI", tol=1e-4, min_iter=1, seed=1, reg=None,
+                        return_F=False):
     """Block GD over (x, r, I) of each atom with the reference's step-size rules.
-    Energies in FP64.  Divergence from the reference, kept on purpose: on a
-    rejected step the sinogram of the reverted atoms is used again
-    (GD_lib.py:157-160 leaves the rejected sinogram in place)."""
+    Energies (fidelity + regulariser) in FP64.  Divergence from the reference,
+    kept on purpose: on a rejected step the sinogram of the reverted atoms is
+    used again (GD_lib.py:157-160 leaves the rejected sinogram in place).
+    Returns (recon, E), or (recon, E, F) with F the fidelity of each proposal."""
+    reg = as_regulariser(reg)
     dev = recon.I.device
     D = recon.x.shape[1]
     dims = "".join(c for c in "xrI" if c.lower() in dim.lower())
@@ -222,7 +278,8 @@ def gd_linesearch_block(recon, data, max_iter, space, dim="xrI", tol=1e-4, min_i
     data = torch.as_tensor(data.data if hasattr(data, "data") and not isinstance(data, torch.Tensor) else data,
                            device=dev).to(torch.float32)
     R = ops.forward(recon, space)
-    E = [0.5 * float(((R - data).double() ** 2).sum())]
+    F = [0.5 * float(((R - data).double() ** 2).sum())]
+    E = [F[0] + (reg(recon) if reg.kind else 0.0)]
     rs = np.random.RandomState(seed)
     resid = R - data
     for it in range(max_iter[0]):
@@ -240,6 +297,8 @@ def gd_linesearch_block(recon, data, max_iter, space, dim="xrI", tol=1e-4, min_i
                         grad = g[1:1 + D]
                     else:
                         grad = g[1 + D:]
+                    if reg.kind:  # grad += reg.grad(recon[j], axis=dim[t]) (GD_lib.py:132)
+                        grad = grad + np.asarray(reg.grad(one, axis=name)).reshape(-1)
                     arr = {"I": recon.I, "x": recon.x, "r": recon.r}[name]
                     old = arr[j].detach().clone()
                     h[name][j] = min(h[name][j], 1.0 / max(np.sqrt(np.sum(grad * grad)), eps))
@@ -248,7 +307,9 @@ def gd_linesearch_block(recon, data, max_iter, space, dim="xrI", tol=1e-4, min_i
                     arr[j] = new
                     Rn = ops.forward(recon, space)
                     rn = Rn - data
-                    En = 0.5 * float((rn.double() ** 2).sum())
+                    Fn = 0.5 * float((rn.double() ** 2).sum())
+                    F.append(Fn)
+                    En = Fn + (reg(recon) if reg.kind else 0.0)
                     if En > E[-1]:
                         arr[j] = old
                         E.append(E[-1])
@@ -265,4 +326,17 @@ def gd_linesearch_block(recon, data, max_iter, space, dim="xrI", tol=1e-4, min_i
                     break
         if brk[0] and it > min_iter:
             break
+    if return_F:
+        return recon, np.array(E), np.array(F)
     return recon, np.array(E)
+
+
+def linesearch_block(recon, data, max_iter, fidelity=None, reg=None, Radon=None, view=None, dim="xrI", guess=None,
+                     tol=1e-4, min_iter=1, **kwargs):
+    """Drop-in for GD_lib.linesearch_block (GD_lib.py:82-178): fits `recon`
+    in place, returns (recon, E, F)."""
+    if guess is not None:
+        raise NotImplementedError("the initial-guess path (KL_GaussRadon) is not part of this build")
+    _check_fidelity(fidelity)
+    space = Radon.range if Radon is not None else kwargs["space"]
+    return gd_linesearch_block(recon, data, max_iter, space, dim, tol, min_iter, reg=reg, return_F=True)

@@ -140,9 +140,45 @@ int gt_ft_eval(const gt_atoms *atoms, const gt_kgrid *grid, float *out, void *ws
                void *stream);
 
 /*
+ * Regularisers (GaussDictCode/regularisation.py:22-442).  Each is a sum over
+ * atoms of one term per atom; params (GT_REG_NPARAM doubles, unused ones 0):
+ *   GT_REG_NULL     -                                    null      :22-38
+ *   GT_REG_JOUBERT  a, b, c, d, e: -a log I + b/2|x|^2 + c v - d log v - e,
+ *                   v = r^dim (iso) or the product of the diagonal slots;
+ *                   from Joubert(dim, intensity, location, (r0, r1)):
+ *                   a = intensity, b = location, c = r1, d = r1 r0,
+ *                   e = r0 r1 (1 - log r0)                Joubert   :41-108
+ *   GT_REG_MASS     m, k, s: s (|I R| - m)^k, R = r^dim (iso) or
+ *                   1 / product of the diagonal slots     Mass      :111-210
+ *   GT_REG_VOLUME   r0^dim, k, s: s |v - r0^dim|^k, v = r^-dim (iso) or the
+ *                   product of the diagonal slots         Volume    :213-318
+ *   GT_REG_RADIUS   r0, s, -k: s sum (r_i - r0)^-k over the diagonal slots
+ *                   (iso: dim (r - r0)^-k)                Radius    :321-386
+ *   GT_REG_ISO      s: s/2 sum of the squared off-diagonal slots  Iso :389-442
+ * A term is +inf outside its domain (Joubert: I <= 0 or a diagonal slot <= 0;
+ * Mass: I < 0; Volume: a diagonal slot <= 0; Radius: a diagonal slot <= r0).
+ * Gradients and Hessians are the exact derivatives of the term.
+ */
+#define GT_REG_NULL 0
+#define GT_REG_JOUBERT 1
+#define GT_REG_MASS 2
+#define GT_REG_VOLUME 3
+#define GT_REG_RADIUS 4
+#define GT_REG_ISO 5
+#define GT_REG_NPARAM 5
+
+/* Per-atom regulariser terms of all atoms (reg(atoms), reg.grad, reg.hess,
+ * regularisation.py:22-442, for one atom at a time there), FP64:
+ * val[n], and unless NULL grad[n][P] and hess[n][P][P], P = 1 + dim +
+ * (iso ? 1 : dim*(dim+1)/2).  params is a HOST array of GT_REG_NPARAM. */
+int gt_reg_eval(const gt_atoms *atoms, int kind, const double *params, double *val, double *grad, double *hess,
+                void *stream);
+
+/*
  * Device-resident damped-Newton fitter (NewtonGaussian.linesearch with
- * GD_lib._step, NewtonGaussian.py:18-109, GD_lib.py:492-536; null
- * regulariser, l2 fidelity).  One gt_fit_slot per fit request, in device
+ * GD_lib._step, NewtonGaussian.py:18-109, GD_lib.py:492-536, or with block =
+ * 1 NewtonGaussian.linesearch_block, :112-237; any of the regularisers
+ * above, l2 fidelity).  One gt_fit_slot per fit request, in device
  * memory; one workgroup (gt_fit_run) or a group of workgroups
  * (gt_fit_run_blocks) advances one slot.  The host fills the problem and
  * state pointers, the initial residual (forward - data) and energy, and the
@@ -160,14 +196,15 @@ typedef struct gt_fit_slot {
     int lenT, sorted;
     int max_outer, max_inner, min_iter;
     int quota;        /* if > 0: at most this many atom updates per gt_fit_run (else `budget`) */
-    double tol;       /* convergence threshold on ||dx|| */
+    double tol;       /* convergence threshold: on ||dx|| (block = 0), on
+                         ||new - old|| / ||old|| of the updated block (block = 1) */
     const float *T, *w0, *w1, *p0, *p1; /* geometry (gt_geom layout) */
     const int *perm;  /* [max_outer][N] atom order of each outer iteration */
     /* state (read-write) */
     float *I, *x, *r; /* atoms, gt_atoms layout */
     float *resid;     /* [nA][n0][n1] current projection minus data */
-    double *h;        /* [N] per-atom step sizes, initialised to 1 */
-    double *Ehist;    /* [max_outer*max_inner*N + 1] energy after each update, or NULL */
+    double *h;        /* [block ? 3 : 1][N] per-atom step sizes, initialised to 1 */
+    double *Ehist;    /* [(block ? 3 : 1)*max_outer*max_inner*N + 1] energy after each update, or NULL */
     float *scratch;   /* gt_fit_scratch_bytes(nA) bytes, zeroed before the first run */
     float *stage;     /* [nA][n0][n1] old residual of the update in flight, or NULL: with it
                          the energy pass commits the proposal at once and a rejected one is
@@ -175,13 +212,26 @@ typedef struct gt_fit_slot {
     /* progress (read-write) */
     int it, pos, inner, jj;  /* outer iteration, position in perm, inner step, history index */
     int brk0, brk1;          /* convergence flags of the current outer / inner loop */
-    int done;                /* 1: converged or out of iterations; 2: singular solve;
+    int done;                /* 1: converged or out of iterations; 2: singular solve (block = 0);
                                 3: the workgroups of a group failed to meet (see gt_fit_run_blocks) */
     int pad1;
-    double E;                /* current energy 1/2 ||resid||^2 */
+    double E;                /* current energy: fidelity + regulariser */
     long long updates;       /* atom updates performed */
     long long accepted;      /* accepted updates */
     long long *phase_ticks;  /* [8] optional: wall-clock ticks (100 MHz) spent per phase, or NULL */
+    /* variant and regulariser (read-only) */
+    int block;               /* 0: one Newton step in all parameters per update (linesearch);
+                                1: I, then x, then r of the atom, each with its own step size,
+                                projection and inner loop (linesearch_block) */
+    int reg_kind;            /* GT_REG_* */
+    double reg_p[GT_REG_NPARAM];
+    double *Fhist;           /* like Ehist: fidelity of each update's proposal (also when
+                                rejected, as the reference's F records), or NULL */
+    /* regularised energy (read-write): E = F + (ninf ? +inf : Rsum) */
+    int blk;                 /* current parameter block (block variant), 0 otherwise */
+    int ninf;                /* atoms whose regulariser term is not finite */
+    double F;                /* fidelity 1/2 ||resid||^2 */
+    double Rsum;             /* sum of the finite regulariser terms */
 } gt_fit_slot;
 
 /* nA must be <= GT_FIT_MAX_ANGLES and n0, n1 <= GT_FIT_MAX_DETECTOR. dim 2 or 3. */

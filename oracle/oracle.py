@@ -18,7 +18,11 @@ reference in NumPy:
 * ``l2_squared_loss``                    GaussDictCode/transport_loss.py:567-585
 * ``newton_linesearch``                  NewtonGaussian.py:18-109 with
                                          GD_lib.py:492-536 (``_step``)
+* ``newton_fit`` (also block variant)    NewtonGaussian.py:18-237, any regulariser
 * ``gd_linesearch_block``                GD_lib.py:82-178
+* ``Reg`` (regulariser terms)            GaussDictCode/regularisation.py:22-442;
+  values restated from the ``__call__`` methods, gradients and Hessians by
+  FP64 automatic differentiation of those values (torch.autograd on the CPU)
 """
 from __future__ import annotations
 
@@ -441,3 +445,207 @@ def gd_linesearch_block(I, x, r, geom, data, max_iter, dim="xrI", tol=1e-4, min_
         if brk[0] and it > min_iter:
             break
     return I, x, r, np.array(E)
+
+
+# ---------------------------------------------------------------------------
+# Regularisers, GaussDictCode/regularisation.py:22-442
+# ---------------------------------------------------------------------------
+class Reg:
+    """One regulariser, built from the reference's constructor arguments:
+
+    Reg("null", dim) | Reg("Joubert", dim, intensity, location, (r0, r1))
+    | Reg("Mass", dim, m, k=2, s=1) | Reg("Volume", dim, r0, k=2, s=1)
+    | Reg("Radius", dim, r0, s, k) | Reg("Iso", dim, s)
+
+    term(th) is the value of one atom's term (th = [I, x, r], one r if iso) as
+    the reference's __call__ sums it; derivs(th) its gradient and Hessian by
+    automatic differentiation.  Domain tests as in the reference, except that
+    Joubert's looks at the diagonal slots only (its logarithm's argument;
+    regularisation.py:63-64 tests every slot)."""
+
+    def __init__(self, name, dim, *args):
+        self.name, self.dim, self.args = name, dim, args
+        if name == "Joubert":
+            intensity, location, radius = args
+            self.p = (intensity, location, radius[1], radius[1] * radius[0],
+                      float(np.prod(radius)) * (1 - np.log(radius[0])))          # :41-51
+        elif name in ("Mass", "Volume"):
+            a = list(args) + [2, 1][len(args) - 1:]
+            self.p = (abs(a[0]) ** dim, a[1], a[2]) if name == "Volume" else tuple(a)  # :111-117, :213-217
+        elif name == "Radius":
+            r0, s_, k = args
+            self.p = (r0, s_, -k)                                                # :321-327
+        elif name == "Iso":
+            self.p = (args[0],)
+        elif name != "null":
+            raise ValueError(name)
+
+    def _value(self, th, iso):
+        import torch
+        D = self.dim
+        I, x, r = th[0], th[1:1 + D], th[1 + D:]
+        if self.name == "null":
+            return th.sum() * 0
+        if self.name == "Joubert":                                                # :53-70
+            a, b, c, d, e = self.p
+            v = r[0] ** D if iso else torch.prod(r[:D])
+            return -a * torch.log(I) + (b / 2) * (x * x).sum() + c * v - d * torch.log(v) - e
+        if self.name == "Mass":                                                   # :114-129
+            m, k, s_ = self.p
+            V = I * r[0] ** D if iso else I / torch.prod(r[:D])
+            # |V| with the derivative of V at V = 0 (the one-sided derivative the
+            # reference's grad formula gives, regularisation.py:131-160)
+            return s_ * (torch.where(V >= 0, V, -V) - m) ** k
+        if self.name == "Volume":                                                 # :218-236
+            p0, k, s_ = self.p
+            dd = r[0] ** -D - p0 if iso else torch.prod(r[:D]) - p0
+            return s_ * torch.abs(dd) ** k
+        if self.name == "Radius":                                                 # :329-343
+            r0, s_, kk = self.p
+            return s_ * (D * (r[0] - r0) ** kk if iso else ((r[:D] - r0) ** kk).sum())
+        if self.name == "Iso":                                                    # :398-407
+            return th.sum() * 0 if iso else self.p[0] * (r[D:] * r[D:]).sum() / 2
+        raise ValueError(self.name)
+
+    def _outside(self, th, iso):
+        D = self.dim
+        I, r = th[0], np.asarray(th[1 + D:])
+        diag = np.repeat(r[:1], D) if iso else r[:D]
+        if self.name == "Joubert":
+            return I <= 0 or diag.min() <= 0
+        if self.name == "Mass":
+            return I < 0
+        if self.name == "Volume":
+            return diag.min() <= 0
+        if self.name == "Radius":
+            return diag.min() <= self.p[0]
+        return False
+
+    def term(self, th, iso):
+        import torch
+        th = np.asarray(th, float)
+        if self._outside(th, iso):
+            return np.inf
+        return float(self._value(torch.as_tensor(th, dtype=torch.float64), iso))
+
+    def derivs(self, th, iso):
+        import torch
+        f = lambda t: self._value(t, iso)
+        t = torch.as_tensor(np.asarray(th, float), dtype=torch.float64)
+        g = torch.autograd.functional.jacobian(f, t)
+        H = torch.autograd.functional.hessian(f, t)
+        return g.numpy(), H.numpy()
+
+    def total(self, I, x, r):
+        iso = r.shape[1] == 1
+        vals = [self.term(np.concatenate(([I[j]], x[j], r[j])), iso) for j in range(I.size)]
+        return float(np.sum(vals)) if vals else 0.0
+
+
+def _project_block(D, iso, t, th_old, d):
+    """Projections of NewtonGaussian.linesearch_block (:159-187): block t of
+    th = [I, x, r] moved by d, then I >= 0, x clamped to [-0.99, 0.99], r
+    with the sign rule of GD_lib._step (the reference writes the 3D rule only)."""
+    th = th_old.copy()
+    o = (0, 1, 1 + D)[t]
+    th[o:o + d.size] += d
+    if t == 0:
+        th[0] = max(0.0, th[0])
+    elif t == 1:
+        th[1:1 + D] = np.maximum(-0.99, np.minimum(0.99, th[1:1 + D]))
+    else:
+        _, _, th[1 + D:] = _project(D, iso, 0.0, np.zeros(D), th[1 + D:], np.zeros(th.size))
+    return th
+
+
+def newton_fit(I, x, r, geom, data, max_iter, tol=1e-4, min_iter=1, seed=1, reg=None, block=False):
+    """NewtonGaussian.linesearch (block=False, :18-109, with GD_lib._step) or
+    NewtonGaussian.linesearch_block (block=True, :112-237) with regulariser
+    `reg` (a Reg, None = null), FP64.  Energies E = F + reg(all atoms),
+    recomputed in full after every update as the reference does.
+
+    The block variant restates the intent: its blocks are sliced with the
+    spatial dimension (the reference slices with the string 'Ixr', :135,163,172,
+    so that its x and r solves raise inside the try and are skipped).
+
+    Returns (I, x, r, E, F): E after each update, F the fidelity of each
+    update's proposal (also when rejected, as the reference records it)."""
+    D = geom.dim
+    I, x = np.array(I, float).reshape(-1), np.array(x, float).reshape(-1, D)
+    r = np.array(r, float).reshape(I.size, -1)
+    iso = r.shape[1] == 1
+    n = I.size
+    if not hasattr(max_iter, "__iter__"):
+        max_iter = [max_iter, 1]
+    max_iter = list(max_iter) + [1] * (2 - len(max_iter))
+    eps = 1e-4
+    nblk = 3 if block else 1
+    h = [[1.0] * n for _ in range(nblk)]
+    data = np.asarray(data, float)
+    R = radproj(I, x, r, geom)
+    F = [l2_squared_loss(R, data)]
+    E = [F[0] + (reg.total(I, x, r) if reg is not None else 0.0)]
+    rs = np.random.RandomState(seed)
+    th_of = lambda j: np.concatenate(([I[j]], x[j], r[j]))
+
+    def put(j, th):
+        I[j], x[j], r[j] = th[0], th[1:1 + D], th[1 + D:]
+
+    for it in range(max_iter[0]):
+        brk = [True, True]
+        for j in rs.permutation(np.arange(n)):
+            for t in range(nblk):
+                for _ in range(max_iter[1]):
+                    brk[1] = True
+                    Rj = radproj(I[j:j + 1], x[j:j + 1], r[j:j + 1], geom)
+                    _, df, ddf = derivs(I[j:j + 1], x[j:j + 1], r[j:j + 1], geom, data + Rj - R, "L2", 2)
+                    df, ddf = df[0].copy(), ddf[0].copy()
+                    old = th_of(j)
+                    if reg is not None:
+                        gr, Hr = reg.derivs(old, iso)
+                        df += gr
+                        ddf += Hr
+                    ddf[np.diag_indices_from(ddf)] += 1.0 / h[t][j]
+                    if not block:
+                        dx = np.linalg.solve(ddf, -df)
+                        I2, x2, r2 = _project(D, iso, old[0], old[1:1 + D], old[1 + D:], dx)
+                        new = np.concatenate(([I2], x2, r2))
+                    else:
+                        o, m = (0, 1) if t == 0 else ((1, D) if t == 1 else (1 + D, old.size - 1 - D))
+                        if t == 0:
+                            if not ddf[0, 0] > 1e-8:
+                                break
+                            d = np.array([-df[0] / ddf[0, 0]])
+                        else:
+                            try:
+                                d = np.linalg.solve(ddf[o:o + m, o:o + m], -df[o:o + m])
+                            except np.linalg.LinAlgError:
+                                break
+                        new = _project_block(D, iso, t, old, d)
+                    put(j, new)
+                    Rnew = R + radproj(I[j:j + 1], x[j:j + 1], r[j:j + 1], geom) - Rj
+                    Fn = l2_squared_loss(Rnew, data)
+                    En = Fn + (reg.total(I, x, r) if reg is not None else 0.0)
+                    F.append(Fn)
+                    if En > E[-1]:
+                        put(j, old)
+                        E.append(E[-1])
+                        if h[t][j] > 2 * eps:
+                            brk = [False, False]
+                        h[t][j] /= 10
+                    else:
+                        R = Rnew
+                        E.append(En)
+                        if block:
+                            moved = np.linalg.norm((new - old)[o:o + m]) > tol * np.linalg.norm(old[o:o + m])
+                        else:
+                            moved = np.sqrt(np.sum(dx * dx)) > tol
+                        if moved:
+                            brk = [False, False]
+                        h[t][j] *= 1.3 if block else 2
+                    h[t][j] = min(max(h[t][j], eps), 1 / eps)
+                    if brk[1]:
+                        break
+        if brk[0] and it > min_iter:
+            break
+    return I, x, r, np.array(E), np.array(F)

@@ -96,3 +96,29 @@ def test_workload_cholesky_convention():
         y = rng.normal(size=3)
         assert np.isclose(30 * np.sum((U @ y) ** 2), 0.5 * y @ L[j] @ L[j].T @ y)
     assert np.all((x >= -0.7) & (x <= 0.7)) and np.all((I >= 0.5) & (I <= 1.5))
+
+
+def test_fit_slot_layout_matches_header(tmp_path):
+    """The ctypes mirrors of gt_fit_slot / gt_fit_block have the C layout of
+    include/gausstomo.h (offset of every field, and the size), checked by
+    compiling the header with the host C compiler."""
+    import shutil
+    import subprocess
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no host C compiler")
+    F = importlib.import_module("2018_dictionary_tomography_amd.fit")
+    checks = []
+    for cls, cname in ((F.gt_fit_slot, "gt_fit_slot"), (F.gt_fit_block, "gt_fit_block")):
+        checks.append((f"sizeof({cname})", ctypes.sizeof(cls)))
+        for name, _ in cls._fields_:
+            checks.append((f"offsetof({cname}, {name})", getattr(cls, name).offset))
+    src = "#include <stdio.h>\n#include <stddef.h>\n#include \"gausstomo.h\"\nint main(void) {\n"
+    src += "".join(f'    printf("%zu\\n", (size_t){expr});\n' for expr, _ in checks) + "    return 0;\n}\n"
+    (tmp_path / "layout.c").write_text(src)
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(tmp_path / "layout.c"), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    for (expr, want), g in zip(checks, got):
+        assert g == want, (expr, g, want)
