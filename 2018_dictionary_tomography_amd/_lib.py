@@ -44,6 +44,20 @@ class gt_kgrid(ctypes.Structure):
                 ("sorted", ctypes.c_int), ("k0", ctypes.c_void_p), ("k1", ctypes.c_void_p), ("k2", ctypes.c_void_p)]
 
 
+class gt_kplanes(ctypes.Structure):
+    _fields_ = [("form", ctypes.c_int), ("nA", ctypes.c_int), ("m0", ctypes.c_int), ("m1", ctypes.c_int),
+                ("sorted", ctypes.c_int), ("k0", ctypes.c_void_p), ("k1", ctypes.c_void_p), ("w0", ctypes.c_void_p),
+                ("w1", ctypes.c_void_p)]
+
+
+class gt_kplanes64(ctypes.Structure):
+    _fields_ = gt_kplanes._fields_
+
+
+class gt_atoms64(ctypes.Structure):
+    _fields_ = gt_atoms._fields_
+GT_KP_3D, GT_KP_2P5D, GT_KP_2D_LINE, GT_KP_2D_GRID = 0, 1, 2, 3
+
 _P = ctypes.POINTER
 _vp, _sz, _ci = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
 
@@ -60,6 +74,12 @@ SIGNATURES = {
     "gt_volume": (_ci, [_P(gt_atoms), _P(gt_vol), _vp, _vp, _sz, _vp]),
     "gt_ft_workspace": (_sz, [_P(gt_atoms), _P(gt_kgrid)]),
     "gt_ft_eval": (_ci, [_P(gt_atoms), _P(gt_kgrid), _vp, _vp, _sz, _vp]),
+    "gt_ft_planes_workspace": (_sz, [_ci, _P(gt_kplanes)]),
+    "gt_ft_planes_eval": (_ci, [_P(gt_atoms), _P(gt_kplanes), _vp, _vp, _sz, _vp]),
+    "gt_ft_planes_workspace_f64": (_sz, [_ci, _P(gt_kplanes64)]),
+    "gt_ft_planes_eval_f64": (_ci, [_P(gt_atoms64), _P(gt_kplanes64), _vp, _vp, _sz, _vp]),
+    "gt_ft_derivs_workspace": (_sz, [_P(gt_atoms), _P(gt_kplanes)]),
+    "gt_ft_derivs": (_ci, [_P(gt_atoms), _P(gt_kplanes), _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "gt_reg_eval": (_ci, [_P(gt_atoms), _ci, _P(ctypes.c_double), _vp, _vp, _vp, _vp]),
     "gt_fit_run": (_ci, [_vp, _ci, _ci, _ci, _vp]),
     "gt_fit_scratch_bytes": (_sz, [_ci]),

@@ -222,11 +222,15 @@ class ProjSpace:
 
 
 class ProjElement:
+    """Projection data (float32), or its Fourier-slice transform (complex)."""
+
     def __init__(self, space, u, device=None):
         self.space = space
         device = device or default_device()
         if isinstance(u, torch.Tensor):
-            u = u.to(device=device, dtype=torch.float32)
+            u = u.to(device=device, dtype=u.dtype if u.is_complex() else torch.float32)
+        elif np.iscomplexobj(u):
+            u = torch.as_tensor(np.asarray(u), device=device)
         else:
             u = torch.as_tensor(np.asarray(u, dtype=np.float32), device=device)
         if u.numel() == 1:

@@ -22,7 +22,10 @@
  *                    GaussDictCode/atomFuncs.py:130-158)
  *   gt_volume    <- VolProj          GaussDictCode/bin/NewtonBases.py:691-695
  *                                     GaussDictCode/bin/NewtonBases2D.py:252-256
- *   gt_ft_eval   <- evaluate         GaussDictCode/bin/FTGaussian3D.py:12-52
+ *   gt_ft_eval   <- evaluate         GaussDictCode/bin/FTGaussian3D.py:12-52 (grid)
+ *   gt_ft_planes_eval <- evaluate    GaussDictCode/bin/FTGaussian3D.py:12-38 (planes),
+ *                                     GaussDictCode/bin/FTGaussian2D.py:13-35
+ *   gt_ft_derivs <- derivs           GaussDictCode/bin/FTGaussian3D.py:55-92
  */
 #ifndef GAUSSTOMO_H
 #define GAUSSTOMO_H
@@ -138,6 +141,57 @@ int gt_volume(const gt_atoms *atoms, const gt_vol *vol, float *u, void *ws, size
 size_t gt_ft_workspace(const gt_atoms *atoms, const gt_kgrid *grid);
 int gt_ft_eval(const gt_atoms *atoms, const gt_kgrid *grid, float *out, void *ws, size_t ws_bytes,
                void *stream);
+
+/*
+ * Fourier evaluation on sample planes (FTGaussian3D.evaluate hyperplane and
+ * 2.5D branches, GaussDictCode/bin/FTGaussian3D.py:12-38,113-148,167-196;
+ * FTGaussian2D.evaluate, GaussDictCode/bin/FTGaussian2D.py:13-35,53-100):
+ *   out[a][i0][i1] = sum over FT-space atoms of I exp(-|L k|^2/2) exp(-i x.k),
+ *   |L k|^2 < 30 (3D atoms) or 60 (2D atoms), rk = L k with L lower-triangular
+ *   from the slots (3D: rk_1 = r3 k_0 + r1 k_1, rk_2 = r5 k_0 + r4 k_1 + r2 k_2;
+ *   2D: rk_1 = r2 k_0 + r1 k_1); isotropic: |k|^2 / r^2 (3D), r^2 |k|^2 (2D).
+ * Sample forms (k for sample (a, i0, i1)):
+ */
+#define GT_KP_3D 0      /* dim 3: k0[i0] w0[a] + k1[i1] w1[a]; w0, w1 [nA][3]               */
+#define GT_KP_2P5D 1    /* dim 3: (k0[i0] w0[a][0], k0[i0] w0[a][1], k1[i1]); w0 [nA][2]     */
+#define GT_KP_2D_LINE 2 /* dim 2: k0[i0] w0[a]; w0 [nA][2]; m1 = 1, k1 unused                */
+#define GT_KP_2D_GRID 3 /* dim 2: (k0[i0], k1[i1]); nA = 1, w0 / w1 unused                   */
+typedef struct gt_kplanes {
+    int form;
+    int nA, m0, m1;
+    int sorted; /* 1 if k0 (and k1) ascend: enables exact culling */
+    const float *k0, *k1, *w0, *w1;
+} gt_kplanes;
+/* FP64 variants (the reference's 2D evaluator has float64 signatures). */
+typedef struct gt_kplanes64 {
+    int form;
+    int nA, m0, m1;
+    int sorted;
+    const double *k0, *k1, *w0, *w1;
+} gt_kplanes64;
+typedef struct gt_atoms64 {
+    int dim, n, iso;
+    const double *I, *x, *r;
+} gt_atoms64;
+
+/* out: [nA][m0][m1] complex64 (interleaved floats) / complex128. */
+size_t gt_ft_planes_workspace(int n_atoms, const gt_kplanes *planes);
+int gt_ft_planes_eval(const gt_atoms *atoms, const gt_kplanes *planes, float *out, void *ws, size_t ws_bytes,
+                      void *stream);
+size_t gt_ft_planes_workspace_f64(int n_atoms, const gt_kplanes64 *planes);
+int gt_ft_planes_eval_f64(const gt_atoms64 *atoms, const gt_kplanes64 *planes, double *out, void *ws,
+                          size_t ws_bytes, void *stream);
+
+/* Complex L2 derivatives over hyperplane samples (FTGaussian3D.derivs,
+ * GaussDictCode/bin/FTGaussian3D.py:55-92,199-652), for every 3D FT-space atom
+ * independently against all of C ([nA][m0][m1] complex64, forms GT_KP_3D /
+ * GT_KP_2P5D): f[n] = 1/2 sum |g - C|^2, g[n][P] = Re sum (g - C) conj(dg),
+ * H[n][P][P] = Re sum dg conj(dg)^T + (g - C) conj(ddg), g cut where
+ * |L k|^2 > 60; P = 10 (iso 5), order [I, x, slots].  FP64 outputs; f and H may
+ * be NULL.  Workspace: gt_ft_derivs_workspace. */
+size_t gt_ft_derivs_workspace(const gt_atoms *atoms, const gt_kplanes *planes);
+int gt_ft_derivs(const gt_atoms *atoms, const gt_kplanes *planes, const float *C, double *f, double *g, double *H,
+                 void *ws, size_t ws_bytes, void *stream);
 
 /*
  * Regularisers (GaussDictCode/regularisation.py:22-442).  Each is a sum over

@@ -649,3 +649,89 @@ def newton_fit(I, x, r, geom, data, max_iter, tol=1e-4, min_iter=1, seed=1, reg=
         if brk[0] and it > min_iter:
             break
     return I, x, r, np.array(E), np.array(F)
+
+
+# ---------------------------------------------------------------------------
+# Fourier-space derivatives on hyperplane samples, GaussDictCode/bin/FTGaussian3D.py:55-92, 199-652
+# ---------------------------------------------------------------------------
+_FT_ROW = (0, 1, 2, 1, 2, 2)   # slot -> row of L in rk = L k (FTGaussian3D.py:231-234)
+_FT_COL = (0, 1, 2, 0, 1, 0)   # slot -> column
+
+
+def ft_plane_points(k0, k1, w0, w1=None):
+    """Sample points of FTGaussian3D's hyperplane samplers: (nA, m0, m1, 3) with
+    k = k0[i0] w0[a] + k1[i1] w1[a], or the 2.5D form (k0 w0[a,0], k0 w0[a,1], k1)."""
+    k0, k1, w0 = (np.asarray(v, float) for v in (k0, k1, w0))
+    if w1 is None:
+        K = np.stack(np.broadcast_arrays(k0[None, :, None] * w0[:, None, None, 0],
+                                         k0[None, :, None] * w0[:, None, None, 1],
+                                         np.broadcast_to(k1[None, None, :], (w0.shape[0], k0.size, k1.size))), -1)
+    else:
+        w1 = np.asarray(w1, float)
+        K = k0[None, :, None, None] * w0[:, None, None, :] + k1[None, None, :, None] * w1[:, None, None, :]
+    return K
+
+
+def ft_derivs(I, x, r, k0, k1, w0, w1, C, iso=False):
+    """Per atom (each against all of C, as FTGaussian3D.derivs does for atom 0):
+    f = 1/2 sum |g - C|^2, Df = Re sum (g - C) conj(dg), DDf = Re sum dg conj(dg)^T
+    + (g - C) conj(ddg) over the hyperplane samples, g = I exp(-|L k|^2/2 - i x.k)
+    (L lower-triangular from the slots, rk = L k; iso: |k|^2 / r^2), cut where
+    |L k|^2 > 60 (FTGaussian3D.py:236-242, 456-462).  dg and ddg restate
+    __derivs_aniso_aux / __derivs_iso_aux (:199-305, :436-497) per sample in
+    FP64.  Returns f (n,), Df (n, P), DDf (n, P, P), P = 10 (5 iso)."""
+    K = ft_plane_points(k0, k1, w0, w1).reshape(-1, 3)
+    C = np.asarray(C, complex).reshape(-1)
+    I, x = np.asarray(I, float).reshape(-1), np.asarray(x, float).reshape(-1, 3)
+    r = np.asarray(r, float).reshape(I.size, -1)
+    P = 5 if iso else 10
+    f, Df, DDf = np.zeros(I.size), np.zeros((I.size, P)), np.zeros((I.size, P, P))
+    for j in range(I.size):
+        phase = np.exp(-1j * (K @ x[j]))
+        if iso:
+            rr = r[j, 0]
+            N = (K * K).sum(1) / rr ** 2
+            inside = N <= 60
+            G = np.exp(-0.5 * N) * phase * inside
+            g = I[j] * G
+            dg = np.zeros((K.shape[0], 5), complex)
+            ddg = np.zeros((K.shape[0], 5, 5), complex)
+            dg[:, 0] = G
+            dg[:, 1:4] = -1j * K * g[:, None]
+            dg[:, 4] = N * g / rr
+            ddg[:, 0, 1:4] = -1j * K * G[:, None]
+            ddg[:, 0, 4] = N * G / rr
+            ddg[:, 1:4, 1:4] = -K[:, :, None] * K[:, None, :] * g[:, None, None]
+            ddg[:, 1:4, 4] = -1j * K * dg[:, 4:5]
+            ddg[:, 4, 4] = (N - 3) * N * g / rr ** 2
+        else:
+            L = np.zeros((3, 3))
+            for s in range(6):
+                L[_FT_ROW[s], _FT_COL[s]] = r[j, s]
+            rk = K @ L.T
+            N = (rk * rk).sum(1)
+            inside = N <= 60
+            G = np.exp(-0.5 * N) * phase * inside
+            g = I[j] * G
+            dg = np.zeros((K.shape[0], 10), complex)
+            ddg = np.zeros((K.shape[0], 10, 10), complex)
+            dg[:, 0] = G
+            dg[:, 1:4] = -1j * K * g[:, None]
+            for s in range(6):
+                a, b = _FT_ROW[s], _FT_COL[s]
+                dg[:, 4 + s] = -rk[:, a] * K[:, b] * g
+                ddg[:, 0, 4 + s] = -rk[:, a] * K[:, b] * G
+                ddg[:, 1:4, 4 + s] = -1j * K * dg[:, 4 + s:5 + s]
+                for t in range(6):
+                    c, d = _FT_ROW[t], _FT_COL[t]
+                    ddg[:, 4 + s, 4 + t] = g * (rk[:, a] * K[:, b] * rk[:, c] * K[:, d] - (a == c) * K[:, b] * K[:, d])
+            ddg[:, 0, 1:4] = -1j * K * G[:, None]
+            ddg[:, 1:4, 1:4] = -K[:, :, None] * K[:, None, :] * g[:, None, None]
+        iu = np.triu_indices(P, 1)
+        for i, l in zip(*iu):
+            ddg[:, l, i] = ddg[:, i, l]
+        res = g - C
+        f[j] = 0.5 * np.sum(np.abs(res) ** 2)
+        Df[j] = np.real(res @ np.conj(dg))
+        DDf[j] = np.real(np.einsum("pi,pl->il", dg, np.conj(dg)) + np.einsum("p,pil->il", res, np.conj(ddg)))
+    return f, Df, DDf

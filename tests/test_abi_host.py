@@ -98,10 +98,10 @@ def test_workload_cholesky_convention():
     assert np.all((x >= -0.7) & (x <= 0.7)) and np.all((I >= 0.5) & (I <= 1.5))
 
 
-def test_fit_slot_layout_matches_header(tmp_path):
-    """The ctypes mirrors of gt_fit_slot / gt_fit_block have the C layout of
-    include/gausstomo.h (offset of every field, and the size), checked by
-    compiling the header with the host C compiler."""
+def test_struct_layouts_match_header(tmp_path):
+    """The ctypes mirrors of every struct of include/gausstomo.h have its C
+    layout (offset of every field, and the size), checked by compiling the
+    header with the host C compiler."""
     import shutil
     import subprocess
 
@@ -109,8 +109,11 @@ def test_fit_slot_layout_matches_header(tmp_path):
     if cc is None:
         pytest.skip("no host C compiler")
     F = importlib.import_module("2018_dictionary_tomography_amd.fit")
+    L = importlib.import_module("2018_dictionary_tomography_amd._lib")
     checks = []
-    for cls, cname in ((F.gt_fit_slot, "gt_fit_slot"), (F.gt_fit_block, "gt_fit_block")):
+    for cls, cname in ((F.gt_fit_slot, "gt_fit_slot"), (F.gt_fit_block, "gt_fit_block"), (L.gt_atoms, "gt_atoms"),
+                       (L.gt_geom, "gt_geom"), (L.gt_vol, "gt_vol"), (L.gt_kgrid, "gt_kgrid"),
+                       (L.gt_kplanes, "gt_kplanes"), (L.gt_kplanes64, "gt_kplanes64"), (L.gt_atoms64, "gt_atoms64")):
         checks.append((f"sizeof({cname})", ctypes.sizeof(cls)))
         for name, _ in cls._fields_:
             checks.append((f"offsetof({cname}, {name})", getattr(cls, name).offset))
