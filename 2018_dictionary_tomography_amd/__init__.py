@@ -8,16 +8,17 @@ The package name starts with a digit, so import it with
 """
 from . import _lib, ops, spaces
 from ._lib import GTError
-from .atomFuncs import GaussTomo
+from .atomFuncs import GaussTomo, SingleParticleGaussTomo
 from .losses import l2_squared_loss
 from .ops import L2derivs_RadProj, RadProj, VolProj, derivs_RadProj
 from .regularisation import Iso, Joubert, Mass, Radius, Volume, null
 from .spaces import (AtomElement, AtomSpace, ProjElement, ProjSpace, VolElement, VolSpace, axis_tilt_geometry,
                      boundary_nodes, cell_centres, perp, theta_to_vec)
-from .problem import UniformPartition, getFid, getGeom, getReg, getSolver, standardGaussTomo
+from .problem import (UniformPartition, getFid, getGeom, getReg, getSolver, standardGaussTomo,
+                      standardSingleParticleGaussTomo)
 
 __all__ = [
-    "GTError", "GaussTomo", "RadProj", "derivs_RadProj", "L2derivs_RadProj", "VolProj", "AtomSpace",
+    "GTError", "GaussTomo", "SingleParticleGaussTomo", "standardSingleParticleGaussTomo", "RadProj", "derivs_RadProj", "L2derivs_RadProj", "VolProj", "AtomSpace",
     "AtomElement", "ProjSpace", "ProjElement", "VolSpace", "VolElement", "theta_to_vec", "perp",
     "axis_tilt_geometry", "cell_centres", "boundary_nodes", "ops", "spaces", "l2_squared_loss", "null", "Joubert",
     "Mass", "Volume", "Radius", "Iso", "UniformPartition", "standardGaussTomo", "getGeom", "getReg", "getSolver",

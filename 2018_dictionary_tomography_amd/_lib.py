@@ -80,6 +80,12 @@ SIGNATURES = {
     "gt_ft_planes_eval_f64": (_ci, [_P(gt_atoms64), _P(gt_kplanes64), _vp, _vp, _sz, _vp]),
     "gt_ft_derivs_workspace": (_sz, [_P(gt_atoms), _P(gt_kplanes)]),
     "gt_ft_derivs": (_ci, [_P(gt_atoms), _P(gt_kplanes), _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "gt_psf_forward_workspace": (_sz, [_P(gt_atoms), _P(gt_geom)]),
+    "gt_psf_forward": (_ci, [_P(gt_atoms), _P(gt_geom), _vp, _vp, _sz, _vp]),
+    "gt_psf_derivs_workspace": (_sz, [_P(gt_atoms), _P(gt_geom)]),
+    "gt_psf_derivs": (_ci, [_P(gt_atoms), _P(gt_geom), _vp, _ci, _ci, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "gt_psf_volume_workspace": (_sz, [_P(gt_atoms), _P(gt_vol)]),
+    "gt_psf_volume": (_ci, [_P(gt_atoms), _P(gt_vol), _vp, _vp, _sz, _vp]),
     "gt_reg_eval": (_ci, [_P(gt_atoms), _ci, _P(ctypes.c_double), _vp, _vp, _vp, _vp]),
     "gt_fit_run": (_ci, [_vp, _ci, _ci, _ci, _vp]),
     "gt_fit_scratch_bytes": (_sz, [_ci]),

@@ -1,17 +1,19 @@
-"""GaussTomo: the reference's projector operator over the HIP library.
+"""GaussTomo and SingleParticleGaussTomo: the reference's projector
+operators over the HIP library.
 
 Same methods, argument meaning and return shapes as
-GaussDictCode/atomFuncs.py:77-158; the four module functions of
+GaussDictCode/atomFuncs.py:77-172; the four module functions of
 GaussTomo._getOps (atomFuncs.py:106-113) are replaced by batched device
 calls, so derivs / L2_derivs cost one launch sequence for all atoms instead of
-one per atom.
+one per atom.  SingleParticleGaussTomo (:161-172) binds the pixel-integrated
+(point-spread) projector instead.
 """
 from __future__ import annotations
 
 import numpy as np
 import torch
 
-from . import ops
+from . import ops, psf
 from .spaces import ProjElement, VolElement
 
 
@@ -63,3 +65,30 @@ class GaussTomo:
     # -- device-resident variants (no host copies) ------------------------
     def derivs_device(self, atoms, C, order=2, mode="linear"):
         return ops.derivs(atoms, self.range, C, mode, order)
+
+
+class SingleParticleGaussTomo(GaussTomo):
+    """GaussTomo with the point-spread projector (atomFuncs.py:161-172):
+    detector values integrated over pixels, the volume over voxels."""
+
+    def __call__(self, atoms, out=None):
+        data = out.data if out is not None else None
+        R = psf.forward(atoms, self.range, out=data)
+        return out if out is not None else ProjElement(self.range, R)
+
+    def discretise(self, atoms, out=None):
+        data = out.data if out is not None else None
+        u = psf.volume(atoms, self.embedding, out=data)
+        return out if out is not None else VolElement(self.embedding, u)
+
+    def _derivs(self, atoms, C, order, mode):
+        C = C.data if hasattr(C, "data") and not isinstance(C, torch.Tensor) else C
+        f, g, H = psf.derivs(atoms, self.range, C, mode, order)
+        f = f.cpu().numpy().reshape(-1, 1)
+        g = g.cpu().numpy()
+        if order == 1:
+            return f, g
+        return f, g, H.cpu().numpy()
+
+    def derivs_device(self, atoms, C, order=2, mode="linear"):
+        return psf.derivs(atoms, self.range, C, mode, order)

@@ -293,9 +293,14 @@ __host__ __device__ inline float pair_q(const float *q, float dp0, float dp1)
 //   only needs to know that v = sum res dR/dgamma uses w1 with "res" the
 //   residual (L2, residual modes) or C itself (linear mode).
 // Accumulates into grad[P] and hess[P][P] (full, symmetric).
+// Qgn (optional, replaces S2): the products sum_k M_k[m] M_k[m'] over the
+// low monomials m, m' of degree <= 2 (NL = 6 for 2D detectors, 3 for 1D),
+// for values that are pixel integrals M_k[m] = int_k mono E rather than point
+// samples (the Gauss-Newton term is then not a single moment).
 template <int D>
 __host__ __device__ inline void pair_finish(const PairGeom<D> &g, const double *pstar, const double *S1,
-                                            const double *S2, int gn, int want_hess, double *grad, double *hess)
+                                            const double *S2, int gn, int want_hess, double *grad, double *hess,
+                                            const double *Qgn = nullptr)
 {
     using Dm = Dims<D>;
     constexpr int K = Dm::K, NB = Dm::NB, NG = Dm::NG, P = Dm::P, NR = Dm::NR;
@@ -317,19 +322,23 @@ __host__ __device__ inline void pair_finish(const PairGeom<D> &g, const double *
     }
     // v = sum res dR/dgamma ; Kg = sum dR dR^T (gn) + sum res d2R
     double v[NG], Kg[NG][NG];
+    constexpr int NL = (K == 2) ? 6 : 3;
     v[0] = S1[0];
     for (int p = 0; p < NB; p++) v[1 + p] = -30.0 * al * fac[p] * S1[mono_index<K>(pe0[p], pe1[p])];
-    Kg[0][0] = gn ? S2[0] : 0.0;
+    Kg[0][0] = gn ? (Qgn ? Qgn[0] : S2[0]) : 0.0;
     for (int p = 0; p < NB; p++) {
         int m = mono_index<K>(pe0[p], pe1[p]);
-        double val = -30.0 * fac[p] * (S1[m] + (gn ? al * S2[m] : 0.0));
+        const double gnv = gn ? (Qgn ? Qgn[m] : S2[m]) : 0.0;
+        double val = -30.0 * fac[p] * (S1[m] + al * gnv);
         Kg[0][1 + p] = val;
         Kg[1 + p][0] = val;
     }
     for (int p = 0; p < NB; p++)
         for (int q = p; q < NB; q++) {
             int m = mono_index<K>(pe0[p] + pe0[q], pe1[p] + pe1[q]);
-            double val = 900.0 * fac[p] * fac[q] * (al * S1[m] + (gn ? al * al * S2[m] : 0.0));
+            const double gnv =
+                gn ? (Qgn ? Qgn[mono_index<K>(pe0[p], pe1[p]) * NL + mono_index<K>(pe0[q], pe1[q])] : S2[m]) : 0.0;
+            double val = 900.0 * fac[p] * fac[q] * (al * S1[m] + al * al * gnv);
             Kg[1 + p][1 + q] = val;
             Kg[1 + q][1 + p] = val;
         }

@@ -11,6 +11,13 @@ newton_linesearch_block  NewtonGaussian.linesearch_block (NewtonGaussian.py:
 gd_linesearch_block      GD_lib.linesearch_block (GD_lib.py:82-178): block
                          gradient descent driven from the host over the batched
                          forward and derivative kernels.
+newton_host              the Newton fitters driven from the host over an
+                         operator's forward and derivative kernels: what the
+                         drop-ins run for SingleParticleGaussTomo, whose
+                         pixel-integrated projector the device fitter does not
+                         implement (NewtonGaussian.linesearch only calls
+                         Radon(.) and Radon.L2_derivs, so it fits with any
+                         operator).
 
 The fidelity is l2 (transport_loss.py:567-600); the Newton fitters take any
 regulariser of the regularisation module, evaluated inside the fit kernel.
@@ -22,7 +29,7 @@ import ctypes
 import numpy as np
 import torch
 
-from . import ops
+from . import ops, psf
 from ._lib import check, load, stream_handle
 from .regularisation import as_regulariser
 
@@ -218,11 +225,170 @@ def _check_fidelity(fidelity):
         raise NotImplementedError("the fitters take the l2 fidelity (l2_squared_loss) only")
 
 
+class Kernels:
+    """The three device calls the host-driven fitters make: the sinogram of a
+    set of atoms, per-atom L2 derivatives against a sinogram, and per-atom
+    gradients of the full loss given its residual."""
+
+    def __init__(self, forward, derivs, resid_grad):
+        self.forward, self.derivs, self.resid_grad = forward, derivs, resid_grad
+
+
+POINT = Kernels(ops.forward, lambda a, s, C, order: ops.derivs(a, s, C, "L2", order),
+                lambda a, s, res: ops.derivs(a, s, res, "resid", order=1, want_f=False)[1])
+PIXEL = Kernels(psf.forward, lambda a, s, C, order: psf.derivs(a, s, C, "L2", order),
+                lambda a, s, res: psf.derivs(a, s, res, "linear", 1)[1])
+
+
+def kernels_of(Radon):
+    """POINT for GaussTomo (and a bare projection space), PIXEL for
+    SingleParticleGaussTomo."""
+    from .atomFuncs import SingleParticleGaussTomo
+    return PIXEL if isinstance(Radon, SingleParticleGaussTomo) else POINT
+
+
+def _sumsq(R, data):
+    d = (R - data).double()
+    return 0.5 * float((d * d).sum())
+
+
+def _project_np(D, iso, th, d):
+    """th + d with the projections of GD_lib._step (GD_lib.py:502-522)."""
+    t = th + d
+    t[0] = max(0.0, t[0])
+    r = t[1 + D:]
+    if iso:
+        r[0] = abs(r[0])
+        return t
+    sign_sets = ((0, 2), (1,)) if D == 2 else ((0, 3, 5), (1, 4), (2,))
+    for s in sign_sets:
+        if r[s[0]] < 0:
+            r[list(s)] *= -1
+    return t
+
+
+def _project_block_np(D, iso, blk, th, d):
+    """Block blk of th = [I, x, r] moved by d with the projections of
+    NewtonGaussian.linesearch_block (NewtonGaussian.py:159-187)."""
+    t = th.copy()
+    o = (0, 1, 1 + D)[blk]
+    t[o:o + d.size] += d
+    if blk == 0:
+        t[0] = max(0.0, t[0])
+    elif blk == 1:
+        t[1:1 + D] = np.clip(t[1:1 + D], -0.99, 0.99)
+    else:
+        t = _project_np(D, iso, t, np.zeros_like(t))
+        t[0] = th[0]
+    return t
+
+
+def newton_host(recon, space, data, max_iter, kernels=POINT, reg=None, tol=1e-4, min_iter=1, seed=1,
+                block=False):
+    """NewtonGaussian.linesearch (block=False, NewtonGaussian.py:18-109, with
+    GD_lib._step) or linesearch_block (block=True, :112-237) driven from the
+    host: per update one forward of the atom, one L2 derivative call against
+    data + R_j - R and one forward of the proposal, all on the device; the
+    damped system is solved in FP64 on the host.  Same control flow, step
+    sizes, projections and energy bookkeeping as the device fitter (fidelity
+    in FP64 from the float32 sinograms; E = F + reg; a proposal with a
+    non-finite fidelity is rejected).  Fits `recon` in place; returns
+    (recon, E, F)."""
+    reg = as_regulariser(reg)
+    dev = recon.I.device
+    n = recon.I.numel()
+    D = recon.x.shape[1]
+    iso = recon.r.reshape(max(n, 1), -1).shape[1] == 1
+    if not hasattr(max_iter, "__iter__"):
+        max_iter = [max_iter, 1]
+    max_iter = list(max_iter) + [1] * (2 - len(max_iter))
+    eps = 1e-4
+    nblk = 3 if block else 1
+    h = np.ones((nblk, n))
+    data = data.data if hasattr(data, "data") and not isinstance(data, torch.Tensor) else data
+    data = torch.as_tensor(data, device=dev).to(torch.float32)
+    R = kernels.forward(recon, space)
+    F = [_sumsq(R, data)]
+    E = [F[0] + (reg(recon) if reg.kind else 0.0)]
+    rs = np.random.RandomState(seed)
+
+    def theta(j):
+        return np.concatenate((recon.I[j:j + 1].double().cpu().numpy(), recon.x[j].double().cpu().numpy(),
+                               recon.r[j].reshape(-1).double().cpu().numpy()))
+
+    def put(j, th):
+        t = torch.as_tensor(th, dtype=torch.float32, device=dev)
+        recon.I[j] = t[0]
+        recon.x[j] = t[1:1 + D]
+        recon.r[j] = t[1 + D:].reshape(recon.r[j].shape)
+
+    for it in range(max_iter[0]):
+        brk = [True, True]
+        for j in rs.permutation(np.arange(n)):
+            one = ops._AtomView(recon.I[j:j + 1], recon.x[j:j + 1], recon.r[j:j + 1])
+            for blk in range(nblk):
+                for _ in range(max_iter[1]):
+                    brk[1] = True
+                    Rj = kernels.forward(one, space)
+                    _, g, H = kernels.derivs(one, space, data + Rj - R, 2)
+                    df, ddf = g[0].cpu().numpy(), H[0].cpu().numpy()
+                    if reg.kind:
+                        _, gr, Hr = reg.terms(one, 2)
+                        df = df + gr[0].cpu().numpy()
+                        ddf = ddf + Hr[0].cpu().numpy()
+                    ddf[np.diag_indices_from(ddf)] += 1.0 / h[blk, j]
+                    old = theta(j)
+                    if not block:
+                        dx = np.linalg.solve(ddf, -df)
+                        new = _project_np(D, iso, old, dx)
+                    else:
+                        o, m = (0, 1) if blk == 0 else ((1, D) if blk == 1 else (1 + D, old.size - 1 - D))
+                        if blk == 0:
+                            if not ddf[0, 0] > 1e-8:
+                                break
+                            d = np.array([-df[0] / ddf[0, 0]])
+                        else:
+                            try:
+                                d = np.linalg.solve(ddf[o:o + m, o:o + m], -df[o:o + m])
+                            except np.linalg.LinAlgError:
+                                break
+                        new = _project_block_np(D, iso, blk, old, d)
+                    put(j, new)
+                    Rnew = R + kernels.forward(one, space) - Rj
+                    Fn = _sumsq(Rnew, data)
+                    F.append(Fn)
+                    En = Fn + (reg(recon) if reg.kind else 0.0)
+                    if En > E[-1] or not np.isfinite(Fn):
+                        put(j, old)
+                        E.append(E[-1])
+                        if h[blk, j] > 2 * eps:
+                            brk = [False, False]
+                        h[blk, j] /= 10
+                    else:
+                        R = Rnew
+                        E.append(En)
+                        if block:
+                            moved = np.linalg.norm((new - old)[o:o + m]) > tol * np.linalg.norm(old[o:o + m])
+                        else:
+                            moved = np.sqrt(np.sum(dx * dx)) > tol
+                        if moved:
+                            brk = [False, False]
+                        h[blk, j] *= 1.3 if block else 2
+                    h[blk, j] = min(max(h[blk, j], eps), 1 / eps)
+                    if brk[1]:
+                        break
+        if brk[0] and it > min_iter:
+            break
+    return recon, np.array(E), np.array(F)
+
+
 def _newton_dropin(recon, data, max_iter, fidelity, reg, Radon, guess, tol, min_iter, block, kwargs):
     if guess is not None:
         raise NotImplementedError("the initial-guess path (KL_GaussRadon) is not part of this build")
     _check_fidelity(fidelity)
     space = Radon.range if Radon is not None else kwargs["space"]
+    if kernels_of(Radon) is not POINT:
+        return newton_host(recon, space, data, max_iter, kernels_of(Radon), reg, tol, min_iter, block=block)
     req, s = newton_fit(recon, space, data, max_iter, tol, min_iter, reg=reg, block=block,
                         group=kwargs.get("group"))
     if s.done == 2:
@@ -259,7 +425,7 @@ def newton_linesearch_block(recon, data, max_iter, fidelity=None, reg=None, Rado
 # Block gradient descent, GD_lib.linesearch_block (GD_lib.py:82-178)
 # ---------------------------------------------------------------------------
 def gd_linesearch_block(recon, data, max_iter, space, dim="xrI", tol=1e-4, min_iter=1, seed=1, reg=None,
-                        return_F=False):
+                        return_F=False, kernels=POINT):
     """Block GD over (x, r, I) of each atom with the reference's step-size rules.
     Energies (fidelity + regulariser) in FP64.  Divergence from the reference,
     kept on purpose: on a rejected step the sinogram of the reverted atoms is
@@ -277,7 +443,7 @@ def gd_linesearch_block(recon, data, max_iter, space, dim="xrI", tol=1e-4, min_i
     h = {c: np.full(n, 1e-2) for c in dims}
     data = torch.as_tensor(data.data if hasattr(data, "data") and not isinstance(data, torch.Tensor) else data,
                            device=dev).to(torch.float32)
-    R = ops.forward(recon, space)
+    R = kernels.forward(recon, space)
     F = [0.5 * float(((R - data).double() ** 2).sum())]
     E = [F[0] + (reg(recon) if reg.kind else 0.0)]
     rs = np.random.RandomState(seed)
@@ -289,8 +455,7 @@ def gd_linesearch_block(recon, data, max_iter, space, dim="xrI", tol=1e-4, min_i
                 brk[1] = True
                 for name in dims:
                     one = ops._AtomView(recon.I[j:j + 1], recon.x[j:j + 1], recon.r[j:j + 1])
-                    _, g, _ = ops.derivs(one, space, resid, "resid", order=1, want_f=False)
-                    g = g[0].cpu().numpy()
+                    g = kernels.resid_grad(one, space, resid)[0].cpu().numpy()
                     if name == "I":
                         grad = g[0:1]
                     elif name == "x":
@@ -305,7 +470,7 @@ def gd_linesearch_block(recon, data, max_iter, space, dim="xrI", tol=1e-4, min_i
                     step = torch.as_tensor(h[name][j] * grad, dtype=torch.float32, device=dev)
                     new = old - (step[0] if name == "I" else step)
                     arr[j] = new
-                    Rn = ops.forward(recon, space)
+                    Rn = kernels.forward(recon, space)
                     rn = Rn - data
                     Fn = 0.5 * float((rn.double() ** 2).sum())
                     F.append(Fn)
@@ -339,4 +504,5 @@ def linesearch_block(recon, data, max_iter, fidelity=None, reg=None, Radon=None,
         raise NotImplementedError("the initial-guess path (KL_GaussRadon) is not part of this build")
     _check_fidelity(fidelity)
     space = Radon.range if Radon is not None else kwargs["space"]
-    return gd_linesearch_block(recon, data, max_iter, space, dim, tol, min_iter, reg=reg, return_F=True)
+    return gd_linesearch_block(recon, data, max_iter, space, dim, tol, min_iter, reg=reg, return_F=True,
+                               kernels=kernels_of(Radon))

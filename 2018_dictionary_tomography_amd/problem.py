@@ -26,7 +26,7 @@ import numpy as np
 import torch
 
 from . import fit
-from .atomFuncs import GaussTomo
+from .atomFuncs import GaussTomo, SingleParticleGaussTomo
 from .losses import l2_squared_loss
 from .regularisation import Iso, Joubert, Mass, Radius, Volume, null
 from .spaces import AtomElement, AtomSpace, ProjElement, ProjSpace, VolSpace
@@ -178,7 +178,7 @@ def getFid(dim, fidelity, device=None):
 
 def standardGaussTomo(gt=None, data=None, noise=None, dim=3, device="GPU", isotropic=False, vol_box=None,
                       vol_size=None, angles=None, angle_range=None, angle_num=None, axis=(0, 0, 1), det_box=None,
-                      det_size=None, fidelity="l2_squared", reg=None, solver="Newton"):
+                      det_size=None, fidelity="l2_squared", reg=None, solver="Newton", _operator=GaussTomo):
     """GaussDictCode/__init__.py:19-69 for the Gaussian solvers.  Returns
     (Radon, fidelity, data, ASpace, PSpace, (reg, GD)).
 
@@ -192,7 +192,7 @@ def standardGaussTomo(gt=None, data=None, noise=None, dim=3, device="GPU", isotr
                                     data)
     ASpace = AtomSpace(dim, isotropic=isotropic)
     PSpace = ProjSpace(angles.coord_vectors, detector.coord_vectors)
-    Radon = GaussTomo(ASpace, vol, PSpace, device=device)
+    Radon = _operator(ASpace, vol, PSpace, device=device)
 
     if data is None:
         if gt is None:
@@ -215,3 +215,15 @@ def standardGaussTomo(gt=None, data=None, noise=None, dim=3, device="GPU", isotr
     fidelity = getFid(dim, fidelity, device)
     reg, GD = getReg(dim, reg), getSolver(solver)
     return Radon, fidelity, data, ASpace, PSpace, (reg, GD)
+
+
+def standardSingleParticleGaussTomo(gt=None, data=None, noise=None, dim=3, device="GPU", isotropic=False,
+                                    vol_box=None, vol_size=None, angles=None, angle_range=None, angle_num=None,
+                                    axis=(0, 0, 1), det_box=None, det_size=None, fidelity="l2_squared", reg=None,
+                                    solver="Newton"):
+    """GaussDictCode/__init__.py:72-122: standardGaussTomo with the
+    point-spread projector (SingleParticleGaussTomo).  Data simulated from
+    atoms as ground truth use that projector."""
+    return standardGaussTomo(gt, data, noise, dim, device, isotropic, vol_box, vol_size, angles, angle_range,
+                             angle_num, axis, det_box, det_size, fidelity, reg, solver,
+                             _operator=SingleParticleGaussTomo)

@@ -143,6 +143,28 @@ int gt_ft_eval(const gt_atoms *atoms, const gt_kgrid *grid, float *out, void *ws
                void *stream);
 
 /*
+ * Point-spread ("single particle") projector, the operator
+ * SingleParticleGaussTomo binds (GaussDictCode/atomFuncs.py:161-172):
+ *   gt_psf_forward  <- RadProj          GaussDictCode/bin/point_spread_functions.py:371-385
+ *   gt_psf_derivs   <- L2derivs_RadProj :1077-1115 (mode GT_MODE_L2)
+ *                   <- derivs_RadProj   :1314-1350 (mode GT_MODE_LINEAR)
+ *   gt_psf_volume   <- VolProj          :168-197
+ * Each detector value is the integral of the atom's line integral over its
+ * pixel (the square of side |p[1] - p[0]| around the detector coordinate,
+ * per axis); each volume value the integral of the atom's profile over its
+ * voxel.  Detector and volume coordinates must ascend with at least two
+ * entries per axis.  Outputs and workspaces as gt_forward / gt_derivs /
+ * gt_volume (derivatives batched over atoms, FP64, iso collapsed).
+ */
+size_t gt_psf_forward_workspace(const gt_atoms *atoms, const gt_geom *geom);
+int gt_psf_forward(const gt_atoms *atoms, const gt_geom *geom, float *R, void *ws, size_t ws_bytes, void *stream);
+size_t gt_psf_derivs_workspace(const gt_atoms *atoms, const gt_geom *geom);
+int gt_psf_derivs(const gt_atoms *atoms, const gt_geom *geom, const float *C, int mode, int order, double *f,
+                  double *g, double *H, void *ws, size_t ws_bytes, void *stream);
+size_t gt_psf_volume_workspace(const gt_atoms *atoms, const gt_vol *vol);
+int gt_psf_volume(const gt_atoms *atoms, const gt_vol *vol, float *u, void *ws, size_t ws_bytes, void *stream);
+
+/*
  * Fourier evaluation on sample planes (FTGaussian3D.evaluate hyperplane and
  * 2.5D branches, GaussDictCode/bin/FTGaussian3D.py:12-38,113-148,167-196;
  * FTGaussian2D.evaluate, GaussDictCode/bin/FTGaussian2D.py:13-35,53-100):

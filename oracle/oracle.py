@@ -387,7 +387,20 @@ def newton_linesearch(I, x, r, geom, data, max_iter, tol=1e-4, min_iter=1, seed=
     return I, x, r, np.array(E)
 
 
-def gd_linesearch_block(I, x, r, geom, data, max_iter, dim="xrI", tol=1e-4, min_iter=1, seed=1):
+def _projector(kind, n_sub):
+    """(radproj, derivs) of the point projector (GaussTomo) or of the
+    pixel-integrated one (SingleParticleGaussTomo, n_sub nodes per pixel
+    and axis)."""
+    if kind == "point":
+        return radproj, derivs
+    if kind == "psf":
+        return (lambda I, x, r, g: psf_radproj(I, x, r, g, n_sub),
+                lambda I, x, r, g, C, mode, order: psf_derivs(I, x, r, g, C, mode, order, n_sub))
+    raise ValueError(kind)
+
+
+def gd_linesearch_block(I, x, r, geom, data, max_iter, dim="xrI", tol=1e-4, min_iter=1, seed=1, projector="point",
+                        n_sub=12):
     """Block gradient descent (GD_lib.py:82-178), null regulariser, FP64.
 
     Divergence from the reference, kept deliberately: on reject the sinogram is
@@ -403,6 +416,7 @@ def gd_linesearch_block(I, x, r, geom, data, max_iter, dim="xrI", tol=1e-4, min_
     eps = 1e-8
     h = [[1e-2] * n for _ in dim]
     data = np.asarray(data, float)
+    radproj, derivs = _projector(projector, n_sub)
     R = radproj(I, x, r, geom)
     E = [l2_squared_loss(R, data)]
     rs = np.random.RandomState(seed)
@@ -558,7 +572,8 @@ def _project_block(D, iso, t, th_old, d):
     return th
 
 
-def newton_fit(I, x, r, geom, data, max_iter, tol=1e-4, min_iter=1, seed=1, reg=None, block=False):
+def newton_fit(I, x, r, geom, data, max_iter, tol=1e-4, min_iter=1, seed=1, reg=None, block=False,
+               projector="point", n_sub=12):
     """NewtonGaussian.linesearch (block=False, :18-109, with GD_lib._step) or
     NewtonGaussian.linesearch_block (block=True, :112-237) with regulariser
     `reg` (a Reg, None = null), FP64.  Energies E = F + reg(all atoms),
@@ -569,7 +584,10 @@ def newton_fit(I, x, r, geom, data, max_iter, tol=1e-4, min_iter=1, seed=1, reg=
     so that its x and r solves raise inside the try and are skipped).
 
     Returns (I, x, r, E, F): E after each update, F the fidelity of each
-    update's proposal (also when rejected, as the reference records it)."""
+    update's proposal (also when rejected, as the reference records it).
+    projector: 'point' (GaussTomo) or 'psf' (SingleParticleGaussTomo; the
+    reference's Newton only calls Radon(.) and Radon.L2_derivs)."""
+    radproj, derivs = _projector(projector, n_sub)
     D = geom.dim
     I, x = np.array(I, float).reshape(-1), np.array(x, float).reshape(-1, D)
     r = np.array(r, float).reshape(I.size, -1)
@@ -735,3 +753,107 @@ def ft_derivs(I, x, r, k0, k1, w0, w1, C, iso=False):
         Df[j] = np.real(res @ np.conj(dg))
         DDf[j] = np.real(np.einsum("pi,pl->il", dg, np.conj(dg)) + np.einsum("p,pil->il", res, np.conj(ddg)))
     return f, Df, DDf
+
+
+# ---------------------------------------------------------------------------
+# Point-spread projector, GaussDictCode/bin/point_spread_functions.py:168-1350
+# ---------------------------------------------------------------------------
+def _cell_nodes(p, n_sub):
+    """Gauss-Legendre nodes n_sub per cell of a uniform coordinate vector (cell
+    = the interval of width |p[1] - p[0]| around each coordinate) and the
+    weights of one cell."""
+    p = np.asarray(p, float)
+    hs = abs(p[1] - p[0]) / 2
+    xg, wg = np.polynomial.legendre.leggauss(n_sub)
+    return (p[:, None] + hs * xg[None, :]).reshape(-1), hs * wg
+
+
+def _fine_geometry(geom, n_sub):
+    D, T, lenT, w0, w1, p0, p1 = _geom_args(geom)
+    f0, c0 = _cell_nodes(p0, n_sub)
+    det = [f0]
+    cw = [c0]
+    if D == 3:
+        f1, c1 = _cell_nodes(p1, n_sub)
+        det.append(f1)
+        cw.append(c1)
+    return Geometry(geom.orientations, geom.ortho, det), cw
+
+
+def psf_radproj(I, x, r, geom, n_sub=24):
+    """Pixel-integrated projection P[a][k] = int_pixel R_a (RadProj,
+    point_spread_functions.py:371-385, as the pixel integral it approximates):
+    the pinned point projector radproj on n_sub Gauss-Legendre nodes per pixel
+    and axis, FP64."""
+    fine, cw = _fine_geometry(geom, n_sub)
+    Rf = radproj(I, x, r, fine)
+    nA = Rf.shape[0]
+    if len(cw) == 1:
+        return Rf.reshape(nA, -1, n_sub) @ cw[0]
+    n0, n1 = Rf.shape[1] // n_sub, Rf.shape[2] // n_sub
+    return np.einsum("aibjc,b,c->aij", Rf.reshape(nA, n0, n_sub, n1, n_sub), cw[0], cw[1])
+
+
+def _psf_weights(C, cw, n_sub):
+    """A pixel-valued array spread to the fine nodes with the node weights."""
+    C = np.asarray(C, float)
+    if len(cw) == 1:
+        return (C[:, :, None] * cw[0][None, None, :]).reshape(C.shape[0], -1)
+    W = C[:, :, None, :, None] * cw[0][None, None, :, None, None] * cw[1][None, None, None, None, :]
+    return W.reshape(C.shape[0], C.shape[1] * n_sub, C.shape[2] * n_sub)
+
+
+def psf_derivs(I, x, r, geom, C, mode="L2", order=2, n_sub=24, h=1e-6):
+    """Per atom: mode 'linear' f = sum_k P_k C_k (derivs_RadProj,
+    point_spread_functions.py:1314-1350), mode 'L2' f = 1/2 sum (P_k - C_k)^2
+    (L2derivs_RadProj, :1077-1115), with gradient and Hessian.  Linear
+    functionals of P are linear functionals of the line integral on the fine
+    nodes (the pinned derivs, linear mode); the Gauss-Newton term of the L2
+    Hessian takes the pixel gradients dP_k from central differences of
+    psf_radproj (h relative)."""
+    fine, cw = _fine_geometry(geom, n_sub)
+    I, x = np.asarray(I, float).reshape(-1), np.asarray(x, float).reshape(I.size if np.ndim(I) else 1, -1)
+    D = x.shape[1]
+    r = np.asarray(r, float).reshape(I.size, -1)
+    C = np.asarray(C, float)
+    if mode == "linear":
+        return derivs(I, x, r, fine, _psf_weights(C, cw, n_sub), "linear", order)
+    P = 1 + D + r.shape[1]
+    f = np.zeros(I.size)
+    Df = np.zeros((I.size, P))
+    DDf = np.zeros((I.size, P, P))
+    for j in range(I.size):
+        sl = slice(j, j + 1)
+        Pj = psf_radproj(I[sl], x[sl], r[sl], geom, n_sub)
+        res = Pj - C
+        f[j] = 0.5 * np.sum(res * res)
+        _, g, H = derivs(I[sl], x[sl], r[sl], fine, _psf_weights(res, cw, n_sub), "linear", order)
+        Df[j] = g[0]
+        if order > 1:
+            th = np.concatenate(([I[j]], x[j], r[j]))
+            J = []
+            for i in range(P):
+                e = np.zeros(P)
+                e[i] = h * max(1.0, abs(th[i]))
+                tp, tm = th + e, th - e
+                Pp = psf_radproj(tp[:1], tp[1:1 + D][None], tp[1 + D:][None], geom, n_sub)
+                Pm = psf_radproj(tm[:1], tm[1:1 + D][None], tm[1 + D:][None], geom, n_sub)
+                J.append(((Pp - Pm) / (2 * e[i])).reshape(-1))
+            J = np.array(J)
+            DDf[j] = H[0] + J @ J.T
+    return f, Df, DDf
+
+
+def psf_volproj(I, x, r, coords, n_sub=12):
+    """Voxel-integrated render u[v] = int_voxel f (VolProj,
+    point_spread_functions.py:168-197): the pinned volproj on n_sub
+    Gauss-Legendre nodes per voxel and axis, FP64."""
+    nodes, cw = zip(*[_cell_nodes(c, n_sub) for c in coords])
+    uf = volproj(I, x, r, list(nodes))
+    shp = []
+    for c in coords:
+        shp += [len(c), n_sub]
+    uf = uf.reshape(shp)
+    if len(coords) == 2:
+        return np.einsum("iajb,a,b->ij", uf, cw[0], cw[1])
+    return np.einsum("iajbkc,a,b,c->ijk", uf, cw[0], cw[1], cw[2])
