@@ -108,10 +108,13 @@ class FitRequest:
     """Device state of one Newton fit: atoms, residual, step sizes, history.
 
     reg: a regularisation.Regulariser (None: null); block: update I, x and r
-    of each atom in turn (NewtonGaussian.linesearch_block)."""
+    of each atom in turn (NewtonGaussian.linesearch_block); perm: the atom
+    order of each outer iteration (default: permutations(N, max_outer,
+    seed)); h: starting step sizes ([3][N] for the block variant, else [N];
+    default 1)."""
 
     def __init__(self, atoms, space, data, max_iter, tol=1e-4, min_iter=1, seed=1, record=True, device=None,
-                 reg=None, block=False):
+                 reg=None, block=False, perm=None, h=None):
         dev = device or atoms.I.device
         self.reg = as_regulariser(reg)
         self.block = bool(block)
@@ -145,8 +148,19 @@ class FitRequest:
             self.ninf0 = int((~fin).sum())
             self.Rsum0 = float(val[fin].sum())
         self.E0 = self.F0 + (float("inf") if self.ninf0 else self.Rsum0)
-        self.perm = torch.as_tensor(permutations(self.N, self.max_outer, seed), device=dev).contiguous()
-        self.h = torch.ones(nblk * self.N, dtype=torch.float64, device=dev)
+        if perm is None:
+            perm = permutations(self.N, self.max_outer, seed)
+        perm = np.asarray(perm, dtype=np.int32).reshape(-1, self.N)
+        if perm.shape[0] < self.max_outer or not all(np.array_equal(np.sort(p), np.arange(self.N)) for p in perm):
+            raise ValueError("perm must hold max_outer permutations of range(N)")
+        self.perm = torch.as_tensor(perm[: self.max_outer].copy(), device=dev).contiguous()
+        if h is None:
+            self.h = torch.ones(nblk * self.N, dtype=torch.float64, device=dev)
+        else:
+            h = np.asarray(h, dtype=np.float64).reshape(-1)
+            if h.size != nblk * self.N:
+                raise ValueError(f"h must hold {nblk * self.N} step sizes")
+            self.h = torch.as_tensor(h.copy(), device=dev)
         nh = nblk * self.max_outer * self.max_inner * self.N + 1
         self.Ehist = torch.zeros(nh, dtype=torch.float64, device=dev) if record else None
         self.Fhist = torch.zeros(nh, dtype=torch.float64, device=dev) if record else None
@@ -194,12 +208,13 @@ def run_blocks(dev_table, nslots, dev_blocks, nblocks, dim, budget):
 
 
 def newton_fit(atoms, space, data, max_iter, tol=1e-4, min_iter=1, seed=1, budget=None, record=True, group=None,
-               reg=None, block=False):
+               reg=None, block=False, perm=None, h=None):
     """Run one Newton fit to completion on the device; returns (request, final
     slot).  group: workgroups advancing the fit together (None: default_group
     of the angle count; 1: a single workgroup).  The result does not depend on
     the group size."""
-    req = FitRequest(atoms, space, data, max_iter, tol, min_iter, seed, record, reg=reg, block=block)
+    req = FitRequest(atoms, space, data, max_iter, tol, min_iter, seed, record, reg=reg, block=block, perm=perm,
+                     h=h)
     dev = req.I.device
     table = torch.as_tensor(np.frombuffer(req.slot_bytes(), dtype=np.uint8).copy(), device=dev)
     budget = budget or max(req.N * req.max_inner * (3 if req.block else 1), 1)
@@ -284,7 +299,7 @@ def _project_block_np(D, iso, blk, th, d):
 
 
 def newton_host(recon, space, data, max_iter, kernels=POINT, reg=None, tol=1e-4, min_iter=1, seed=1,
-                block=False):
+                block=False, rs=None, h=None):
     """NewtonGaussian.linesearch (block=False, NewtonGaussian.py:18-109, with
     GD_lib._step) or linesearch_block (block=True, :112-237) driven from the
     host: per update one forward of the atom, one L2 derivative call against
@@ -293,7 +308,8 @@ def newton_host(recon, space, data, max_iter, kernels=POINT, reg=None, tol=1e-4,
     sizes, projections and energy bookkeeping as the device fitter (fidelity
     in FP64 from the float32 sinograms; E = F + reg; a proposal with a
     non-finite fidelity is rejected).  Fits `recon` in place; returns
-    (recon, E, F)."""
+    (recon, E, F).  rs (a numpy RandomState) and h ([3 or 1][N] step
+    sizes, updated in place) let a caller carry both across calls."""
     reg = as_regulariser(reg)
     dev = recon.I.device
     n = recon.I.numel()
@@ -304,13 +320,13 @@ def newton_host(recon, space, data, max_iter, kernels=POINT, reg=None, tol=1e-4,
     max_iter = list(max_iter) + [1] * (2 - len(max_iter))
     eps = 1e-4
     nblk = 3 if block else 1
-    h = np.ones((nblk, n))
+    h = np.ones((nblk, n)) if h is None else h
     data = data.data if hasattr(data, "data") and not isinstance(data, torch.Tensor) else data
     data = torch.as_tensor(data, device=dev).to(torch.float32)
     R = kernels.forward(recon, space)
     F = [_sumsq(R, data)]
     E = [F[0] + (reg(recon) if reg.kind else 0.0)]
-    rs = np.random.RandomState(seed)
+    rs = np.random.RandomState(seed) if rs is None else rs
 
     def theta(j):
         return np.concatenate((recon.I[j:j + 1].double().cpu().numpy(), recon.x[j].double().cpu().numpy(),
@@ -382,11 +398,106 @@ def newton_host(recon, space, data, max_iter, kernels=POINT, reg=None, tol=1e-4,
     return recon, np.array(E), np.array(F)
 
 
+def _device_stage(sub, space, data, mi, tol, min_iter, reg, block, rs, h, group):
+    """One stage of the greedy path on the device fitter: the permutations
+    are drawn from `rs` as the reference's loop would draw them, and `rs` is
+    left advanced by the outer iterations the fit actually ran."""
+    n = len(sub)
+    state = rs.get_state()
+    perm = np.stack([rs.permutation(np.arange(n)) for _ in range(mi[0])])
+    req, s = newton_fit(sub, space, data, mi, tol, min_iter, reg=reg, block=block, group=group, perm=perm,
+                        h=h[:, :n])
+    if s.done == 2:
+        raise np.linalg.LinAlgError("singular Newton system")
+    used = mi[0] if s.it >= mi[0] else s.it + 1
+    rs.set_state(state)
+    for _ in range(used):
+        rs.permutation(np.arange(n))
+    h[:, :n] = req.h.reshape(h.shape[0], n).cpu().numpy()
+    sub.I.copy_(req.I)
+    sub.x.copy_(req.x)
+    sub.r.copy_(req.r)
+    return req.Ehist[: s.jj + 1].cpu().numpy(), req.Fhist[: s.jj + 1].cpu().numpy()
+
+
+def greedy(recon, data, max_iter, reg, space, guess, stage):
+    """The initial-guess path of NewtonGaussian.linesearch /
+    linesearch_block and GD_lib.linesearch_block (NewtonGaussian.py:38-50,
+    GD_lib.py:113-126): for n = 1..N, atom n-1 is placed by
+    guess(data - Radon(recon[:n]), recon[n-1]), the last energy entry is
+    replaced by the energy of recon[:n], and stage(recon[:n], max_iter) runs
+    the fitter's sweeps over those atoms (returning its (E, F) histories,
+    whose first entry is the stage's start); with three entries in max_iter
+    a last stage of max_iter[2] sweeps over all atoms follows.  The stage
+    carries the step sizes and the permutation stream from one stage to the
+    next, as the reference's single loop does."""
+    reg = as_regulariser(reg)
+    N = len(recon)
+    dev = recon.I.device
+    from .spaces import ProjElement
+    fwd = stage.kernels.forward
+    data = data.data if hasattr(data, "data") and not isinstance(data, torch.Tensor) else data
+    data = torch.as_tensor(data, device=dev).to(torch.float32)
+    if not hasattr(max_iter, "__iter__"):
+        max_iter = [max_iter, 1]
+    mi = list(max_iter) + [1] * (2 - len(max_iter))
+    E, F = [], []
+    n = 1
+    while n <= N + 1:
+        if n == N + 1:
+            if len(mi) == 2:
+                break
+            n = N
+            mi = [mi[2], mi[1]]
+        else:
+            res = ProjElement(space, data - fwd(recon[:n], space))
+            recon[n - 1] = guess(res, recon[n - 1])
+            sub = recon[:n]
+            Fv = _sumsq(fwd(sub, space), data)
+            Ev = Fv + (reg(sub) if reg.kind else 0.0)
+            if E:
+                E[-1], F[-1] = Ev, Fv
+            else:
+                E, F = [Ev], [Fv]
+        sub = recon[:n]
+        Es, Fs = stage(sub, mi[:2])
+        recon[:n] = sub
+        E += list(Es[1:])
+        F += list(Fs[1:])
+        n += 1
+    return recon, np.array(E), np.array(F)
+
+
+class _Stage:
+    """Callable stage for `greedy`, holding the step sizes and the RNG."""
+
+    def __init__(self, kernels, run):
+        self.kernels, self.run = kernels, run
+
+    def __call__(self, sub, mi):
+        return self.run(sub, mi)
+
+
 def _newton_dropin(recon, data, max_iter, fidelity, reg, Radon, guess, tol, min_iter, block, kwargs):
-    if guess is not None:
-        raise NotImplementedError("the initial-guess path (KL_GaussRadon) is not part of this build")
     _check_fidelity(fidelity)
     space = Radon.range if Radon is not None else kwargs["space"]
+    if guess is not None:
+        kern = kernels_of(Radon)
+        h = np.ones((3 if block else 1, len(recon)))
+        rs = np.random.RandomState(1)
+        dev_data = None
+
+        def run(sub, mi):
+            nonlocal dev_data
+            if kern is POINT:
+                if dev_data is None:
+                    d = data.data if hasattr(data, "data") and not isinstance(data, torch.Tensor) else data
+                    dev_data = torch.as_tensor(d, device=sub.I.device).to(torch.float32)
+                return _device_stage(sub, space, dev_data, mi, tol, min_iter, reg, block, rs, h, kwargs.get("group"))
+            _, Es, Fs = newton_host(sub, space, data, mi, kern, reg, tol, min_iter, block=block, rs=rs, h=h[:, :len(sub)])
+            return Es, Fs
+
+        return greedy(recon, data, max_iter, reg, space, guess, _Stage(kern, run))
     if kernels_of(Radon) is not POINT:
         return newton_host(recon, space, data, max_iter, kernels_of(Radon), reg, tol, min_iter, block=block)
     req, s = newton_fit(recon, space, data, max_iter, tol, min_iter, reg=reg, block=block,
@@ -425,7 +536,7 @@ def newton_linesearch_block(recon, data, max_iter, fidelity=None, reg=None, Rado
 # Block gradient descent, GD_lib.linesearch_block (GD_lib.py:82-178)
 # ---------------------------------------------------------------------------
 def gd_linesearch_block(recon, data, max_iter, space, dim="xrI", tol=1e-4, min_iter=1, seed=1, reg=None,
-                        return_F=False, kernels=POINT):
+                        return_F=False, kernels=POINT, rs=None, h=None):
     """Block GD over (x, r, I) of each atom with the reference's step-size rules.
     Energies (fidelity + regulariser) in FP64.  Divergence from the reference,
     kept on purpose: on a rejected step the sinogram of the reverted atoms is
@@ -440,13 +551,15 @@ def gd_linesearch_block(recon, data, max_iter, space, dim="xrI", tol=1e-4, min_i
     max_iter = list(max_iter) + [1] * (2 - len(max_iter))
     eps = 1e-8
     n = recon.I.numel()
-    h = {c: np.full(n, 1e-2) for c in dims}
+    if h is None:
+        h = np.full((len(dims), n), 1e-2)
+    h = {c: h[t] for t, c in enumerate(dims)}  # rows of the caller's array, updated in place
     data = torch.as_tensor(data.data if hasattr(data, "data") and not isinstance(data, torch.Tensor) else data,
                            device=dev).to(torch.float32)
     R = kernels.forward(recon, space)
     F = [0.5 * float(((R - data).double() ** 2).sum())]
     E = [F[0] + (reg(recon) if reg.kind else 0.0)]
-    rs = np.random.RandomState(seed)
+    rs = np.random.RandomState(seed) if rs is None else rs
     resid = R - data
     for it in range(max_iter[0]):
         brk = [True, True]
@@ -500,9 +613,19 @@ def linesearch_block(recon, data, max_iter, fidelity=None, reg=None, Radon=None,
                      tol=1e-4, min_iter=1, **kwargs):
     """Drop-in for GD_lib.linesearch_block (GD_lib.py:82-178): fits `recon`
     in place, returns (recon, E, F)."""
-    if guess is not None:
-        raise NotImplementedError("the initial-guess path (KL_GaussRadon) is not part of this build")
     _check_fidelity(fidelity)
     space = Radon.range if Radon is not None else kwargs["space"]
+    kern = kernels_of(Radon)
+    if guess is not None:
+        ndim = sum(c.lower() in dim.lower() for c in "xrI")
+        h = np.full((ndim, len(recon)), 1e-2)
+        rs = np.random.RandomState(1)
+
+        def run(sub, mi):
+            _, Es, Fs = gd_linesearch_block(sub, data, mi, space, dim, tol, min_iter, reg=reg, return_F=True,
+                                            kernels=kern, rs=rs, h=h[:, :len(sub)])
+            return Es, Fs
+
+        return greedy(recon, data, max_iter, reg, space, guess, _Stage(kern, run))
     return gd_linesearch_block(recon, data, max_iter, space, dim, tol, min_iter, reg=reg, return_F=True,
-                               kernels=kernels_of(Radon))
+                               kernels=kern)

@@ -165,6 +165,26 @@ size_t gt_psf_volume_workspace(const gt_atoms *atoms, const gt_vol *vol);
 int gt_psf_volume(const gt_atoms *atoms, const gt_vol *vol, float *u, void *ws, size_t ws_bytes, void *stream);
 
 /*
+ * Sinogram moments for the KL moment initialiser (KL_GaussRadon.py:57-190):
+ * replaces MOM0 / MOM1 / MOM2 (:221-392) and the f updates of
+ * doKL_ProjGDStep_iso (:128) and doKL_LagrangeStep_iso (Newton_sino_2D,
+ * :193-229).  Per angle a, FP64, into out[a][5] (device):
+ *   sum f, sum f p0, sum f p1, sum f p0^2, sum f p1^2   (p1 terms 0 in 2D)
+ * over the detector, with f formed per pixel from S and Q ([nA][n0][n1],
+ * device, float32):
+ *   GT_KL_CLAMP     f = max(0, a S + b Q)        (Q may be NULL: b ignored)
+ *   GT_KL_LAGRANGE  f + a log f = S + a log(b Q), a >= 0 (f = S when a == 0,
+ *                   f = 0 where b Q <= 1e-15)
+ * Only geom->dim, nA, n0, n1, p0 and p1 are used.  Deterministic (fixed-order
+ * reduction).
+ */
+#define GT_KL_CLAMP 0
+#define GT_KL_LAGRANGE 1
+size_t gt_sino_moments_workspace(const gt_geom *geom);
+int gt_sino_moments(const gt_geom *geom, const float *S, const float *Q, int mode, double a, double b, double *out,
+                    void *ws, size_t ws_bytes, void *stream);
+
+/*
  * Fourier evaluation on sample planes (FTGaussian3D.evaluate hyperplane and
  * 2.5D branches, GaussDictCode/bin/FTGaussian3D.py:12-38,113-148,167-196;
  * FTGaussian2D.evaluate, GaussDictCode/bin/FTGaussian2D.py:13-35,53-100):
