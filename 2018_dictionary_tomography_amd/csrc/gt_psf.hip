@@ -31,13 +31,42 @@
 
 namespace gt {
 
-// 8-point Gauss-Legendre nodes and weights on [-1, 1] (Abramowitz and Stegun, table 25.4).
-__device__ constexpr float GL8X[8] = {-0.9602898564975363f, -0.7966664774136267f, -0.5255324099163290f,
-                                      -0.1834346424956498f, 0.1834346424956498f,  0.5255324099163290f,
-                                      0.7966664774136267f,  0.9602898564975363f};
-__device__ constexpr float GL8W[8] = {0.1012285362903763f, 0.2223810344533745f, 0.3137066458778873f,
-                                      0.3626837833783620f, 0.3626837833783620f, 0.3137066458778873f,
-                                      0.2223810344533745f, 0.1012285362903763f};
+// Gauss-Legendre rules of 3, 4, 5, 6 and 8 points on [-1, 1], concatenated
+// (offsets 0, 3, 7, 12, 18).
+__device__ constexpr float GLX[26] = {
+    -0.7745966692414834f, 0.0000000000000000f, 0.7745966692414834f, -0.8611363115940526f, -0.3399810435848563f,
+    0.3399810435848563f, 0.8611363115940526f, -0.9061798459386640f, -0.5384693101056831f, 0.0000000000000000f,
+    0.5384693101056831f, 0.9061798459386640f, -0.9324695142031521f, -0.6612093864662645f, -0.2386191860831969f,
+    0.2386191860831969f, 0.6612093864662645f, 0.9324695142031521f, -0.9602898564975362f, -0.7966664774136267f,
+    -0.5255324099163290f, -0.1834346424956498f, 0.1834346424956498f, 0.5255324099163290f, 0.7966664774136267f,
+    0.9602898564975362f};
+__device__ constexpr float GLW[26] = {
+    0.5555555555555557f, 0.8888888888888888f, 0.5555555555555557f, 0.3478548451374537f, 0.6521451548625462f,
+    0.6521451548625462f, 0.3478548451374537f, 0.2369268850561894f, 0.4786286704993662f, 0.5688888888888890f,
+    0.4786286704993662f, 0.2369268850561894f, 0.1713244923791697f, 0.3607615730481389f, 0.4679139345726914f,
+    0.4679139345726914f, 0.3607615730481389f, 0.1713244923791697f, 0.1012285362903767f, 0.2223810344533743f,
+    0.3137066458778870f, 0.3626837833783618f, 0.3626837833783618f, 0.3137066458778870f, 0.2223810344533743f,
+    0.1012285362903767f};
+
+// Points of the rule for an interval of length `len` over which the
+// integrand varies on the scale `sc` (a standard deviation): the error of
+// each is below about 1e-7 of (peak x len) wherever the interval sits on a
+// Gaussian (3 points to len <= 0.5 sc, 4 to 1.0, 5 to 1.4, 6 to 2.0; 8 beyond).
+__device__ inline int psf_rule(float len, float sc, int &off)
+{
+    const float H = len / sc;
+    if (H <= 0.5f) { off = 0; return 3; }
+    if (H <= 1.0f) { off = 3; return 4; }
+    if (H <= 1.4f) { off = 7; return 5; }
+    if (H <= 2.0f) { off = 12; return 6; }
+    off = 18;
+    return 8;
+}
+
+// Conditional standard deviation along an axis whose log2-exponent
+// coefficient is qd (< 0): the integrand's scale along that axis.
+__device__ inline float psf_csig(float qd) { return qd < 0.0f ? rsqrtf(-1.3862943611198906f * qd) : 3.0e38f; }
+
 constexpr int PSF_MAXPIECES = 8;
 constexpr int PSF_MS = 40;  // floats per (atom, angle) moment record: S1[15] | Q[21] | f part | pad
 
@@ -138,11 +167,101 @@ __device__ inline int psf_pieces(float L, float U, float sig, float &h)
     return n;
 }
 
+// Inner integrals in closed form: G[i] = int_l^h u^i exp(-k u^2) du, i < NJ,
+// from erf / erfc (G0) and the recursion
+//   G_i = [-u^(i-1) exp(-k u^2) / (2k)]_l^h + (i - 1) / (2k) G_(i-2).
+// Used only when sqrt(k) (h - l) >= sqrt(2), where the erf difference is well
+// conditioned; erfc of the side away from the peak avoids cancellation in
+// the tails.
+template <int NJ> __device__ inline void psf_inner(float k, float l, float h, float *G)
+{
+    const float sk = sqrtf(k);
+    const float al = sk * l, ah = sk * h;
+    float F;
+    if (al >= 0.0f)
+        F = erfcf(al) - erfcf(ah);
+    else if (ah <= 0.0f)
+        F = erfcf(-ah) - erfcf(-al);
+    else
+        F = erff(ah) - erff(al);
+    G[0] = 0.88622692545275801f / sk * F;  // sqrt(pi) / 2
+    if (NJ > 1) {
+        const float el = expf(-al * al), eh = expf(-ah * ah);
+        const float i2k = 0.5f / k;
+        G[1] = (el - eh) * i2k;
+        if (NJ > 2) G[2] = fmaf(l * el - h * eh, i2k, G[0] * i2k);
+        if (NJ > 3) G[3] = fmaf(l * l * el - h * h * eh, i2k, 2.0f * G[1] * i2k);
+        if (NJ > 4) G[4] = fmaf(l * l * l * el - h * h * h * eh, i2k, 3.0f * G[2] * i2k);
+    }
+}
+
+// int t^j exp(-k (t - mu)^2) over the same interval, j < NJ, from the G's (t = mu + u).
+template <int NJ> __device__ inline void psf_shift(float mu, const float *G, float *T)
+{
+    T[0] = G[0];
+    if (NJ > 1) T[1] = fmaf(mu, G[0], G[1]);
+    if (NJ > 2) T[2] = fmaf(mu, fmaf(mu, G[0], 2.0f * G[1]), G[2]);
+    if (NJ > 3) T[3] = fmaf(mu, fmaf(mu, fmaf(mu, G[0], 3.0f * G[1]), 3.0f * G[2]), G[3]);
+    if (NJ > 4) T[4] = fmaf(mu, fmaf(mu, fmaf(mu, fmaf(mu, G[0], 4.0f * G[1]), 6.0f * G[2]), 4.0f * G[3]), G[4]);
+}
+
+// NP x NP Gauss-Legendre rule on [L0, U0] x [L1, U1] (NP = 3 or 4), moments as psf_pixel.
+template <int NP, int NM>
+__device__ inline void psf_tensor(const float *q, float L0, float U0, float L1, float U1, float *M)
+{
+    constexpr float X3[3] = {-0.7745966692414834f, 0.0f, 0.7745966692414834f};
+    constexpr float W3[3] = {0.5555555555555557f, 0.8888888888888888f, 0.5555555555555557f};
+    constexpr float X4[4] = {-0.8611363115940526f, -0.3399810435848563f, 0.3399810435848563f, 0.8611363115940526f};
+    constexpr float W4[4] = {0.3478548451374537f, 0.6521451548625462f, 0.6521451548625462f, 0.3478548451374537f};
+    const float kap = (float)KAPPA;
+    const float m0 = 0.5f * (L0 + U0), r0 = 0.5f * (U0 - L0);
+    const float m1 = 0.5f * (L1 + U1), r1 = 0.5f * (U1 - L1);
+#pragma unroll
+    for (int a = 0; a < NP; a++) {
+        const float d0 = fmaf(r0, NP == 3 ? X3[a] : X4[a], m0);
+        const float wa = r0 * r1 * (NP == 3 ? W3[a] : W4[a]);
+        const float P0[5] = {1.0f, d0, d0 * d0, d0 * d0 * d0, d0 * d0 * d0 * d0};
+#pragma unroll
+        for (int b = 0; b < NP; b++) {
+            const float d1 = fmaf(r1, NP == 3 ? X3[b] : X4[b], m1);
+            const float e = pair_q(q, d0, d1);
+            const float w = (e >= kap) ? wa * (NP == 3 ? W3[b] : W4[b]) * __builtin_amdgcn_exp2f(e) : 0.0f;
+            if (NM == 1) {
+                M[0] += w;
+            } else {
+                const float p1[5] = {1.0f, d1, d1 * d1, d1 * d1 * d1, d1 * d1 * d1 * d1};
+#pragma unroll
+                for (int deg = 0; deg <= 4; deg++)
+#pragma unroll
+                    for (int j = 0; j <= deg; j++) {
+                        const int m = deg * (deg + 1) / 2 + j;
+                        if (m < NM) M[m] = fmaf(w, P0[deg - j] * p1[j], M[m]);
+                    }
+            }
+        }
+    }
+}
+
+template <int NM> struct PsfDeg {
+    static constexpr int NJ = NM == 1 ? 1 : NM <= 3 ? 2 : NM <= 6 ? 3 : NM <= 10 ? 4 : 5;
+};
+
 // Pixel-integrated moments of one (atom, angle) over the pixel centred at
 // (pc0, pc1): M[mono] = int mono(d) 2^q(d) dd (NM = 1: the value only).
+// The pixel is cut to the support's bounding box.  An axis whose interval
+// is longer than two marginal standard deviations is integrated in closed
+// form (the profile is a 1D Gaussian along it: the last axis for 2D
+// detectors, given the first-axis node); otherwise by composite 8-point
+// Gauss-Legendre on pieces of at most 2 sigma.  The closed form integrates
+// the untruncated Gaussian, which differs from the profile cut at m > 1 by
+// less than exp(-30) of its peak.  The number of points per piece follows
+// the piece's length over the integrand's scale along the axis (psf_rule).
 template <int K, int NM>
 __device__ inline void psf_pixel(const PsfRec &R, float pc0, float pc1, float *M)
 {
+    // 2D detectors: NM counts monomials in (d0, d1); 1D: powers of d0
+    constexpr int NJ = (K == 1) ? NM : PsfDeg<NM>::NJ;
+    constexpr float LN2 = 0.69314718055994531f;
 #pragma unroll
     for (int m = 0; m < NM; m++) M[m] = 0.0f;
     const float c0 = (pc0 - R.r.c_hi[0]) - R.r.c_lo[0];
@@ -153,14 +272,29 @@ __device__ inline void psf_pixel(const PsfRec &R, float pc0, float pc1, float *M
     const float *q = R.r.q;
     const float kap = (float)KAPPA;
     if (K == 1) {
+        if (n0 > 1) {
+            // exponent (natural log) -k (d - mu)^2 + peak
+            const float k = -LN2 * q[0];
+            const float mu = LN2 * q[3] / (2.0f * k);
+            const float peak = fmaf(fmaf(q[0], mu, q[3]), mu, q[5]);
+            if (peak < kap) return;
+            float G[NJ], T[NJ];
+            psf_inner<NJ>(k, L0 - mu, U0 - mu, G);
+            psf_shift<NJ>(mu, G, T);
+            const float w = __builtin_amdgcn_exp2f(peak);
+#pragma unroll
+            for (int m = 0; m < NM; m++) M[m] = w * T[m];
+            return;
+        }
+        int o0;
+        const int g0 = psf_rule(h0, psf_csig(q[0]), o0);
         for (int i0 = 0; i0 < n0; i0++) {
             const float mid = L0 + h0 * ((float)i0 + 0.5f);
-#pragma unroll
-            for (int a = 0; a < 8; a++) {
-                const float d = fmaf(0.5f * h0, GL8X[a], mid);
+            for (int a = 0; a < g0; a++) {
+                const float d = fmaf(0.5f * h0, GLX[o0 + a], mid);
                 const float e = fmaf(fmaf(q[0], d, q[3]), d, q[5]);
                 if (e >= kap) {
-                    float w = 0.5f * h0 * GL8W[a] * __builtin_amdgcn_exp2f(e);
+                    float w = 0.5f * h0 * GLW[o0 + a] * __builtin_amdgcn_exp2f(e);
 #pragma unroll
                     for (int m = 0; m < NM; m++) {
                         M[m] = fmaf(w, 1.0f, M[m]);
@@ -176,31 +310,63 @@ __device__ inline void psf_pixel(const PsfRec &R, float pc0, float pc1, float *M
     if (!(L1 < U1)) return;
     float h1;
     const int n1 = psf_pieces(L1, U1, R.sig[1], h1);
+    const bool closed = n1 > 1;
+    const float k1 = -LN2 * q[2];
+    int o0, o1;
+    const int g0 = psf_rule(h0, psf_csig(q[0]), o0);
+    const int g1 = psf_rule(h1, psf_csig(q[2]), o1);
+    if (!closed && n0 == 1 && g0 <= 4 && g1 <= 4) {
+        // the common case of an atom wider than the pixel: one tensor rule,
+        // unrolled, so that the lanes of a wave (neighbouring pixels of the
+        // same atom) run the same instructions
+        if (g0 <= 3 && g1 <= 3)
+            psf_tensor<3, NM>(q, L0, U0, L1, U1, M);
+        else
+            psf_tensor<4, NM>(q, L0, U0, L1, U1, M);
+        return;
+    }
     for (int i0 = 0; i0 < n0; i0++) {
         const float mid0 = L0 + h0 * ((float)i0 + 0.5f);
-        for (int a = 0; a < 8; a++) {
-            const float d0 = fmaf(0.5f * h0, GL8X[a], mid0);
-            const float wa = 0.5f * h0 * GL8W[a];
+        for (int a = 0; a < g0; a++) {
+            const float d0 = fmaf(0.5f * h0, GLX[o0 + a], mid0);
+            const float wa = 0.5f * h0 * GLW[o0 + a];
+            float P0[5] = {1.0f, d0, d0 * d0, d0 * d0 * d0, d0 * d0 * d0 * d0};
+            if (closed) {
+                // given d0 the profile is exp(-k1 (d1 - mu)^2) 2^peak
+                const float mu = LN2 * fmaf(q[1], d0, q[4]) / (2.0f * k1);
+                const float peak = pair_q(q, d0, mu);
+                if (peak < kap) continue;
+                float G[NJ], T[NJ];
+                psf_inner<NJ>(k1, L1 - mu, U1 - mu, G);
+                psf_shift<NJ>(mu, G, T);
+                const float w = wa * __builtin_amdgcn_exp2f(peak);
+#pragma unroll
+                for (int deg = 0; deg <= 4; deg++)
+#pragma unroll
+                    for (int j = 0; j <= deg; j++) {
+                        const int m = deg * (deg + 1) / 2 + j;
+                        if (m < NM) M[m] = fmaf(w * P0[deg - j], T[j < NJ ? j : 0], M[m]);
+                    }
+                continue;
+            }
             for (int i1 = 0; i1 < n1; i1++) {
                 const float mid1 = L1 + h1 * ((float)i1 + 0.5f);
-#pragma unroll
-                for (int b = 0; b < 8; b++) {
-                    const float d1 = fmaf(0.5f * h1, GL8X[b], mid1);
+                for (int b = 0; b < g1; b++) {
+                    const float d1 = fmaf(0.5f * h1, GLX[o1 + b], mid1);
                     const float e = pair_q(q, d0, d1);
                     if (e >= kap) {
-                        const float w = wa * (0.5f * h1 * GL8W[b]) * __builtin_amdgcn_exp2f(e);
+                        const float w = wa * (0.5f * h1 * GLW[o1 + b]) * __builtin_amdgcn_exp2f(e);
                         if (NM == 1) {
                             M[0] += w;
                         } else {
                             // monomials by degree: 1 | d0 d1 | d0^2 d0d1 d1^2 | ... (mono2 order)
-                            float p0[5] = {1.0f, d0, d0 * d0, d0 * d0 * d0, d0 * d0 * d0 * d0};
                             float p1[5] = {1.0f, d1, d1 * d1, d1 * d1 * d1, d1 * d1 * d1 * d1};
 #pragma unroll
                             for (int deg = 0; deg <= 4; deg++)
 #pragma unroll
                                 for (int j = 0; j <= deg; j++) {
                                     const int m = deg * (deg + 1) / 2 + j;
-                                    if (m < NM) M[m] = fmaf(w, p0[deg - j] * p1[j], M[m]);
+                                    if (m < NM) M[m] = fmaf(w, P0[deg - j] * p1[j], M[m]);
                                 }
                         }
                     }
@@ -560,40 +726,88 @@ template <int D> __device__ inline float psf_vexp(const PsfVolRec &R, const floa
     return -fmaf(a0, a0, a1 * a1);
 }
 
-// voxel integral of one atom: composite Gauss-Legendre over voxel x support box
+// NP^D tensor Gauss-Legendre rule over the box [L, L + 2r] (NP = 3 or 4).
+template <int D, int NP> __device__ inline float psf_vtensor(const PsfVolRec &R, const float *L, const float *r)
+{
+    constexpr float X3[3] = {-0.7745966692414834f, 0.0f, 0.7745966692414834f};
+    constexpr float W3[3] = {0.5555555555555557f, 0.8888888888888888f, 0.5555555555555557f};
+    constexpr float X4[4] = {-0.8611363115940526f, -0.3399810435848563f, 0.3399810435848563f, 0.8611363115940526f};
+    constexpr float W4[4] = {0.3478548451374537f, 0.6521451548625462f, 0.6521451548625462f, 0.3478548451374537f};
+    const float kap = (float)KAPPA;
+    float acc = 0.0f, dy[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int a = 0; a < NP; a++) {
+        dy[0] = fmaf(r[0], (NP == 3 ? X3[a] : X4[a]) + 1.0f, L[0]);
+        const float wa = NP == 3 ? W3[a] : W4[a];
+#pragma unroll
+        for (int b = 0; b < NP; b++) {
+            dy[1] = fmaf(r[1], (NP == 3 ? X3[b] : X4[b]) + 1.0f, L[1]);
+            const float wb = wa * (NP == 3 ? W3[b] : W4[b]);
+            if (D == 2) {
+                const float e = psf_vexp<D>(R, dy);
+                acc += (e >= kap) ? wb * __builtin_amdgcn_exp2f(e) : 0.0f;
+                continue;
+            }
+#pragma unroll
+            for (int c = 0; c < NP; c++) {
+                dy[2] = fmaf(r[2], (NP == 3 ? X3[c] : X4[c]) + 1.0f, L[2]);
+                const float e = psf_vexp<D>(R, dy);
+                acc += (e >= kap) ? wb * (NP == 3 ? W3[c] : W4[c]) * __builtin_amdgcn_exp2f(e) : 0.0f;
+            }
+        }
+    }
+    return acc * r[0] * r[1] * (D == 3 ? r[2] : 1.0f);
+}
+
+// voxel integral of one atom: composite Gauss-Legendre over voxel x support
+// box, pieces of at most 2 marginal standard deviations, points per piece
+// from the piece's length over the profile's scale along the axis
+// (psf_rule; the scale is the conditional standard deviation, 1 / sqrt(2 ln2
+// |column of U'|^2)).
 template <int D> __device__ inline float psf_voxel(const PsfVolRec &R, const float *yc, const float *hv)
 {
     float L[3], h[3];
-    int n[3] = {1, 1, 1};
+    int n[3] = {1, 1, 1}, g[3] = {1, 1, 1}, o[3] = {0, 0, 0};
+    const float cn[3] = {R.u[0] * R.u[0],
+                         D == 3 ? fmaf(R.u[3], R.u[3], R.u[1] * R.u[1]) : fmaf(R.u[2], R.u[2], R.u[1] * R.u[1]),
+                         D == 3 ? fmaf(R.u[5], R.u[5], fmaf(R.u[4], R.u[4], R.u[2] * R.u[2])) : 0.0f};
+    bool one = true;
+    int gmax = 0;
     for (int c = 0; c < D; c++) {
         const float dc = (yc[c] - R.xh[c]) - R.xl[c];
         L[c] = fmaxf(dc - hv[c], -R.hw[c]);
         const float U = fminf(dc + hv[c], R.hw[c]);
         if (!(L[c] < U)) return 0.0f;
         n[c] = psf_pieces(L[c], U, R.hw[c] * 0.12909944487358055f, h[c]);  // sigma = hw / sqrt(60)
+        g[c] = psf_rule(h[c], psf_csig(-cn[c]), o[c]);
+        one = one && n[c] == 1;
+        gmax = max(gmax, g[c]);
+    }
+    if (one && gmax <= 4) {
+        const float r[3] = {0.5f * h[0], 0.5f * h[1], 0.5f * h[2]};
+        return R.amp * (gmax <= 3 ? psf_vtensor<D, 3>(R, L, r) : psf_vtensor<D, 4>(R, L, r));
     }
     const float kap = (float)KAPPA;
     float acc = 0.0f;
     float dy[3] = {0.0f, 0.0f, 0.0f};
     for (int i0 = 0; i0 < n[0]; i0++)
-        for (int a = 0; a < 8; a++) {
-            dy[0] = fmaf(0.5f * h[0], GL8X[a], L[0] + h[0] * ((float)i0 + 0.5f));
-            const float w0 = 0.5f * h[0] * GL8W[a];
+        for (int a = 0; a < g[0]; a++) {
+            dy[0] = fmaf(0.5f * h[0], GLX[o[0] + a], L[0] + h[0] * ((float)i0 + 0.5f));
+            const float w0 = 0.5f * h[0] * GLW[o[0] + a];
             for (int i1 = 0; i1 < n[1]; i1++)
-                for (int b = 0; b < 8; b++) {
-                    dy[1] = fmaf(0.5f * h[1], GL8X[b], L[1] + h[1] * ((float)i1 + 0.5f));
-                    const float w1 = w0 * 0.5f * h[1] * GL8W[b];
+                for (int b = 0; b < g[1]; b++) {
+                    dy[1] = fmaf(0.5f * h[1], GLX[o[1] + b], L[1] + h[1] * ((float)i1 + 0.5f));
+                    const float w1 = w0 * 0.5f * h[1] * GLW[o[1] + b];
                     if (D == 2) {
                         const float e = psf_vexp<D>(R, dy);
                         if (e >= kap) acc = fmaf(w1, __builtin_amdgcn_exp2f(e), acc);
                         continue;
                     }
                     for (int i2 = 0; i2 < n[2]; i2++)
-#pragma unroll
-                        for (int c = 0; c < 8; c++) {
-                            dy[2] = fmaf(0.5f * h[2], GL8X[c], L[2] + h[2] * ((float)i2 + 0.5f));
+                        for (int c = 0; c < g[2]; c++) {
+                            dy[2] = fmaf(0.5f * h[2], GLX[o[2] + c], L[2] + h[2] * ((float)i2 + 0.5f));
                             const float e = psf_vexp<D>(R, dy);
-                            if (e >= kap) acc = fmaf(w1 * 0.5f * h[2] * GL8W[c], __builtin_amdgcn_exp2f(e), acc);
+                            if (e >= kap) acc = fmaf(w1 * 0.5f * h[2] * GLW[o[2] + c], __builtin_amdgcn_exp2f(e), acc);
                         }
                 }
         }

@@ -43,6 +43,12 @@ def sino_moments(S, space, Q=None, mode=GT_KL_CLAMP, a=1.0, b=0.0):
     """(nA, 5) FP64 host array: per angle sum f, sum f p0, sum f p1,
     sum f p0^2, sum f p1^2 with f = max(0, a S + b Q) (GT_KL_CLAMP) or the
     root of f + a log f = S + a log(b Q) (GT_KL_LAGRANGE)."""
+    return sino_moments_device(S, space, Q, mode, a, b).cpu().numpy()
+
+
+def sino_moments_device(S, space, Q=None, mode=GT_KL_CLAMP, a=1.0, b=0.0, out=None):
+    """sino_moments into a device (nA, 5) FP64 tensor, enqueued on the
+    current stream without waiting."""
     lib = load()
     dev = S.device
     geom = ops.device_geometry(space, dev)
@@ -53,12 +59,15 @@ def sino_moments(S, space, Q=None, mode=GT_KL_CLAMP, a=1.0, b=0.0):
         Q = Q.to(torch.float32).contiguous()
         if tuple(Q.shape) != tuple(geom.shape):
             raise ValueError(f"Q must have shape {geom.shape}, got {tuple(Q.shape)}")
-    out = torch.empty((geom.nA, 5), dtype=torch.float64, device=dev)
+    if out is None:
+        out = torch.empty((geom.nA, 5), dtype=torch.float64, device=dev)
+    if tuple(out.shape) != (geom.nA, 5) or out.dtype != torch.float64 or not out.is_contiguous():
+        raise ValueError(f"out must be a contiguous float64 tensor of shape ({geom.nA}, 5)")
     need = lib.gt_sino_moments_workspace(ctypes.byref(geom.struct))
     ws = ops._ws.get(need, dev)
     check(lib.gt_sino_moments(ctypes.byref(geom.struct), ops._ptr(S), ops._ptr(Q), int(mode), float(a), float(b),
                               ops._ptr(out), ops._ptr(ws), ws.numel(), stream_handle(dev)), "gt_sino_moments")
-    return out.cpu().numpy()
+    return out
 
 
 def orthog_op(If, Ify, T):

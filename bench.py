@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--autograd-reps", type=int, default=3,
                     help="timed forward+backward passes of the autograd leg (0 skips it)")
     ap.add_argument("--fourier-reps", type=int, default=5, help="timed evaluations of the Fourier leg (0 skips it)")
+    ap.add_argument("--extras-reps", type=int, default=5,
+                    help="timed calls of the point-spread / Fourier-plane / KL-moment leg (0 skips it)")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg and print its JSON")
     ap.add_argument("--skip-checks", action="store_true",
                     help="skip the FP64 parity spot check and the in-support pair count (counter runs)")
@@ -206,6 +208,69 @@ def fourier_leg(torch, gtp, dev, reps):
     pairs = 2000.0 * 128 ** 3
     return {"workload": "2000 atoms on the integer grid [-64, 64)^3, complex64", "ms_per_evaluation": ms,
             "atom_k_pairs_per_s": pairs / (ms * 1e-3), "rel_err_vs_fp64_sampled": err, "reps": reps}
+
+
+def _timed(torch, fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / reps
+
+
+def extras_leg(torch, gtp, dev, reps):
+    """Rows the reference times nowhere, on the 3D workload's geometry (61
+    angles +-60 deg about y, 128x128): the point-spread projector (forward
+    and L2 gradient + Hessian blocks of 1000 atoms drawn as the 3D
+    workload), the Fourier-slice planes of 2000 FT atoms and their complex
+    L2 derivatives, and the KL initialiser's sinogram moments on a 90 x
+    256^2 sinogram (HBM-bound: 8 bytes read per pixel)."""
+    W = importlib.import_module("2018_dictionary_tomography_amd.workloads")
+    psf = importlib.import_module("2018_dictionary_tomography_amd.psf")
+    Fo = importlib.import_module("2018_dictionary_tomography_amd.fourier")
+    KL = importlib.import_module("2018_dictionary_tomography_amd.kl")
+    rng = np.random.default_rng(45)
+    T, ortho, det = W.tilt_series(61, 60.0, det=128)
+    space = gtp.ProjSpace(orientations=T, ortho=ortho, detector=det)
+    rays = 61 * 128 * 128
+    out = {}
+    I, x, r, _ = W.draw_aniso3(1000, rng)
+    atoms = gtp.AtomElement(gtp.AtomSpace(3, isotropic=False), x, r, I, device=dev)
+    P = psf.forward(atoms, space)
+    C = P * 0.9
+    ms_f = _timed(torch, lambda: psf.forward(atoms, space, out=P), reps)
+    ms_d = _timed(torch, lambda: psf.derivs(atoms, space, C, "L2", 2), reps)
+    vol = gtp.VolSpace(box=(-1, 1), shape=64, dim=3)
+    u = psf.volume(atoms, vol)
+    ms_v = _timed(torch, lambda: psf.volume(atoms, vol, out=u), reps)
+    out["psf"] = {"workload": "1000 atoms, 61 x 128^2, pixel-integrated; voxel-integrated render on 64^3",
+                  "forward_ms": ms_f, "derivs_L2_ms": ms_d, "volume_ms": ms_v,
+                  "forward_atom_pixel_per_s": 1000 * rays / (ms_f * 1e-3),
+                  "derivs_atom_pixel_per_s": 1000 * rays / (ms_d * 1e-3)}
+    I, x, r, _ = W.draw_aniso3(2000, rng)
+    nI, nx, nr = Fo.analytic_gauss_fourier(I, x, r * np.sqrt(60.0))
+    ft = Fo.FTAtoms(nI, nx, nr, dev)
+    vol = Fo.GaussFTVolume(gtp.AtomSpace(3, isotropic=False), space)
+    planes = Fo.evaluate(ft, vol.grid)
+    ms_p = _timed(torch, lambda: Fo.evaluate(ft, vol.grid, out=planes), reps)
+    ms_k = _timed(torch, lambda: Fo.derivs_all(ft, vol.grid, planes * 0.9), reps)
+    out["fourier_planes"] = {"workload": "2000 FT atoms on the 61 Fourier-slice planes of 128^2 samples",
+                             "evaluate_ms": ms_p, "derivs_ms": ms_k,
+                             "atom_k_pairs_per_s": 2000 * rays / (ms_p * 1e-3)}
+    T3, o3, d3 = W.tilt_series(90, 90.0, det=256)
+    sp3 = gtp.ProjSpace(orientations=T3, ortho=o3, detector=d3)
+    S = torch.randn(sp3.shape, device=dev)
+    Q = torch.rand(sp3.shape, device=dev)
+    m = KL.sino_moments_device(S, sp3, Q, 0, 0.5, 0.5)
+    ms_m = _timed(torch, lambda: KL.sino_moments_device(S, sp3, Q, 0, 0.5, 0.5, out=m), max(reps, 20))
+    nbytes = 8.0 * S.numel()
+    out["kl_moments"] = {"workload": "90 x 256^2 sinogram, f = max(0, a S + b Q), 5 FP64 moments per angle",
+                         "ms": ms_m, "GB_per_s": nbytes / (ms_m * 1e-3) / 1e9,
+                         "hbm_frac": nbytes / (ms_m * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                         "bytes_per_launch": nbytes}
+    return out
 
 
 def support_pairs(torch, I, x, r, T, w0, w1, p0, p1):
@@ -368,6 +433,7 @@ def main():
     serving = serving_leg(torch, args, dev) if args.serve_requests > 0 else None
     autograd = autograd_leg(torch, gtp, dev, args.autograd_reps) if args.autograd_reps > 0 else None
     fourier = fourier_leg(torch, gtp, dev, args.fourier_reps) if args.fourier_reps > 0 else None
+    extras = extras_leg(torch, gtp, dev, args.extras_reps) if args.extras_reps > 0 else None
     cpu = None
     if args.cpu_atoms > 0:
         cpu = cpu_baseline(np.float32(I).astype(float), np.float32(x_cur).astype(float),
@@ -398,6 +464,7 @@ def main():
         "serving": serving,
         "autograd": autograd,
         "fourier": fourier,
+        "extras": extras,
     }
     print(json.dumps(out))
     return 0
