@@ -450,7 +450,7 @@ def greedy(recon, data, max_iter, reg, space, guess, stage):
             n = N
             mi = [mi[2], mi[1]]
         else:
-            res = ProjElement(space, data - fwd(recon[:n], space))
+            res = ProjElement(space, data - fwd(recon[:n], space), device=dev)
             recon[n - 1] = guess(res, recon[n - 1])
             sub = recon[:n]
             Fv = _sumsq(fwd(sub, space), data)
