@@ -128,7 +128,10 @@ class FTAtoms:
         f = lambda v: torch.as_tensor(np.ascontiguousarray(v, dtype=dtype), device=device)
         n = len(np.asarray(I).reshape(-1))
         dim = dim or (np.asarray(x).size // max(n, 1) if n else 3)
-        self.I, self.x, self.r = f(np.reshape(I, -1)), f(np.reshape(x, (-1, dim))), f(np.reshape(r, (n, -1)))
+        r = np.asarray(r)
+        # an empty set keeps the slot count of its (0, k) array (default: anisotropic)
+        r = r.reshape(n, -1) if n else r.reshape(0, r.shape[-1] if r.ndim == 2 else dim * (dim + 1) // 2)
+        self.I, self.x, self.r = f(np.reshape(I, -1)), f(np.reshape(x, (-1, dim))), f(r)
 
 
 # ---------------------------------------------------------------------------
