@@ -17,9 +17,10 @@
 // axis' spacing for both axes (:563-566).  Here the pixel integral itself is
 // computed: for one (atom, angle) the detector profile is alpha 2^(q(d)) with
 // q the record's quadratic form (d = p - p*), and the pixel is intersected with
-// the bounding box of the support {m <= 1} and integrated by composite
-// 8-point Gauss-Legendre rules on pieces no longer than two marginal standard
-// deviations (error below 1e-9 of the profile's mass).
+// the bounding box of the support {m <= 1} and integrated on pieces no longer
+// than two marginal standard deviations by Gauss-Legendre rules of 3 to 8
+// points sized to the piece (error below about 1e-7 of peak x width), the
+// second detector axis in closed form when its interval is long (psf_pixel).
 //
 // Derivatives use the projector's moment contraction (pair_finish) with the
 // point moments replaced by pixel-integrated ones, M_k[mono] = int_k mono(d)
