@@ -255,16 +255,43 @@ PIXEL = Kernels(psf.forward, lambda a, s, C, order: psf.derivs(a, s, C, "L2", or
                 lambda a, s, res: psf.derivs(a, s, res, "linear", 1)[1])
 
 
+def fourier_kernels(vol):
+    """The Fourier-plane kernels of a GaussFTVolume: values on its sample
+    planes (evaluation cut at |L k|^2 < 30) and the complex L2 derivatives
+    (cut at 60), as Radon(.) and Radon.L2_derivs give them in the reference's
+    Fourier-domain fits (Fourier_Transform.py:177-207, FTGaussian3D.py:12-92)."""
+    from . import fourier
+    grid = vol.grid
+
+    def resid_grad(a, s, res):
+        # gradient of 1/2 |R - data|^2 for the atoms of a, res = R - data
+        return fourier.derivs_all(a, grid, fourier.evaluate(a, grid) - res, 1)[1]
+
+    return Kernels(lambda a, s: fourier.evaluate(a, grid), lambda a, s, C, order: fourier.derivs_all(a, grid, C, order),
+                   resid_grad)
+
+
 def kernels_of(Radon):
     """POINT for GaussTomo (and a bare projection space), PIXEL for
-    SingleParticleGaussTomo."""
+    SingleParticleGaussTomo, the Fourier-plane kernels for GaussFTVolume."""
     from .atomFuncs import SingleParticleGaussTomo
+    from .fourier import GaussFTVolume
+    if isinstance(Radon, GaussFTVolume):
+        return fourier_kernels(Radon)
     return PIXEL if isinstance(Radon, SingleParticleGaussTomo) else POINT
 
 
 def _sumsq(R, data):
-    d = (R - data).double()
+    d = R - data
+    d = torch.view_as_real(d).double() if d.is_complex() else d.double()
     return 0.5 * float((d * d).sum())
+
+
+def _device_data(data, dev):
+    """data on the device: complex64 for Fourier-plane data, else float32."""
+    data = data.data if hasattr(data, "data") and not isinstance(data, torch.Tensor) else data
+    data = torch.as_tensor(data, device=dev)
+    return data.to(torch.complex64) if data.is_complex() else data.to(torch.float32)
 
 
 def _project_np(D, iso, th, d):
@@ -321,8 +348,7 @@ def newton_host(recon, space, data, max_iter, kernels=POINT, reg=None, tol=1e-4,
     eps = 1e-4
     nblk = 3 if block else 1
     h = np.ones((nblk, n)) if h is None else h
-    data = data.data if hasattr(data, "data") and not isinstance(data, torch.Tensor) else data
-    data = torch.as_tensor(data, device=dev).to(torch.float32)
+    data = _device_data(data, dev)
     R = kernels.forward(recon, space)
     F = [_sumsq(R, data)]
     E = [F[0] + (reg(recon) if reg.kind else 0.0)]
@@ -436,8 +462,7 @@ def greedy(recon, data, max_iter, reg, space, guess, stage):
     dev = recon.I.device
     from .spaces import ProjElement
     fwd = stage.kernels.forward
-    data = data.data if hasattr(data, "data") and not isinstance(data, torch.Tensor) else data
-    data = torch.as_tensor(data, device=dev).to(torch.float32)
+    data = _device_data(data, dev)
     if not hasattr(max_iter, "__iter__"):
         max_iter = [max_iter, 1]
     mi = list(max_iter) + [1] * (2 - len(max_iter))
@@ -478,8 +503,16 @@ class _Stage:
         return self.run(sub, mi)
 
 
+def _check_guess(guess, Radon):
+    from .fourier import GaussFTVolume
+    if guess is not None and isinstance(Radon, GaussFTVolume):
+        raise NotImplementedError("the greedy (guess) path places real-space atoms; it is not defined for "
+                                  "Fourier-space fits")
+
+
 def _newton_dropin(recon, data, max_iter, fidelity, reg, Radon, guess, tol, min_iter, block, kwargs):
     _check_fidelity(fidelity)
+    _check_guess(guess, Radon)
     space = Radon.range if Radon is not None else kwargs["space"]
     if guess is not None:
         kern = kernels_of(Radon)
@@ -554,10 +587,9 @@ def gd_linesearch_block(recon, data, max_iter, space, dim="xrI", tol=1e-4, min_i
     if h is None:
         h = np.full((len(dims), n), 1e-2)
     h = {c: h[t] for t, c in enumerate(dims)}  # rows of the caller's array, updated in place
-    data = torch.as_tensor(data.data if hasattr(data, "data") and not isinstance(data, torch.Tensor) else data,
-                           device=dev).to(torch.float32)
+    data = _device_data(data, dev)
     R = kernels.forward(recon, space)
-    F = [0.5 * float(((R - data).double() ** 2).sum())]
+    F = [_sumsq(R, data)]
     E = [F[0] + (reg(recon) if reg.kind else 0.0)]
     rs = np.random.RandomState(seed) if rs is None else rs
     resid = R - data
@@ -585,7 +617,7 @@ def gd_linesearch_block(recon, data, max_iter, space, dim="xrI", tol=1e-4, min_i
                     arr[j] = new
                     Rn = kernels.forward(recon, space)
                     rn = Rn - data
-                    Fn = 0.5 * float((rn.double() ** 2).sum())
+                    Fn = _sumsq(Rn, data)
                     F.append(Fn)
                     En = Fn + (reg(recon) if reg.kind else 0.0)
                     if En > E[-1]:
@@ -614,6 +646,7 @@ def linesearch_block(recon, data, max_iter, fidelity=None, reg=None, Radon=None,
     """Drop-in for GD_lib.linesearch_block (GD_lib.py:82-178): fits `recon`
     in place, returns (recon, E, F)."""
     _check_fidelity(fidelity)
+    _check_guess(guess, Radon)
     space = Radon.range if Radon is not None else kwargs["space"]
     kern = kernels_of(Radon)
     if guess is not None:

@@ -305,8 +305,13 @@ def analytic_gauss_ifourier(I, x, r):
 # Fitters
 # ---------------------------------------------------------------------------
 def l2_squared_loss(R, data):
-    """1/2 ||R - data||^2 (transport_loss.py:567-585), here in FP64."""
-    d = np.asarray(R, float) - np.asarray(data, float)
+    """1/2 ||R - data||^2 (transport_loss.py:567-585), here in FP64 (complex
+    data: the squared modulus)."""
+    R, data = np.asarray(R), np.asarray(data)
+    if np.iscomplexobj(R) or np.iscomplexobj(data):
+        d = R.astype(complex) - data.astype(complex)
+        return 0.5 * float(np.sum(d.real ** 2 + d.imag ** 2))
+    d = R.astype(float) - data.astype(float)
     return 0.5 * float(np.sum(d * d))
 
 
@@ -388,14 +393,25 @@ def newton_linesearch(I, x, r, geom, data, max_iter, tol=1e-4, min_iter=1, seed=
 
 
 def _projector(kind, n_sub):
-    """(radproj, derivs) of the point projector (GaussTomo) or of the
+    """(radproj, derivs) of the point projector (GaussTomo), of the
     pixel-integrated one (SingleParticleGaussTomo, n_sub nodes per pixel
-    and axis)."""
+    and axis), or of the Fourier-slice planes (GaussFTVolume: 'geometry' is
+    the sample tuple (k0, k1, w0, w1), values cut at |L k|^2 < 30 and L2
+    derivatives at 60, FT-space atoms)."""
     if kind == "point":
         return radproj, derivs
     if kind == "psf":
         return (lambda I, x, r, g: psf_radproj(I, x, r, g, n_sub),
                 lambda I, x, r, g, C, mode, order: psf_derivs(I, x, r, g, C, mode, order, n_sub))
+    if kind == "ft":
+        def fwd(I, x, r, g):
+            return ft_hyperplane(I, x, r, g[0], g[1], g[2], g[3], iso=np.asarray(r).shape[-1] == 1)
+
+        def der(I, x, r, g, C, mode, order):
+            if mode != "L2":
+                raise ValueError("Fourier-plane derivatives are L2 only")
+            return ft_derivs(I, x, r, g[0], g[1], g[2], g[3], C, iso=np.asarray(r).shape[-1] == 1)
+        return fwd, der
     raise ValueError(kind)
 
 
@@ -408,7 +424,7 @@ def gd_linesearch_block(I, x, r, geom, data, max_iter, dim="xrI", tol=1e-4, min_
     sinogram in place).  guess: the greedy insertion path as in newton_fit
     (GD_lib.py:113-126).  Returns (I, x, r, E) or, with return_F, also F
     (the fidelity of every proposal)."""
-    D = geom.dim
+    D = 3 if projector == "ft" else geom.dim
     I, x = np.array(I, float).reshape(-1), np.array(x, float).reshape(-1, D)
     r = np.array(r, float).reshape(I.size, -1)
     n_all = I.size
@@ -418,7 +434,8 @@ def gd_linesearch_block(I, x, r, geom, data, max_iter, dim="xrI", tol=1e-4, min_
     max_iter = list(max_iter) + [1] * (2 - len(max_iter))
     eps = 1e-8
     h = [[1e-2] * n_all for _ in dim]
-    data = np.asarray(data, float)
+    data = np.asarray(data)
+    data = data.astype(complex) if np.iscomplexobj(data) else data.astype(float)
     radproj, derivs = _projector(projector, n_sub)
     rs = np.random.RandomState(seed)
     arrs = {"I": I, "x": x, "r": r}
@@ -622,9 +639,10 @@ def newton_fit(I, x, r, geom, data, max_iter, tol=1e-4, min_iter=1, seed=1, reg=
     Returns (I, x, r, E, F): E after each update, F the fidelity of each
     update's proposal (also when rejected, as the reference records it).
     projector: 'point' (GaussTomo) or 'psf' (SingleParticleGaussTomo; the
-    reference's Newton only calls Radon(.) and Radon.L2_derivs)."""
+    reference's Newton only calls Radon(.) and Radon.L2_derivs) or 'ft'
+    (GaussFTVolume; geom is then the sample tuple (k0, k1, w0, w1))."""
     radproj, derivs = _projector(projector, n_sub)
-    D = geom.dim
+    D = 3 if projector == "ft" else geom.dim
     I, x = np.array(I, float).reshape(-1), np.array(x, float).reshape(-1, D)
     r = np.array(r, float).reshape(I.size, -1)
     iso = r.shape[1] == 1
@@ -635,7 +653,8 @@ def newton_fit(I, x, r, geom, data, max_iter, tol=1e-4, min_iter=1, seed=1, reg=
     eps = 1e-4
     nblk = 3 if block else 1
     h = [[1.0] * n_all for _ in range(nblk)]
-    data = np.asarray(data, float)
+    data = np.asarray(data)
+    data = data.astype(complex) if np.iscomplexobj(data) else data.astype(float)
     rs = np.random.RandomState(seed)
     th_of = lambda j: np.concatenate(([I[j]], x[j], r[j]))
 

@@ -151,3 +151,42 @@ def test_gauss_ft_volume_and_projection_dft(gt, oracle, cuda, Fo):
     # inverse DFT returns the sinogram
     back = Fo.dft_projections(FT, space, inverse=True)
     assert float((back.real - sino).abs().max() / sino.abs().max()) < 1e-5
+
+
+def _ft_fit_case(gt, Fo, cuda, seed):
+    W = importlib.import_module("2018_dictionary_tomography_amd.workloads")
+    T, ortho, det = W.tilt_series(5, 60.0, det=24)
+    space = gt.ProjSpace(orientations=T, ortho=ortho, detector=det)
+    vol = Fo.GaussFTVolume(gt.AtomSpace(3, isotropic=False), space)
+    rng = np.random.default_rng(seed)
+    n = 2
+    I = rng.uniform(0.8, 1.2, n)
+    x = rng.uniform(-0.4, 0.4, (n, 3))
+    r = np.concatenate((rng.uniform(0.5, 0.8, (n, 3)), rng.normal(0, 0.1, (n, 3))), 1)
+    data = Fo.evaluate(Fo.FTAtoms(I, x, r, cuda), vol.grid)
+    noise = rng.normal(size=data.shape) + 1j * rng.normal(size=data.shape)
+    data = data + torch.as_tensor(0.01 * float(data.abs().max()) * noise, dtype=torch.complex64, device=cuda)
+    start = (f32(I * 0.85), f32(x + 0.03), f32(r * 1.05))
+    g = tuple(f32(v) for v in vol.grid)
+    return vol, data, start, g
+
+
+@pytest.mark.parametrize("solver", ["Newton", "GD_block"])
+def test_fit_with_gauss_ft_volume_matches_oracle(gt, oracle, cuda, Fo, solver):
+    """The reference's fitters call only Radon(.) and Radon.L2_derivs, so
+    with a GaussFTVolume they fit FT-space atoms to Fourier-plane data (the
+    complex l2 loss): same energy histories as the FP64 restatement."""
+    vol, data, (I0, x0, r0), g = _ft_fit_case(gt, Fo, cuda, 61 + (solver == "Newton"))
+    recon = Fo.FTAtoms(I0, x0, r0, cuda)
+    recon, E, F = gt.getSolver(solver)(recon, data, [4, 1], None, None, vol, tol=1e-6)
+    d = data.cpu().numpy().astype(np.complex128)
+    if solver == "Newton":
+        Io, xo, ro, Eo, Fo_ = oracle.newton_fit(I0, x0, r0, g, d, [4, 1], tol=1e-6, projector="ft")
+        assert np.all(np.abs(F - Fo_) < 1e-5 * np.maximum(Fo_, Fo_[0]))
+    else:
+        Io, xo, ro, Eo = oracle.gd_linesearch_block(I0, x0, r0, g, d, [4, 1], tol=1e-6, projector="ft")
+    assert E.size == Eo.size and Eo[-1] < 0.5 * Eo[0]
+    assert np.abs(E - Eo).max() < 1e-5 * Eo[0]
+    assert np.abs(recon.x.cpu().numpy() - xo).max() < 1e-4
+    with pytest.raises(NotImplementedError):
+        gt.getSolver(solver)(recon, data, [1, 1], None, None, vol, guess=lambda res, a: a)

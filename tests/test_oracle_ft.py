@@ -69,3 +69,27 @@ def test_plane_points_match_point_evaluator(oracle):
         a = oracle.ft_points(I, x, r, K.reshape(-1, 3)).reshape(K.shape[:3])
         b = oracle.ft_hyperplane(I, x, r, k0, k1, w0, w1)
         assert np.allclose(a, b, rtol=0, atol=1e-14)
+
+
+def test_newton_fit_on_fourier_planes(oracle):
+    """newton_fit with the Fourier-plane projector (a GaussFTVolume fit):
+    energy never increases, its last entry is the loss of the final atoms,
+    and noise-free FT atoms are recovered from a perturbed start."""
+    rng = np.random.default_rng(3)
+    n = 2
+    I = rng.uniform(0.5, 1.5, n)
+    x = rng.uniform(-0.5, 0.5, (n, 3))
+    r = np.concatenate((rng.uniform(3, 6, (n, 3)), rng.normal(0, 0.5, (n, 3))), 1)
+    nI, nx, nr = oracle.analytic_gauss_fourier(I, x, r * np.sqrt(60))
+    nI = nI * 2000  # samples of order 1, so the fidelity dominates the initial damping
+    th = np.linspace(0, np.pi, 6, endpoint=False)
+    w0 = np.stack([np.cos(th), np.sin(th), 0 * th], 1)
+    w1 = np.tile([0, 0, 1.0], (6, 1))
+    k = np.linspace(-40, 40, 25)
+    g = (k, k, w0, w1)
+    data = oracle.ft_hyperplane(nI, nx, nr, k, k, w0, w1)
+    Io, xo, ro, E, F = oracle.newton_fit(nI * 0.8, nx + 0.02, nr * 1.1, g, data, [14, 1], tol=1e-12, projector="ft")
+    assert np.all(np.diff(E) <= 0)
+    assert abs(E[-1] - oracle.l2_squared_loss(oracle.ft_hyperplane(Io, xo, ro, k, k, w0, w1), data)) < 1e-12 * E[0]
+    assert E[-1] < 1e-7 * E[0]
+    assert np.abs(xo - nx).max() < 1e-4 and np.abs(Io / nI - 1).max() < 1e-3
