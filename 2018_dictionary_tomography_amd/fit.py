@@ -111,10 +111,13 @@ class FitRequest:
     of each atom in turn (NewtonGaussian.linesearch_block); perm: the atom
     order of each outer iteration (default: permutations(N, max_outer,
     seed)); h: starting step sizes ([3][N] for the block variant, else [N];
-    default 1)."""
+    default 1); stage: give the fit a staging slab (the energy pass then
+    commits at once and a rejected update is undone from the slab; without
+    it an accepted update is committed by a second pass).  Same results
+    either way."""
 
     def __init__(self, atoms, space, data, max_iter, tol=1e-4, min_iter=1, seed=1, record=True, device=None,
-                 reg=None, block=False, perm=None, h=None):
+                 reg=None, block=False, perm=None, h=None, stage=True):
         dev = device or atoms.I.device
         self.reg = as_regulariser(reg)
         self.block = bool(block)
@@ -170,7 +173,7 @@ class FitRequest:
         lib = _declare()
         self.scratch = torch.zeros(lib.gt_fit_scratch_bytes(self.geom.nA), dtype=torch.uint8, device=dev)
         # old residual of the update in flight: lets the energy pass commit at once
-        self.stage = torch.empty(self.geom.shape, dtype=torch.float32, device=dev)
+        self.stage = torch.empty(self.geom.shape, dtype=torch.float32, device=dev) if stage else None
         g = self.geom
         self.slot = gt_fit_slot(
             active=1 if self.N > 0 else 0, iso=self.iso, N=self.N, nA=g.nA, n0=g.n0, n1=g.n1, lenT=g.lenT,
@@ -179,7 +182,7 @@ class FitRequest:
             p0=g.p0.data_ptr(), p1=g.p1.data_ptr() if g.p1 is not None else None, perm=self.perm.data_ptr(),
             I=self.I.data_ptr(), x=self.x.data_ptr(), r=self.r.data_ptr(), resid=self.resid.data_ptr(),
             h=self.h.data_ptr(), Ehist=self.Ehist.data_ptr() if record else None, scratch=self.scratch.data_ptr(),
-            stage=self.stage.data_ptr(),
+            stage=self.stage.data_ptr() if stage else None,
             it=0, pos=0, inner=0, jj=0, brk0=1, brk1=1, done=0 if self.N > 0 else 1, pad1=0, E=self.E0,
             updates=0, accepted=0, block=int(self.block), reg_kind=int(self.reg.kind),
             Fhist=self.Fhist.data_ptr() if record else None, blk=0, ninf=self.ninf0, F=self.F0, Rsum=self.Rsum0)

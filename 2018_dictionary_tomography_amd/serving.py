@@ -182,7 +182,7 @@ class FitServer:
     always has at least one workgroup."""
 
     def __init__(self, device=None, max_slots=None, hbm_budget=64 << 30, work_per_launch=4.0e7, use_graph=True,
-                 keep_results=False, grouped=True):
+                 keep_results=False, grouped=True, stage_max_atoms=500):
         self.dev = device or torch.device("cuda", torch.cuda.current_device())
         self.capacity = F.max_blocks()
         self.max_slots = min(max_slots or self.capacity, self.capacity)
@@ -190,6 +190,10 @@ class FitServer:
         self.work_per_launch = work_per_launch
         self.use_graph = use_graph
         self.keep_results = keep_results
+        # fits of at most this many atoms get a staging slab (see FitRequest);
+        # measured on the serving workloads, a mix of larger fits serves more
+        # fits/s without it
+        self.stage_max_atoms = stage_max_atoms
         self.grouped = grouped
         self.table = torch.zeros(self.max_slots * F.SLOT_BYTES, dtype=torch.uint8, device=self.dev)
         self.host = torch.zeros(self.max_slots * F.SLOT_BYTES, dtype=torch.uint8).pin_memory()
@@ -282,7 +286,8 @@ class FitServer:
                 spec = queue.pop(0)
                 i = free.pop(0)
                 start, space, data = payloads[spec.rid] if payloads else build_request(spec, self.dev)
-                req = F.FitRequest(start, space, data, [spec.max_iter, 1], record=False)
+                req = F.FitRequest(start, space, data, [spec.max_iter, 1], record=False,
+                                   stage=spec.N <= self.stage_max_atoms)
                 self._write_slot(i, req.slot)
                 self.slots[i] = (spec, req, time.perf_counter() - t0)
                 progress[i] = 0

@@ -117,9 +117,9 @@ def test_newton_group_size_does_not_change_result(gt, oracle, cuda):
     start = gt.AtomElement(aspace, x0, r, I * 0.9, device=cuda)
     runs = []
     for G, staged in ((1, True), (2, True), (3, True), (5, True), (1, False), (3, False)):
-        req = F.FitRequest(start, space, data, [3, 1], tol=1e-6)
-        if not staged:  # commit after the accept decision instead of restoring on reject
-            req.slot.stage = None
+        # without a staging slab: commit after the accept decision instead of restoring on reject
+        req = F.FitRequest(start, space, data, [3, 1], tol=1e-6, stage=staged)
+        assert (req.stage is None) == (not staged)
         table = torch.as_tensor(np.frombuffer(req.slot_bytes(), dtype=np.uint8).copy(), device=cuda)
         blocks = torch.as_tensor(F.block_table([G], [4]), device=cuda)
         for _ in range(100):
