@@ -541,13 +541,12 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
     const int nd[2] = {S.n0, S.n1};
     char *scr = (char *)S.scratch;
     double *prec = (double *)scr;
-    // The staging slab lets the energy pass commit at once, so no pass over
-    // the residual follows an accepted update.  A single workgroup fitting
-    // alone is faster without it (304 vs 278 us per update on 61 angles x
-    // 128^2: the slab costs a store of every pixel of the union box), but in
-    // a served mix, which is bound by HBM traffic, the extra read of the
-    // commit pass costs more (about 2.5% fewer fits/s), so the slab is always
-    // used.  Both paths add the same dR to the same old value.
+    // A slot with a staging slab commits in the energy pass (no pass over the
+    // residual follows an accepted update; a rejected one is undone from the
+    // slab); without one, an accepted update is committed by a second pass.
+    // The slab costs a store of every pixel of the union box; which is faster
+    // depends on the load (the host decides per slot).  Both paths add the
+    // same dR to the same old value, so results do not depend on it.
     float *stage = S.stage;
     double *dEa = (double *)(scr + fit_off_dE(nA));
     unsigned *bar = (unsigned *)(scr + fit_off_bar(nA));
