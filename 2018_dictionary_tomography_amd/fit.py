@@ -552,7 +552,10 @@ def newton_linesearch(recon, data, max_iter, fidelity=None, reg=None, Radon=None
                       min_iter=1, **kwargs):
     """Drop-in for NewtonGaussian.linesearch: fits `recon` in place and returns
     (recon, E, F): E the energy (fidelity + regulariser) after each update, F
-    the fidelity of each update's proposal."""
+    the fidelity of each update's proposal.  Radon may be GaussTomo (device
+    fitter), SingleParticleGaussTomo or GaussFTVolume (host-driven loop over
+    that operator's kernels); guess (e.g. a KL initialiser) selects the
+    greedy insertion path, max_iter may then have a third entry."""
     return _newton_dropin(recon, data, max_iter, fidelity, reg, Radon, guess, tol, min_iter, False, kwargs)
 
 
@@ -647,7 +650,8 @@ def gd_linesearch_block(recon, data, max_iter, space, dim="xrI", tol=1e-4, min_i
 def linesearch_block(recon, data, max_iter, fidelity=None, reg=None, Radon=None, view=None, dim="xrI", guess=None,
                      tol=1e-4, min_iter=1, **kwargs):
     """Drop-in for GD_lib.linesearch_block (GD_lib.py:82-178): fits `recon`
-    in place, returns (recon, E, F)."""
+    in place, returns (recon, E, F); operators and guess as
+    newton_linesearch."""
     _check_fidelity(fidelity)
     _check_guess(guess, Radon)
     space = Radon.range if Radon is not None else kwargs["space"]
