@@ -229,3 +229,31 @@ def test_host_newton_with_point_kernels_equals_device_fitter(gt, cuda, psf):
     assert Eh.size == Ed.size
     assert np.abs(Eh - Ed).max() < 1e-5 * Ed[0] and np.abs(Fh - Fd).max() < 1e-5 * Fd[0]
     assert np.abs(a.x.cpu().numpy() - b.x.cpu().numpy()).max() < 1e-4
+
+
+@pytest.mark.parametrize("mode", ["L2", "linear"])
+def test_psf_wide_atoms_vs_fp64(gt, oracle, cuda, psf, mode):
+    """Atoms 3-10 pixels wide, the spread of the 3D workload's atoms on its
+    detector: every pixel takes the unrolled 3 x 3 / 4 x 4 rules."""
+    rng = np.random.default_rng(77 + (mode == "L2"))
+    W = importlib.import_module("2018_dictionary_tomography_amd.workloads")
+    T, ortho, _ = W.tilt_series(3, 60.0, axis=(0.2, 1.0, 0.1), det=40)
+    p = np.linspace(-1, 1, 40)
+    space = gt.ProjSpace(orientations=T, ortho=ortho, detector=[p, p])
+    pix = p[1] - p[0]
+    n = 3
+    sig = rng.uniform(3 * pix, 10 * pix, n)
+    r = np.concatenate(((1.0 / (sig * np.sqrt(60)))[:, None] * rng.uniform(0.9, 1.1, (n, 3)),
+                        (1.0 / (sig * np.sqrt(60)))[:, None] * rng.normal(0, 0.15, (n, 3))), 1)
+    I = rng.uniform(0.5, 1.5, n)
+    x = rng.uniform(-0.3, 0.3, (n, 3))
+    atoms = gt.AtomElement(gt.AtomSpace(3, isotropic=False), x, r, I, device=cuda)
+    P = psf.forward(atoms, space).cpu().numpy()
+    ref = oracle.psf_radproj(f32(I), f32(x), f32(r), _og(oracle, space), 12)
+    assert _rel(P, ref) < 1e-5
+    C = torch.as_tensor(ref * 0.9 + rng.normal(0, 0.01 * ref.max(), ref.shape), dtype=torch.float32, device=cuda)
+    f, g, H = (t.cpu().numpy() for t in psf.derivs(atoms, space, C, mode, 2))
+    fo, go, Ho = oracle.psf_derivs(f32(I), f32(x), f32(r), _og(oracle, space), C.double().cpu().numpy(), mode, 2, 8)
+    assert _rel(f, fo) < 1e-5
+    for j in range(n):
+        assert _rel(g[j], go[j]) < 1e-5 and _rel(H[j], Ho[j]) < 1e-5, j
