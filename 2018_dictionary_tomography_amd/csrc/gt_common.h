@@ -36,6 +36,15 @@ void prof_end(hipStream_t st, int id);
 constexpr size_t ALIGN = 256;
 __host__ __device__ inline size_t align_up(size_t v) { return (v + ALIGN - 1) / ALIGN * ALIGN; }
 
+// sin(2 pi t) and cos(2 pi t) of a phase t in turns, |t| <= 256: the
+// hardware sine and cosine, whose argument is in turns (quarter-rate, like
+// exp2); measured maximum absolute error 1.25e-7 for |t| <= 1.5 on gfx950.
+__device__ inline void ft_sincos_turns(float t, float &sn, float &cs)
+{
+    sn = __builtin_amdgcn_sinf(t);
+    cs = __builtin_amdgcn_cosf(t);
+}
+
 int check_atoms(const gt_atoms *a);
 int check_geom(const gt_geom *g, int dim);
 

@@ -9,7 +9,8 @@
 // would carry ~1e-5 relative error.  The setup stores x / (2 pi) as an FP32
 // pair (xh + xl, ~48 bits) and the kernel forms the phase in turns with an
 // exact FMA product xh * k = p + e, reduces p exactly by rint, and evaluates
-// sincospi on the small remainder.  Per-point complex sums are Kahan sums.
+// the hardware sine and cosine (argument in turns) on the small remainder.
+// Per-point complex sums are Kahan sums.
 #include "gt_common.h"
 
 namespace gt {
@@ -173,7 +174,7 @@ __global__ __launch_bounds__(256) void ft_grid_kernel(const FtRec *__restrict__ 
                         float err = err0;
                         const float t = t02 + turns(r.xh[1], r.xl[1], k1[q], err) + err;
                         float sn, cs;
-                        sincospif(2.0f * t, &sn, &cs);
+                        ft_sincos_turns(t, sn, cs);
                         const float a = r.amp * __builtin_amdgcn_exp2f(-nq);
                         ckahan(sr[q], cr[q], a * cs);
                         ckahan(si[q], ci[q], -a * sn);
@@ -193,7 +194,7 @@ __global__ __launch_bounds__(256) void ft_grid_kernel(const FtRec *__restrict__ 
                         float err = err0;
                         const float t = t02 + turns(r.xh[1], r.xl[1], k1[q], err) + err;
                         float sn, cs;
-                        sincospif(2.0f * t, &sn, &cs);
+                        ft_sincos_turns(t, sn, cs);
                         const float a = r.amp * __builtin_amdgcn_exp2f(-nq);
                         ckahan(sr[q], cr[q], a * cs);
                         ckahan(si[q], ci[q], -a * sn);

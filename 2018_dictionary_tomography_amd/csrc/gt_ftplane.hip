@@ -11,9 +11,10 @@
 // isotropic forms are |k|^2 / r^2 (3D) and r^2 |k|^2 (2D).  cut = 30 (3D),
 // 60 (2D).  For one atom and one angle the exponent is a 2x2 quadratic form
 // in p and the phase is linear in p, so an FP64 setup per (atom, angle)
-// reduces each sample to one quadratic, one compare, one exp2 and one
-// sincospi of a phase formed in turns from an exact split of the FP64
-// coefficients (as the grid kernel does).
+// reduces each sample to one quadratic, one compare, one exp2 and the
+// hardware sine and cosine of a phase formed in turns from an exact split of
+// the FP64 coefficients (as the grid kernel does; the FP64 evaluator uses
+// sincospi).
 //
 // Derivatives (FTGaussian3D.derivs, :55-92, kernels :199-652), for every atom
 // independently against all of C (the reference uses atom 0):
@@ -273,7 +274,7 @@ __global__ __launch_bounds__(256) void kp_eval_kernel(const KpRec<R> *__restrict
                 float err = 0.0f;
                 const float t = kp_turns(r.ch[0], r.cl[0], p0, err) + kp_turns(r.ch[1], r.cl[1], p1, err) + err;
                 float sn, cs;
-                sincospif(2.0f * t, &sn, &cs);
+                ft_sincos_turns(t, sn, cs);
                 const float am = r.amp * __builtin_amdgcn_exp2f(ex);
                 kp_kahan(sr, cr, am * cs);
                 kp_kahan(si, ci, -am * sn);
@@ -477,7 +478,7 @@ __global__ __launch_bounds__(KD_THREADS) void kd_kernel(gt_atoms A, KpDev<float>
                     float err = err1;
                     const float t = kp_turns(rec.ch[0], rec.cl[0], p0, err) + t1v + err;
                     float sn, cs;
-                    sincospif(2.0f * t, &sn, &cs);
+                    ft_sincos_turns(t, sn, cs);
                     const float e = __builtin_amdgcn_exp2f(ex);
                     const float2 cv = Ca[(size_t)i * P.m1 + c];
                     // w = (I E - C) conj(E), conj(E) = e (cs + i sn)
