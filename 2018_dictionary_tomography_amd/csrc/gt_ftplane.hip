@@ -300,6 +300,7 @@ __global__ __launch_bounds__(256) void kp_eval_kernel(const KpRec<R> *__restrict
 // Derivatives
 // ---------------------------------------------------------------------------
 constexpr int KD_THREADS = 512;
+constexpr int KD_UNR = 4;  // rows per step of the derivative walk (loads first)
 constexpr int KD_WAVES = KD_THREADS / 64;
 constexpr int KD_NV = 66;   // f part, 10 gradient entries, 55 upper Hessian entries
 constexpr double KD_CUT = 60.0;
@@ -471,30 +472,42 @@ __global__ __launch_bounds__(KD_THREADS) void kd_kernel(gt_atoms A, KpDev<float>
             float Are[5] = {0, 0, 0, 0, 0}, Aim[4] = {0, 0, 0, 0}, Av[5] = {0, 0, 0, 0, 0};
             float err1 = 0.0f;
             const float t1v = kp_turns(rec.ch[1], rec.cl[1], p1, err1);
-            for (int i = rec.lo[0]; i <= rec.hi[0]; i++) {
-                const float p0 = P.k0[i];
-                const float ex = p0 * fmaf(rec.q[0], p0, rec.q[1] * p1) + rec.q[2] * p1 * p1;
-                if (ex >= cutlog) {
-                    float err = err1;
-                    const float t = kp_turns(rec.ch[0], rec.cl[0], p0, err) + t1v + err;
-                    float sn, cs;
-                    ft_sincos_turns(t, sn, cs);
-                    const float e = __builtin_amdgcn_exp2f(ex);
-                    const float2 cv = Ca[(size_t)i * P.m1 + c];
-                    // w = (I E - C) conj(E), conj(E) = e (cs + i sn)
-                    const float wr = amp * e * e - e * (cv.x * cs - cv.y * sn);
-                    const float wi = -e * (cv.x * sn + cv.y * cs);
-                    const float vv = e * e;
-                    float pw = 1.0f;
+            // rows in order, KD_UNR per step with their loads issued first;
+            // outside the cut e = 0 and the sample adds exact zeros
+            const float q2p1 = rec.q[2] * p1 * p1, q1p1 = rec.q[1] * p1;
+            auto sample = [&](float p0, float2 cv) {
+                const float ex = p0 * fmaf(rec.q[0], p0, q1p1) + q2p1;
+                float err = err1;
+                const float t = kp_turns(rec.ch[0], rec.cl[0], p0, err) + t1v + err;
+                float sn, cs;
+                ft_sincos_turns(t, sn, cs);
+                const float e = ex >= cutlog ? __builtin_amdgcn_exp2f(ex) : 0.0f;
+                // w = (I E - C) conj(E), conj(E) = e (cs + i sn)
+                const float wr = amp * e * e - e * (cv.x * cs - cv.y * sn);
+                const float wi = -e * (cv.x * sn + cv.y * cs);
+                const float vv = e * e;
+                float pw = 1.0f;
 #pragma unroll
-                    for (int q = 0; q < 5; q++) {
-                        Are[q] = fmaf(wr, pw, Are[q]);
-                        Av[q] = fmaf(vv, pw, Av[q]);
-                        if (q < 4) Aim[q] = fmaf(wi, pw, Aim[q]);
-                        pw *= p0;
-                    }
+                for (int q = 0; q < 5; q++) {
+                    Are[q] = fmaf(wr, pw, Are[q]);
+                    Av[q] = fmaf(vv, pw, Av[q]);
+                    if (q < 4) Aim[q] = fmaf(wi, pw, Aim[q]);
+                    pw *= p0;
                 }
+            };
+            int i = rec.lo[0];
+            for (; i + KD_UNR - 1 <= rec.hi[0]; i += KD_UNR) {
+                float p0v[KD_UNR];
+                float2 cv[KD_UNR];
+#pragma unroll
+                for (int u = 0; u < KD_UNR; u++) {
+                    p0v[u] = P.k0[i + u];
+                    cv[u] = Ca[(size_t)(i + u) * P.m1 + c];
+                }
+#pragma unroll
+                for (int u = 0; u < KD_UNR; u++) sample(p0v[u], cv[u]);
             }
+            for (; i <= rec.hi[0]; i++) sample(P.k0[i], Ca[(size_t)i * P.m1 + c]);
             // combine the column sums with the powers of p1
             float pj[5] = {1.0f, p1, p1 * p1, p1 * p1 * p1, p1 * p1 * p1 * p1};
 #pragma unroll
