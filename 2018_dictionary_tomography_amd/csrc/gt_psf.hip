@@ -206,6 +206,10 @@ template <int NJ> __device__ inline void psf_shift(float mu, const float *G, flo
     if (NJ > 4) T[4] = fmaf(mu, fmaf(mu, fmaf(mu, fmaf(mu, G[0], 4.0f * G[1]), 6.0f * G[2]), 4.0f * G[3]), G[4]);
 }
 
+template <int NM> struct PsfDeg {
+    static constexpr int NJ = NM == 1 ? 1 : NM <= 3 ? 2 : NM <= 6 ? 3 : NM <= 10 ? 4 : 5;
+};
+
 // NP x NP Gauss-Legendre rule on [L0, U0] x [L1, U1] (NP = 3 or 4), moments as psf_pixel.
 template <int NP, int NM>
 __device__ inline void psf_tensor(const float *q, float L0, float U0, float L1, float U1, float *M)
@@ -217,35 +221,42 @@ __device__ inline void psf_tensor(const float *q, float L0, float U0, float L1, 
     const float kap = (float)KAPPA;
     const float m0 = 0.5f * (L0 + U0), r0 = 0.5f * (U0 - L0);
     const float m1 = 0.5f * (L1 + U1), r1 = 0.5f * (U1 - L1);
+    constexpr int NJ = PsfDeg<NM>::NJ;
 #pragma unroll
     for (int a = 0; a < NP; a++) {
         const float d0 = fmaf(r0, NP == 3 ? X3[a] : X4[a], m0);
         const float wa = r0 * r1 * (NP == 3 ? W3[a] : W4[a]);
-        const float P0[5] = {1.0f, d0, d0 * d0, d0 * d0 * d0, d0 * d0 * d0 * d0};
+        // B_j = sum over the inner nodes of w d1^j, then M[i, j] += d0^i B_j
+        float B[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; j++) B[j] = 0.0f;
 #pragma unroll
         for (int b = 0; b < NP; b++) {
             const float d1 = fmaf(r1, NP == 3 ? X3[b] : X4[b], m1);
             const float e = pair_q(q, d0, d1);
-            const float w = (e >= kap) ? wa * (NP == 3 ? W3[b] : W4[b]) * __builtin_amdgcn_exp2f(e) : 0.0f;
-            if (NM == 1) {
-                M[0] += w;
-            } else {
-                const float p1[5] = {1.0f, d1, d1 * d1, d1 * d1 * d1, d1 * d1 * d1 * d1};
+            float w = (e >= kap) ? wa * (NP == 3 ? W3[b] : W4[b]) * __builtin_amdgcn_exp2f(e) : 0.0f;
 #pragma unroll
-                for (int deg = 0; deg <= 4; deg++)
-#pragma unroll
-                    for (int j = 0; j <= deg; j++) {
-                        const int m = deg * (deg + 1) / 2 + j;
-                        if (m < NM) M[m] = fmaf(w, P0[deg - j] * p1[j], M[m]);
-                    }
+            for (int j = 0; j < NJ; j++) {
+                B[j] += w;
+                w *= d1;
             }
         }
+        if (NM == 1) {
+            M[0] += B[0];
+            continue;
+        }
+        const float P0[5] = {1.0f, d0, d0 * d0, d0 * d0 * d0, d0 * d0 * d0 * d0};
+#pragma unroll
+        for (int deg = 0; deg <= 4; deg++)
+#pragma unroll
+            for (int j = 0; j <= deg; j++) {
+                const int m = deg * (deg + 1) / 2 + j;
+                if (m < NM) M[m] = fmaf(P0[deg - j], B[j < NJ ? j : 0], M[m]);
+            }
     }
 }
 
-template <int NM> struct PsfDeg {
-    static constexpr int NJ = NM == 1 ? 1 : NM <= 3 ? 2 : NM <= 6 ? 3 : NM <= 10 ? 4 : 5;
-};
+
 
 // Pixel-integrated moments of one (atom, angle) over the pixel centred at
 // (pc0, pc1): M[mono] = int mono(d) 2^q(d) dd (NM = 1: the value only).
