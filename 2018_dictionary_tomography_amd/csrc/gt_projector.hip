@@ -2,9 +2,9 @@
 //
 // Pipeline (all stream-ordered, no host synchronisation):
 //   setup    one thread per (atom, angle): FP64 detector Gaussian -> PairRec
-//   forward  one block per (angle, 16x64 pixel tile): stages the angle's
-//            records that touch the tile in LDS; each of 8 waves owns an
-//            8x16 rectangle (2 pixels per lane), Kahan sums
+//   forward  one block per (angle, 16x64 pixel tile), centre tiles first:
+//            stages the angle's records that touch the tile in LDS; each of
+//            8 waves owns an 8x16 rectangle (2 pixels per lane), Kahan sums
 //   moments  one wave per (atom, angle): FP32 separable pixel moments over
 //            the footprint box, LDS-transposed wave reduction, no atomics
 //   finish   one wave per atom, one lane per angle: FP64 moment -> (f, g, H)
@@ -101,6 +101,39 @@ __device__ inline int stage_chunk(const PairRec *__restrict__ ra, int base, int 
 // order is fixed): a wave then walks only records that reach its pixels.
 constexpr int F3_WAVES = 8, F3_CHUNK = F3_WAVES * 64;
 
+// The tiles at the detector centre carry several times the records of those
+// at its edges, and the grid holds more blocks than fit on the device at once:
+// blocks are numbered so that the heavy tiles start first and the light ones
+// fill the tail.  Block b takes angle b % nA and the tile of Morton index
+// b / nA over the centre-out ranks of the tile row and column (grid of
+// 2^e0 x 2^e1 ranks; ranks beyond the tile counts are empty blocks).
+
+// t-th index of 0..R-1 from the centre outwards: c, c+1, c-1, c+2, ...
+__device__ inline int centre_out(int t, int R)
+{
+    const int c = (R - 1) >> 1;
+    return (t & 1) ? c + ((t + 1) >> 1) : c - (t >> 1);
+}
+
+// Morton index m of a 2^e0 x 2^e1 grid -> (t0, t1); the bits beyond the
+// smaller side all go to the larger one.
+__device__ inline void morton_split(unsigned m, int e0, int e1, int &t0, int &t1)
+{
+    const int e = min(e0, e1);
+    unsigned u0 = 0u, u1 = 0u;
+    for (int i = 0; i < e; i++) {
+        u0 |= ((m >> (2 * i)) & 1u) << i;
+        u1 |= ((m >> (2 * i + 1)) & 1u) << i;
+    }
+    const unsigned rest = m >> (2 * e);
+    if (e0 > e)
+        u0 |= rest << e;
+    else
+        u1 |= rest << e;
+    t0 = (int)u0;
+    t1 = (int)u1;
+}
+
 __device__ inline int stage_rects(const PairRec *__restrict__ ra, int base, int N, int row0, int col0, int thi0,
                                   int thi1, PairRec *list, unsigned short (*wl)[F3_CHUNK], int (*cnt)[F3_WAVES],
                                   int *wlen)
@@ -185,19 +218,21 @@ __device__ inline void f3_pixels(const float4 &f0, const float4 &f1, const float
     kahan_add(sB, cB, qB >= kap ? f0.x * eB : 0.0f);
 }
 
-__global__ __launch_bounds__(F3_WAVES * 64) void forward3_kernel(const PairRec *__restrict__ recs, int N, int n0,
-                                                                 int n1, const float *__restrict__ p0,
+__global__ __launch_bounds__(F3_WAVES * 64) void forward3_kernel(const PairRec *__restrict__ recs, int N, int nA,
+                                                                 int n0, int n1, const float *__restrict__ p0,
                                                                  const float *__restrict__ p1, float *__restrict__ R,
-                                                                 const float *__restrict__ sub)
+                                                                 const float *__restrict__ sub, int e0, int e1)
 {
     __shared__ PairRec list[F3_CHUNK];
     __shared__ alignas(16) unsigned short wl[F3_WAVES][F3_CHUNK];  // 8-byte reads of 4 indices
     __shared__ int cnt[F3_WAVES + 1][F3_WAVES];
     __shared__ int wlen[F3_WAVES];
-    const int a = blockIdx.y;
-    const int tilesC = (n1 + TILE_COLS - 1) / TILE_COLS;
-    const int tr = blockIdx.x / tilesC, tc = blockIdx.x - tr * tilesC;
-    const int row0 = tr * TILE_ROWS, col0 = tc * TILE_COLS;
+    const int tilesR = (n0 + TILE_ROWS - 1) / TILE_ROWS, tilesC = (n1 + TILE_COLS - 1) / TILE_COLS;
+    const int a = (int)(blockIdx.x % (unsigned)nA);
+    int t0, t1;
+    morton_split(blockIdx.x / (unsigned)nA, e0, e1, t0, t1);
+    if (t0 >= tilesR || t1 >= tilesC) return;  // whole block: no tile of this rank
+    const int row0 = centre_out(t0, tilesR) * TILE_ROWS, col0 = centre_out(t1, tilesC) * TILE_COLS;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wr0 = row0 + 8 * (wave >> 2), wc0 = col0 + 16 * (wave & 3);
     const int i1 = wc0 + (lane & 15);
@@ -516,9 +551,14 @@ int launch_forward(const gt_atoms *A, const gt_geom *G, const PairRec *recs, flo
     }
     prof_begin(st, GT_K_FORWARD);
     if (G->dim == 3) {
-        const int tiles = ((G->n0 + TILE_ROWS - 1) / TILE_ROWS) * ((G->n1 + TILE_COLS - 1) / TILE_COLS);
-        hipLaunchKernelGGL(forward3_kernel, dim3(tiles, G->nA), dim3(F3_WAVES * 64), 0, st, recs, A->n, G->n0, G->n1,
-                           G->p0, G->p1, R, sub);
+        int e0 = 0, e1 = 0;
+        while ((1 << e0) < (G->n0 + TILE_ROWS - 1) / TILE_ROWS) e0++;
+        while ((1 << e1) < (G->n1 + TILE_COLS - 1) / TILE_COLS) e1++;
+        const unsigned long long blocks = (unsigned long long)G->nA << (e0 + e1);
+        GT_CHECK_ARG(blocks < (1ull << 31), "forward: too many detector tiles (%d x %d, %d angles)", G->n0, G->n1,
+                     G->nA);
+        hipLaunchKernelGGL(forward3_kernel, dim3((unsigned)blocks), dim3(F3_WAVES * 64), 0, st, recs, A->n, G->nA,
+                           G->n0, G->n1, G->p0, G->p1, R, sub, e0, e1);
     } else {
         const int tiles = (G->n0 + 255) / 256;
         hipLaunchKernelGGL(forward2_kernel, dim3(tiles, G->nA), dim3(256), 0, st, recs, A->n, G->n0, G->p0, R, sub);

@@ -65,6 +65,28 @@ def test_forward_2p5d_and_nonsquare(gt, oracle, cuda):
     assert relerr(R, R64) < TOL
 
 
+@pytest.mark.parametrize("N,shape", [(1100, (37, 53)), (300, (5, 100)), (300, (130, 7)), (65, (1, 1)),
+                                     (513, (64, 200))])
+def test_forward_3d_tile_order_shapes(gt, oracle, cuda, N, shape):
+    """Detectors that are not multiples of the 16 x 64 pixel tiles or their
+    8 x 16 wave rectangles, tile grids that are not powers of two (empty
+    blocks in the centre-first block order), several 512-record staging chunks
+    per tile, a one-pixel detector; repeated calls give the same bits."""
+    W = __import__("importlib").import_module("2018_dictionary_tomography_amd.workloads")
+    rng = np.random.default_rng(N + shape[0])
+    I, x, r, _ = W.draw_aniso3(N, rng)
+    T, ortho, _ = W.tilt_series(7, 60.0, det=8)
+    det = [gt.cell_centres(-1, 1, shape[0]), gt.cell_centres(-0.9, 1.1, shape[1])]
+    space = _space3(gt, T, ortho, det)
+    atoms = _atoms(gt, I, x, r, 3, cuda)
+    R = gt.ops.forward(atoms, space)
+    R64 = oracle.radproj(f32(I), f32(x), f32(r), _ogeom(oracle, space))
+    assert R.shape == R64.shape
+    assert relerr(R.cpu().numpy(), R64) < TOL
+    for _ in range(3):
+        assert torch.equal(gt.ops.forward(atoms, space), R)
+
+
 def test_forward_2d_iso_c1(gt, oracle, cuda):
     W = __import__("importlib").import_module("2018_dictionary_tomography_amd.workloads")
     rng = np.random.default_rng(11)
