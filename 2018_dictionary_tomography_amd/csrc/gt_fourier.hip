@@ -108,24 +108,51 @@ __device__ inline float turns(float xh, float xl, float k, float &err)
     return p - rintf(p);
 }
 
-constexpr int FCHUNK = 128;
+// Block = 16 waves on a brick of FT_ROWS (k0, one row per wave of a group)
+// x 4 (k1, per lane) x 64 (k2, lane) samples.  Every atom's envelope is
+// centred at k = 0, so a brick near the origin lists nearly all atoms and one
+// at the edge few: the block's FT_GROUPS wave groups take every FT_GROUPS-th
+// record of the staged list each, and the groups' compensated sums are added
+// in group order at the end (deterministic).  Blocks are numbered bricks
+// nearest the grid centre first (Morton order over the centre-out brick
+// ranks), and the grid holds several times the blocks that fit at once, so
+// the light edge bricks fill the tail.  (2 rows x 8 groups measured best of
+// 1 x 16, 2 x 8 and 4 x 4.)
+constexpr int FT_ROWS = 2, FT_GROUPS = 16 / FT_ROWS, FT_THREADS = 16 * 64, FCHUNK = 256;
 
-// block brick = 4 (k0, one per wave) x 4 (k1, per lane) x 64 (k2, lane)
-__global__ __launch_bounds__(256) void ft_grid_kernel(const FtRec *__restrict__ recs, int N, gt_kgrid Kg,
-                                                      float2 *__restrict__ out)
+// t-th index of 0..R-1 from the centre outwards: c, c+1, c-1, c+2, ...
+__device__ inline int ft_centre_out(int t, int R)
+{
+    const int c = (R - 1) >> 1;
+    return (t & 1) ? c + ((t + 1) >> 1) : c - (t >> 1);
+}
+
+__global__ __launch_bounds__(FT_THREADS) void ft_grid_kernel(const FtRec *__restrict__ recs, int N, gt_kgrid Kg,
+                                                             float2 *__restrict__ out, int e0, int e1, int e2)
 {
     __shared__ FtRec list[FCHUNK];
-    __shared__ int wcnt[4];
+    __shared__ int wcnt[FCHUNK / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nb2 = (Kg.m2 + 63) / 64, nb1 = (Kg.m1 + 3) / 4;
-    int b = blockIdx.x;
-    const int bb2 = b % nb2;
-    b /= nb2;
-    const int bb1 = b % nb1;
-    const int bb0 = b / nb1;
-    const int i0 = bb0 * 4 + wave, i1b = bb1 * 4, i2 = bb2 * 64 + lane;
-    const int blo[3] = {bb0 * 4, i1b, bb2 * 64};
-    const int bhi[3] = {min(bb0 * 4 + 3, Kg.m0 - 1), min(i1b + 3, Kg.m1 - 1), min(bb2 * 64 + 63, Kg.m2 - 1)};
+    const int grp = wave / FT_ROWS, w2 = wave % FT_ROWS;
+    const int nb0 = (Kg.m0 + FT_ROWS - 1) / FT_ROWS, nb1 = (Kg.m1 + 3) / 4, nb2 = (Kg.m2 + 63) / 64;
+    // Morton index -> centre-out ranks (t0, t1, t2); ranks beyond the brick counts are empty blocks
+    int t[3] = {0, 0, 0};
+    {
+        const int e[3] = {e0, e1, e2};
+        unsigned m = blockIdx.x;
+        for (int bit = 0; m != 0u && bit < 24; bit++)
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+                if (bit < e[d]) {
+                    t[d] |= (int)(m & 1u) << bit;
+                    m >>= 1;
+                }
+    }
+    if (t[0] >= nb0 || t[1] >= nb1 || t[2] >= nb2) return;  // whole block
+    const int bb0 = ft_centre_out(t[0], nb0), bb1 = ft_centre_out(t[1], nb1), bb2 = ft_centre_out(t[2], nb2);
+    const int i0 = bb0 * FT_ROWS + w2, i1b = bb1 * 4, i2 = bb2 * 64 + lane;
+    const int blo[3] = {bb0 * FT_ROWS, i1b, bb2 * 64};
+    const int bhi[3] = {min(bb0 * FT_ROWS + FT_ROWS - 1, Kg.m0 - 1), min(i1b + 3, Kg.m1 - 1), min(bb2 * 64 + 63, Kg.m2 - 1)};
     const float k0 = Kg.k0[min(i0, Kg.m0 - 1)];
     const float k2 = Kg.k2[min(i2, Kg.m2 - 1)];
     float k1[4], sr[4], cr[4], si[4], ci[4];
@@ -136,7 +163,7 @@ __global__ __launch_bounds__(256) void ft_grid_kernel(const FtRec *__restrict__ 
     }
     const float cut = (float)(FT_CUT * HALF_LOG2E_SQRT * HALF_LOG2E_SQRT);
     for (int base = 0; base < N; base += FCHUNK) {
-        // stage (same compaction as the volume render)
+        // stage the chunk's records that reach the brick (waves 0..3 scan, atom order kept)
         const int j = base + (int)threadIdx.x;
         bool hit = false;
         if (threadIdx.x < FCHUNK && j < N) {
@@ -145,11 +172,11 @@ __global__ __launch_bounds__(256) void ft_grid_kernel(const FtRec *__restrict__ 
                   r.lo[2] <= bhi[2] && r.hi[2] >= blo[2];
         }
         const unsigned long long msk = __ballot(hit);
-        if (lane == 0) wcnt[wave] = __popcll(msk);
+        if (wave < FCHUNK / 64 && lane == 0) wcnt[wave] = __popcll(msk);
         __syncthreads();
         int off = 0, total = 0;
 #pragma unroll
-        for (int w = 0; w < 4; w++) {
+        for (int w = 0; w < FCHUNK / 64; w++) {
             if (w < wave) off += wcnt[w];
             total += wcnt[w];
         }
@@ -160,7 +187,7 @@ __global__ __launch_bounds__(256) void ft_grid_kernel(const FtRec *__restrict__ 
             for (int k = 0; k < 8; k++) dst[k] = src[k];
         }
         __syncthreads();
-        for (int e = 0; e < total; e++) {
+        for (int e = grp; e < total; e += FT_GROUPS) {
             const FtRec &r = list[e];
             if (r.lo[0] > i0 || r.hi[0] < i0) continue;
             float err0 = 0.0f;
@@ -172,9 +199,9 @@ __global__ __launch_bounds__(256) void ft_grid_kernel(const FtRec *__restrict__ 
                     const float nq = fmaf(k1[q], k1[q], base2) * (r.u[0] * r.u[0]);
                     if (nq < cut) {
                         float err = err0;
-                        const float t = t02 + turns(r.xh[1], r.xl[1], k1[q], err) + err;
+                        const float tt = t02 + turns(r.xh[1], r.xl[1], k1[q], err) + err;
                         float sn, cs;
-                        ft_sincos_turns(t, sn, cs);
+                        ft_sincos_turns(tt, sn, cs);
                         const float a = r.amp * __builtin_amdgcn_exp2f(-nq);
                         ckahan(sr[q], cr[q], a * cs);
                         ckahan(si[q], ci[q], -a * sn);
@@ -192,9 +219,9 @@ __global__ __launch_bounds__(256) void ft_grid_kernel(const FtRec *__restrict__ 
                     const float nq = fmaf(v0, v0, fmaf(v1, v1, v2 * v2));
                     if (nq < cut) {
                         float err = err0;
-                        const float t = t02 + turns(r.xh[1], r.xl[1], k1[q], err) + err;
+                        const float tt = t02 + turns(r.xh[1], r.xl[1], k1[q], err) + err;
                         float sn, cs;
-                        ft_sincos_turns(t, sn, cs);
+                        ft_sincos_turns(tt, sn, cs);
                         const float a = r.amp * __builtin_amdgcn_exp2f(-nq);
                         ckahan(sr[q], cr[q], a * cs);
                         ckahan(si[q], ci[q], -a * sn);
@@ -204,10 +231,24 @@ __global__ __launch_bounds__(256) void ft_grid_kernel(const FtRec *__restrict__ 
         }
         __syncthreads();
     }
-    if (i0 < Kg.m0 && i2 < Kg.m2)
+    // groups' sums (value minus compensation) through LDS, added in group order
+    float2 *red = reinterpret_cast<float2 *>(list);  // [group][w2][q][lane]: 8 x 2 x 4 x 64 float2 = 32 KB
+    static_assert(sizeof(list) >= FT_GROUPS * FT_ROWS * 4 * 64 * sizeof(float2), "reduction buffer");
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-            if (i1b + q < Kg.m1) out[((size_t)i0 * Kg.m1 + i1b + q) * Kg.m2 + i2] = make_float2(sr[q], si[q]);
+    for (int q = 0; q < 4; q++) red[((grp * FT_ROWS + w2) * 4 + q) * 64 + lane] = make_float2(sr[q] - cr[q], si[q] - ci[q]);
+    __syncthreads();
+    if (grp == 0 && i0 < Kg.m0 && i2 < Kg.m2)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (i1b + q >= Kg.m1) continue;
+            float tr = 0.0f, ctr = 0.0f, ti = 0.0f, cti = 0.0f;
+            for (int g = 0; g < FT_GROUPS; g++) {
+                const float2 v = red[((g * FT_ROWS + w2) * 4 + q) * 64 + lane];
+                ckahan(tr, ctr, v.x);
+                ckahan(ti, cti, v.y);
+            }
+            out[((size_t)i0 * Kg.m1 + i1b + q) * Kg.m2 + i2] = make_float2(tr, ti);
+        }
 }
 
 __global__ void cfill_kernel(float2 *p, size_t n)
@@ -254,10 +295,16 @@ extern "C" int gt_ft_eval(const gt_atoms *atoms, const gt_kgrid *grid, float *ou
     }
     FtRec *recs = (FtRec *)ws;
     hipLaunchKernelGGL(ft_setup_kernel, dim3((atoms->n + 255) / 256), dim3(256), 0, st, *atoms, *grid, recs);
-    const long blocks = (long)((grid->m0 + 3) / 4) * ((grid->m1 + 3) / 4) * ((grid->m2 + 63) / 64);
-    GT_CHECK_ARG(blocks < (1L << 31), "gt_ft_eval: grid too large");
+    // bricks of 2 x 4 x 64 samples, numbered by Morton index over 2^e0 x 2^e1 x 2^e2 ranks
+    int e[3] = {0, 0, 0};
+    const int nb[3] = {(grid->m0 + FT_ROWS - 1) / FT_ROWS, (grid->m1 + 3) / 4, (grid->m2 + 63) / 64};
+    for (int d = 0; d < 3; d++)
+        while ((1 << e[d]) < nb[d]) e[d]++;
+    GT_CHECK_ARG(e[0] + e[1] + e[2] <= 30, "gt_ft_eval: grid too large");
+    const unsigned blocks = 1u << (e[0] + e[1] + e[2]);
     prof_begin(st, GT_K_FT);
-    hipLaunchKernelGGL(ft_grid_kernel, dim3((unsigned)blocks), dim3(256), 0, st, recs, atoms->n, *grid, (float2 *)out);
+    hipLaunchKernelGGL(ft_grid_kernel, dim3(blocks), dim3(FT_THREADS), 0, st, recs, atoms->n, *grid, (float2 *)out,
+                       e[0], e[1], e[2]);
     prof_end(st, GT_K_FT);
     GT_HIP(hipGetLastError());
     return 0;

@@ -53,10 +53,15 @@ def test_fourier_iso_vs_fp64(gt, oracle, cuda):
     nx = rng.uniform(-4, 4, (300, 3))
     nr = rng.uniform(2, 6, (300, 1))
     k = [np.linspace(-20, 20, 33), np.linspace(-10, 12, 17), np.linspace(-15, 15, 70)]
-    out = Fo.evaluate(Fo.FTAtoms(nI, nx, nr, cuda), k).cpu().numpy()
+    atoms = Fo.FTAtoms(nI, nx, nr, cuda)
+    first = Fo.evaluate(atoms, k)
+    out = first.cpu().numpy()
     mesh = np.stack(np.meshgrid(*[f32(v) for v in k], indexing="ij"), -1).reshape(-1, 3)
     ref = oracle.ft_points(f32(nI), f32(nx), f32(nr), mesh, iso=True).reshape(out.shape)
     assert np.abs(out - ref).max() < 1e-5 * np.abs(ref).max()
+    # grid sizes off the 2 x 4 x 64 bricks; the wave groups' sums are added in a fixed order
+    for _ in range(3):
+        assert torch.equal(Fo.evaluate(atoms, k), first)
 
 
 def test_fourier_vs_fft_of_render(gt, cuda, c4):
