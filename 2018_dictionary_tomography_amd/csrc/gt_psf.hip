@@ -826,22 +826,45 @@ template <int D> __device__ inline float psf_voxel(const PsfVolRec &R, const flo
     return R.amp * acc;
 }
 
-// block = brick of 4 x 8 x 8 voxels (3D) or 16 x 16 (2D), 256 threads
-template <int D>
-__global__ __launch_bounds__(256) void psf_volume_kernel(const PsfVolRec *__restrict__ recs, int N, gt_vol V,
-                                                         float *__restrict__ u)
+// Block = brick of 4 x 8 x 8 voxels (3D) or 16 x 16 (2D), thread = voxel,
+// PV_GROUPS groups of 256 threads on the same brick: group g takes every
+// PV_GROUPS-th staged record, and the groups' compensated sums are added in
+// group order at the end (deterministic).  Blocks are numbered bricks nearest
+// the volume centre first (Morton order over centre-out brick ranks): the
+// central bricks meet most atoms, and with several times the blocks that fit
+// at once the light outer bricks fill the tail.
+constexpr int PV_GROUPS = 4, PV_THREADS = PV_GROUPS * 256, PV_CHUNK = 256;
+
+__device__ inline int pv_centre_out(int t, int R)
 {
-    __shared__ PsfVolRec list[PSF_CHUNK];
-    __shared__ int wcnt[4];
+    const int c = (R - 1) >> 1;
+    return (t & 1) ? c + ((t + 1) >> 1) : c - (t >> 1);
+}
+
+template <int D>
+__global__ __launch_bounds__(PV_THREADS) void psf_volume_kernel(const PsfVolRec *__restrict__ recs, int N, gt_vol V,
+                                                                float *__restrict__ u, int e0, int e1, int e2)
+{
+    __shared__ PsfVolRec list[PV_CHUNK];
+    __shared__ int wcnt[PV_CHUNK / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int grp = threadIdx.x >> 8, t = threadIdx.x & 255;
     const int B0 = (D == 3) ? 4 : 16, B1 = (D == 3) ? 8 : 16, B2 = (D == 3) ? 8 : 1;
-    const int nb1 = (V.m1 + B1 - 1) / B1, nb2 = (V.m2 + B2 - 1) / B2;
-    int b = blockIdx.x;
-    const int bb2 = b % nb2;
-    b /= nb2;
-    const int bb1 = b % nb1;
-    const int bb0 = b / nb1;
-    const int t = threadIdx.x;
+    const int nb0 = (V.m0 + B0 - 1) / B0, nb1 = (V.m1 + B1 - 1) / B1, nb2 = (V.m2 + B2 - 1) / B2;
+    int tb[3] = {0, 0, 0};
+    {
+        const int e[3] = {e0, e1, e2};
+        unsigned m = blockIdx.x;
+        for (int bit = 0; m != 0u && bit < 31; bit++)
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+                if (bit < e[d]) {
+                    tb[d] |= (int)(m & 1u) << bit;
+                    m >>= 1;
+                }
+    }
+    if (tb[0] >= nb0 || tb[1] >= nb1 || tb[2] >= nb2) return;  // whole block
+    const int bb0 = pv_centre_out(tb[0], nb0), bb1 = pv_centre_out(tb[1], nb1), bb2 = pv_centre_out(tb[2], nb2);
     const int i0 = bb0 * B0 + t / (B1 * B2), i1 = bb1 * B1 + (t / B2) % B1, i2 = bb2 * B2 + t % B2;
     const int blo[3] = {bb0 * B0, bb1 * B1, bb2 * B2};
     const int bhi[3] = {min(blo[0] + B0 - 1, V.m0 - 1), min(blo[1] + B1 - 1, V.m1 - 1), min(blo[2] + B2 - 1, V.m2 - 1)};
@@ -849,26 +872,27 @@ __global__ __launch_bounds__(256) void psf_volume_kernel(const PsfVolRec *__rest
     const float yc[3] = {V.y0[min(i0, V.m0 - 1)], V.y1[min(i1, V.m1 - 1)], (D == 3) ? V.y2[min(i2, V.m2 - 1)] : 0.0f};
     const float hv[3] = {psf_cell_half(V.y0), psf_cell_half(V.y1), D == 3 ? psf_cell_half(V.y2) : 0.0f};
     float s = 0.0f, c = 0.0f;
-    for (int base = 0; base < N; base += PSF_CHUNK) {
-        const int j = base + t;
+    for (int base = 0; base < N; base += PV_CHUNK) {
+        // stage the chunk's records that meet the brick (waves 0..3 scan, atom order kept)
+        const int j = base + (int)threadIdx.x;
         bool hit = false;
-        if (t < PSF_CHUNK && j < N) {
+        if (threadIdx.x < PV_CHUNK && j < N) {
             const PsfVolRec &r = recs[j];
             hit = r.lo[0] <= bhi[0] && r.hi[0] >= blo[0] && r.lo[1] <= bhi[1] && r.hi[1] >= blo[1] &&
                   r.lo[2] <= bhi[2] && r.hi[2] >= blo[2];
         }
         const unsigned long long msk = __ballot(hit);
-        if (lane == 0) wcnt[wave] = __popcll(msk);
+        if (wave < PV_CHUNK / 64 && lane == 0) wcnt[wave] = __popcll(msk);
         __syncthreads();
         int off = 0, total = 0;
 #pragma unroll
-        for (int w = 0; w < 4; w++) {
+        for (int w = 0; w < PV_CHUNK / 64; w++) {
             if (w < wave) off += wcnt[w];
             total += wcnt[w];
         }
         if (hit) list[off + __popcll(msk & ((1ull << lane) - 1ull))] = recs[j];
         __syncthreads();
-        for (int e = 0; e < total; e++) {
+        for (int e = grp; e < total; e += PV_GROUPS) {
             const PsfVolRec &r = list[e];
             if (!valid || r.lo[0] > i0 || r.hi[0] < i0 || r.lo[1] > i1 || r.hi[1] < i1 || r.lo[2] > i2 ||
                 r.hi[2] < i2)
@@ -877,7 +901,16 @@ __global__ __launch_bounds__(256) void psf_volume_kernel(const PsfVolRec *__rest
         }
         __syncthreads();
     }
-    if (valid) u[((size_t)i0 * V.m1 + i1) * V.m2 + i2] = s;
+    // groups' sums (value minus compensation), added in group order
+    float *red = reinterpret_cast<float *>(list);
+    static_assert(sizeof(list) >= PV_THREADS * sizeof(float), "reduction buffer");
+    red[threadIdx.x] = s - c;
+    __syncthreads();
+    if (grp == 0 && valid) {
+        float ts = 0.0f, tc = 0.0f;
+        for (int g = 0; g < PV_GROUPS; g++) psf_kahan(ts, tc, red[g * 256 + t]);
+        u[((size_t)i0 * V.m1 + i1) * V.m2 + i2] = ts;
+    }
 }
 
 __global__ void psf_fill_kernel(float *__restrict__ p, size_t n)
@@ -1068,20 +1101,27 @@ extern "C" int gt_psf_volume(const gt_atoms *atoms, const gt_vol *vol, float *u,
     }
     PsfVolRec *recs = (PsfVolRec *)ws;
     const dim3 sg((unsigned)((atoms->n + 255) / 256));
-    long blocks;
+    // bricks numbered by Morton index over 2^e0 x 2^e1 x 2^e2 centre-out ranks
+    int nb[3];
     if (vol->dim == 3) {
         hipLaunchKernelGGL(psf_vol_setup_kernel<3>, sg, dim3(256), 0, st, *atoms, *vol, recs);
-        blocks = (long)((vol->m0 + 3) / 4) * ((vol->m1 + 7) / 8) * ((vol->m2 + 7) / 8);
+        nb[0] = (vol->m0 + 3) / 4, nb[1] = (vol->m1 + 7) / 8, nb[2] = (vol->m2 + 7) / 8;
     } else {
         hipLaunchKernelGGL(psf_vol_setup_kernel<2>, sg, dim3(256), 0, st, *atoms, *vol, recs);
-        blocks = (long)((vol->m0 + 15) / 16) * ((vol->m1 + 15) / 16);
+        nb[0] = (vol->m0 + 15) / 16, nb[1] = (vol->m1 + 15) / 16, nb[2] = 1;
     }
-    GT_CHECK_ARG(blocks < (1L << 31), "gt_psf_volume: grid too large");
+    int e[3] = {0, 0, 0};
+    for (int d = 0; d < 3; d++)
+        while ((1 << e[d]) < nb[d]) e[d]++;
+    GT_CHECK_ARG(e[0] + e[1] + e[2] <= 30, "gt_psf_volume: grid too large");
+    const unsigned blocks = 1u << (e[0] + e[1] + e[2]);
     prof_begin(st, GT_K_VOLUME);
     if (vol->dim == 3)
-        hipLaunchKernelGGL(psf_volume_kernel<3>, dim3((unsigned)blocks), dim3(256), 0, st, recs, atoms->n, *vol, u);
+        hipLaunchKernelGGL(psf_volume_kernel<3>, dim3(blocks), dim3(PV_THREADS), 0, st, recs, atoms->n, *vol, u, e[0],
+                           e[1], e[2]);
     else
-        hipLaunchKernelGGL(psf_volume_kernel<2>, dim3((unsigned)blocks), dim3(256), 0, st, recs, atoms->n, *vol, u);
+        hipLaunchKernelGGL(psf_volume_kernel<2>, dim3(blocks), dim3(PV_THREADS), 0, st, recs, atoms->n, *vol, u, e[0],
+                           e[1], e[2]);
     prof_end(st, GT_K_VOLUME);
     GT_HIP(hipGetLastError());
     return 0;

@@ -114,6 +114,28 @@ def test_psf_volume_vs_fp64(gt, oracle, cuda, psf, D):
     assert _rel(u, ref) < 1e-5
 
 
+def test_psf_volume_many_atoms_vs_fp64(gt, oracle, cuda, psf):
+    """300 atoms: more than one 256-record staging chunk per brick, each
+    brick's list shared by the block's thread groups; repeated calls give the
+    same bits (the groups' sums are added in a fixed order)."""
+    rng = np.random.default_rng(71)
+    n = 300
+    y = np.linspace(-1, 1, 12)
+    pix = y[1] - y[0]
+    scale = np.exp(rng.uniform(np.log(1 / (2 * pix)), np.log(1 / (0.5 * pix)), n)) / np.sqrt(60)
+    r = np.concatenate((scale[:, None] * rng.uniform(0.8, 1.25, (n, 3)), scale[:, None] * rng.normal(0, 0.2, (n, 3))),
+                       1)
+    I = rng.uniform(0.5, 1.5, n)
+    x = rng.uniform(-0.8, 0.8, (n, 3))
+    atoms = gt.AtomElement(gt.AtomSpace(3, isotropic=False), x, r, I, device=cuda)
+    vol = gt.VolSpace([y] * 3)
+    first = psf.volume(atoms, vol)
+    ref = oracle.psf_volproj(f32(I), f32(x), f32(r), [f32(y)] * 3, 10)
+    assert _rel(first.cpu().numpy(), ref) < 1e-5
+    for _ in range(2):
+        assert torch.equal(psf.volume(atoms, vol), first)
+
+
 def test_single_particle_operator_and_factory(gt, oracle, cuda, psf):
     """SingleParticleGaussTomo has GaussTomo's methods over the point-spread
     kernels; the module drop-ins write into host arrays; the factory builds
