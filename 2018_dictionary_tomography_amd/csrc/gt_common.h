@@ -36,6 +36,41 @@ void prof_end(hipStream_t st, int id);
 constexpr size_t ALIGN = 256;
 __host__ __device__ inline size_t align_up(size_t v) { return (v + ALIGN - 1) / ALIGN * ALIGN; }
 
+// Centre-first numbering of tile grids.  Tiles at the centre of a detector,
+// volume or k grid meet the most atoms; kernels whose grids hold more blocks
+// than fit at once number their blocks so that block b takes the tile whose
+// centre-out ranks (t0, t1, t2) have Morton index b (bits dealt to the axes in
+// turn while each has bits left; grid of 2^e0 x 2^e1 x 2^e2 ranks).  Ranks
+// beyond an axis' tile count are empty blocks.
+
+// t-th index of 0..R-1 from the centre outwards: c, c+1, c-1, c+2, ... with c = (R-1)/2
+__device__ inline int centre_out(int t, int R)
+{
+    const int c = (R - 1) >> 1;
+    return (t & 1) ? c + ((t + 1) >> 1) : c - (t >> 1);
+}
+
+// Morton index m -> ranks t[0..2] of a 2^e[0] x 2^e[1] x 2^e[2] grid
+__device__ inline void morton_ranks(unsigned m, const int *e, int *t)
+{
+    t[0] = t[1] = t[2] = 0;
+    for (int bit = 0; m != 0u && bit < 31; bit++)
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            if (bit < e[d]) {
+                t[d] |= (int)(m & 1u) << bit;
+                m >>= 1;
+            }
+}
+
+// smallest e with 2^e >= v (v >= 1)
+inline int ceil_log2(long v)
+{
+    int e = 0;
+    while ((1L << e) < v) e++;
+    return e;
+}
+
 // sin(2 pi t) and cos(2 pi t) of a phase t in turns, |t| <= 256: the
 // hardware sine and cosine, whose argument is in turns (quarter-rate, like
 // exp2); measured maximum absolute error 1.25e-7 for |t| <= 1.5 on gfx950.

@@ -120,13 +120,6 @@ __device__ inline float turns(float xh, float xl, float k, float &err)
 // 1 x 16, 2 x 8 and 4 x 4.)
 constexpr int FT_ROWS = 2, FT_GROUPS = 16 / FT_ROWS, FT_THREADS = 16 * 64, FCHUNK = 256;
 
-// t-th index of 0..R-1 from the centre outwards: c, c+1, c-1, c+2, ...
-__device__ inline int ft_centre_out(int t, int R)
-{
-    const int c = (R - 1) >> 1;
-    return (t & 1) ? c + ((t + 1) >> 1) : c - (t >> 1);
-}
-
 __global__ __launch_bounds__(FT_THREADS) void ft_grid_kernel(const FtRec *__restrict__ recs, int N, gt_kgrid Kg,
                                                              float2 *__restrict__ out, int e0, int e1, int e2)
 {
@@ -136,20 +129,11 @@ __global__ __launch_bounds__(FT_THREADS) void ft_grid_kernel(const FtRec *__rest
     const int grp = wave / FT_ROWS, w2 = wave % FT_ROWS;
     const int nb0 = (Kg.m0 + FT_ROWS - 1) / FT_ROWS, nb1 = (Kg.m1 + 3) / 4, nb2 = (Kg.m2 + 63) / 64;
     // Morton index -> centre-out ranks (t0, t1, t2); ranks beyond the brick counts are empty blocks
-    int t[3] = {0, 0, 0};
-    {
-        const int e[3] = {e0, e1, e2};
-        unsigned m = blockIdx.x;
-        for (int bit = 0; m != 0u && bit < 24; bit++)
-#pragma unroll
-            for (int d = 0; d < 3; d++)
-                if (bit < e[d]) {
-                    t[d] |= (int)(m & 1u) << bit;
-                    m >>= 1;
-                }
-    }
+    const int e[3] = {e0, e1, e2};
+    int t[3];
+    morton_ranks(blockIdx.x, e, t);
     if (t[0] >= nb0 || t[1] >= nb1 || t[2] >= nb2) return;  // whole block
-    const int bb0 = ft_centre_out(t[0], nb0), bb1 = ft_centre_out(t[1], nb1), bb2 = ft_centre_out(t[2], nb2);
+    const int bb0 = centre_out(t[0], nb0), bb1 = centre_out(t[1], nb1), bb2 = centre_out(t[2], nb2);
     const int i0 = bb0 * FT_ROWS + w2, i1b = bb1 * 4, i2 = bb2 * 64 + lane;
     const int blo[3] = {bb0 * FT_ROWS, i1b, bb2 * 64};
     const int bhi[3] = {min(bb0 * FT_ROWS + FT_ROWS - 1, Kg.m0 - 1), min(i1b + 3, Kg.m1 - 1), min(bb2 * 64 + 63, Kg.m2 - 1)};
@@ -296,10 +280,8 @@ extern "C" int gt_ft_eval(const gt_atoms *atoms, const gt_kgrid *grid, float *ou
     FtRec *recs = (FtRec *)ws;
     hipLaunchKernelGGL(ft_setup_kernel, dim3((atoms->n + 255) / 256), dim3(256), 0, st, *atoms, *grid, recs);
     // bricks of 2 x 4 x 64 samples, numbered by Morton index over 2^e0 x 2^e1 x 2^e2 ranks
-    int e[3] = {0, 0, 0};
-    const int nb[3] = {(grid->m0 + FT_ROWS - 1) / FT_ROWS, (grid->m1 + 3) / 4, (grid->m2 + 63) / 64};
-    for (int d = 0; d < 3; d++)
-        while ((1 << e[d]) < nb[d]) e[d]++;
+    const int e[3] = {ceil_log2((grid->m0 + FT_ROWS - 1) / FT_ROWS), ceil_log2((grid->m1 + 3) / 4),
+                      ceil_log2((grid->m2 + 63) / 64)};
     GT_CHECK_ARG(e[0] + e[1] + e[2] <= 30, "gt_ft_eval: grid too large");
     const unsigned blocks = 1u << (e[0] + e[1] + e[2]);
     prof_begin(st, GT_K_FT);

@@ -224,6 +224,9 @@ __device__ inline void kp_kahan(float &s, float &c, float v)
 constexpr int KP_CHUNK = 128;
 
 // One workgroup: 256 consecutive samples of one angle (flat index i0 * m1 + i1).
+// Block b takes angle b % nA and the tile of centre-out rank b / nA: the tiles
+// near k = 0 meet the most atoms and start first, the light outer ones fill
+// the tail.
 template <typename R>
 __global__ __launch_bounds__(256) void kp_eval_kernel(const KpRec<R> *__restrict__ recs, int N, KpDev<R> P,
                                                       double cut, R *__restrict__ out)
@@ -233,8 +236,8 @@ __global__ __launch_bounds__(256) void kp_eval_kernel(const KpRec<R> *__restrict
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const long plane = (long)P.m0 * P.m1;
     const long tiles = (plane + 255) / 256;
-    const int a = (int)(blockIdx.x / tiles);
-    const long base = (blockIdx.x - (long)a * tiles) * 256;
+    const int a = (int)(blockIdx.x % (unsigned)P.nA);
+    const long base = (long)centre_out((int)(blockIdx.x / (unsigned)P.nA), (int)tiles) * 256;
     const long last = min(base + 255, plane - 1);
     const long idx = base + threadIdx.x;
     const bool valid = idx < plane;

@@ -103,36 +103,9 @@ constexpr int F3_WAVES = 8, F3_CHUNK = F3_WAVES * 64;
 
 // The tiles at the detector centre carry several times the records of those
 // at its edges, and the grid holds more blocks than fit on the device at once:
-// blocks are numbered so that the heavy tiles start first and the light ones
-// fill the tail.  Block b takes angle b % nA and the tile of Morton index
-// b / nA over the centre-out ranks of the tile row and column (grid of
-// 2^e0 x 2^e1 ranks; ranks beyond the tile counts are empty blocks).
-
-// t-th index of 0..R-1 from the centre outwards: c, c+1, c-1, c+2, ...
-__device__ inline int centre_out(int t, int R)
-{
-    const int c = (R - 1) >> 1;
-    return (t & 1) ? c + ((t + 1) >> 1) : c - (t >> 1);
-}
-
-// Morton index m of a 2^e0 x 2^e1 grid -> (t0, t1); the bits beyond the
-// smaller side all go to the larger one.
-__device__ inline void morton_split(unsigned m, int e0, int e1, int &t0, int &t1)
-{
-    const int e = min(e0, e1);
-    unsigned u0 = 0u, u1 = 0u;
-    for (int i = 0; i < e; i++) {
-        u0 |= ((m >> (2 * i)) & 1u) << i;
-        u1 |= ((m >> (2 * i + 1)) & 1u) << i;
-    }
-    const unsigned rest = m >> (2 * e);
-    if (e0 > e)
-        u0 |= rest << e;
-    else
-        u1 |= rest << e;
-    t0 = (int)u0;
-    t1 = (int)u1;
-}
+// blocks are numbered centre tiles first (centre_out / morton_ranks), all
+// angles of one tile together (block b: angle b % nA, tile rank b / nA), so
+// the heavy tiles start first and the light ones fill the tail.
 
 __device__ inline int stage_rects(const PairRec *__restrict__ ra, int base, int N, int row0, int col0, int thi0,
                                   int thi1, PairRec *list, unsigned short (*wl)[F3_CHUNK], int (*cnt)[F3_WAVES],
@@ -229,10 +202,11 @@ __global__ __launch_bounds__(F3_WAVES * 64) void forward3_kernel(const PairRec *
     __shared__ int wlen[F3_WAVES];
     const int tilesR = (n0 + TILE_ROWS - 1) / TILE_ROWS, tilesC = (n1 + TILE_COLS - 1) / TILE_COLS;
     const int a = (int)(blockIdx.x % (unsigned)nA);
-    int t0, t1;
-    morton_split(blockIdx.x / (unsigned)nA, e0, e1, t0, t1);
-    if (t0 >= tilesR || t1 >= tilesC) return;  // whole block: no tile of this rank
-    const int row0 = centre_out(t0, tilesR) * TILE_ROWS, col0 = centre_out(t1, tilesC) * TILE_COLS;
+    const int e[3] = {e0, e1, 0};
+    int t[3];
+    morton_ranks(blockIdx.x / (unsigned)nA, e, t);
+    if (t[0] >= tilesR || t[1] >= tilesC) return;  // whole block: no tile of this rank
+    const int row0 = centre_out(t[0], tilesR) * TILE_ROWS, col0 = centre_out(t[1], tilesC) * TILE_COLS;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wr0 = row0 + 8 * (wave >> 2), wc0 = col0 + 16 * (wave & 3);
     const int i1 = wc0 + (lane & 15);
@@ -551,9 +525,7 @@ int launch_forward(const gt_atoms *A, const gt_geom *G, const PairRec *recs, flo
     }
     prof_begin(st, GT_K_FORWARD);
     if (G->dim == 3) {
-        int e0 = 0, e1 = 0;
-        while ((1 << e0) < (G->n0 + TILE_ROWS - 1) / TILE_ROWS) e0++;
-        while ((1 << e1) < (G->n1 + TILE_COLS - 1) / TILE_COLS) e1++;
+        const int e0 = ceil_log2((G->n0 + TILE_ROWS - 1) / TILE_ROWS), e1 = ceil_log2((G->n1 + TILE_COLS - 1) / TILE_COLS);
         const unsigned long long blocks = (unsigned long long)G->nA << (e0 + e1);
         GT_CHECK_ARG(blocks < (1ull << 31), "forward: too many detector tiles (%d x %d, %d angles)", G->n0, G->n1,
                      G->nA);

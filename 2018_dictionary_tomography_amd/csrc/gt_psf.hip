@@ -423,19 +423,28 @@ __device__ inline int psf_stage(const PsfRec *__restrict__ ra, int base, int N, 
     return total;
 }
 
-// Forward: block = (angle, 16 x 16 pixels) for 2D detectors, (angle, 256 pixels) for 1D ones.
+// Forward: block = (angle, 16 x 16 pixels) for 2D detectors, numbered centre
+// tiles first (block b: angle b % nA, tile of Morton rank b / nA over the
+// centre-out tile ranks, 2^e0 x 2^e1), or (angle = blockIdx.y, 256 pixels) for
+// 1D ones.
 template <int K>
-__global__ __launch_bounds__(256) void psf_forward_kernel(const PsfRec *__restrict__ recs, int N, int n0, int n1,
-                                                          const float *__restrict__ p0, const float *__restrict__ p1,
-                                                          float *__restrict__ R)
+__global__ __launch_bounds__(256) void psf_forward_kernel(const PsfRec *__restrict__ recs, int N, int nA, int n0,
+                                                          int n1, const float *__restrict__ p0,
+                                                          const float *__restrict__ p1, float *__restrict__ R, int e0,
+                                                          int e1)
 {
     __shared__ PsfRec list[PSF_CHUNK];
     __shared__ int wcnt[4];
-    const int a = blockIdx.y;
+    int a = blockIdx.y;
     int i0, i1, rlo, rhi, clo, chi;
     if (K == 2) {
-        const int tc = (n1 + 15) / 16;
-        const int tr = blockIdx.x / tc, tcc = blockIdx.x - tr * tc;
+        const int ntr = (n0 + 15) / 16, ntc = (n1 + 15) / 16;
+        a = (int)(blockIdx.x % (unsigned)nA);
+        const int e[3] = {e0, e1, 0};
+        int t[3];
+        morton_ranks(blockIdx.x / (unsigned)nA, e, t);
+        if (t[0] >= ntr || t[1] >= ntc) return;  // whole block: no tile of this rank
+        const int tr = centre_out(t[0], ntr), tcc = centre_out(t[1], ntc);
         i0 = tr * 16 + (int)(threadIdx.x >> 4);
         i1 = tcc * 16 + (int)(threadIdx.x & 15);
         rlo = tr * 16;
@@ -835,12 +844,6 @@ template <int D> __device__ inline float psf_voxel(const PsfVolRec &R, const flo
 // at once the light outer bricks fill the tail.
 constexpr int PV_GROUPS = 4, PV_THREADS = PV_GROUPS * 256, PV_CHUNK = 256;
 
-__device__ inline int pv_centre_out(int t, int R)
-{
-    const int c = (R - 1) >> 1;
-    return (t & 1) ? c + ((t + 1) >> 1) : c - (t >> 1);
-}
-
 template <int D>
 __global__ __launch_bounds__(PV_THREADS) void psf_volume_kernel(const PsfVolRec *__restrict__ recs, int N, gt_vol V,
                                                                 float *__restrict__ u, int e0, int e1, int e2)
@@ -851,20 +854,11 @@ __global__ __launch_bounds__(PV_THREADS) void psf_volume_kernel(const PsfVolRec 
     const int grp = threadIdx.x >> 8, t = threadIdx.x & 255;
     const int B0 = (D == 3) ? 4 : 16, B1 = (D == 3) ? 8 : 16, B2 = (D == 3) ? 8 : 1;
     const int nb0 = (V.m0 + B0 - 1) / B0, nb1 = (V.m1 + B1 - 1) / B1, nb2 = (V.m2 + B2 - 1) / B2;
-    int tb[3] = {0, 0, 0};
-    {
-        const int e[3] = {e0, e1, e2};
-        unsigned m = blockIdx.x;
-        for (int bit = 0; m != 0u && bit < 31; bit++)
-#pragma unroll
-            for (int d = 0; d < 3; d++)
-                if (bit < e[d]) {
-                    tb[d] |= (int)(m & 1u) << bit;
-                    m >>= 1;
-                }
-    }
+    const int e[3] = {e0, e1, e2};
+    int tb[3];
+    morton_ranks(blockIdx.x, e, tb);
     if (tb[0] >= nb0 || tb[1] >= nb1 || tb[2] >= nb2) return;  // whole block
-    const int bb0 = pv_centre_out(tb[0], nb0), bb1 = pv_centre_out(tb[1], nb1), bb2 = pv_centre_out(tb[2], nb2);
+    const int bb0 = centre_out(tb[0], nb0), bb1 = centre_out(tb[1], nb1), bb2 = centre_out(tb[2], nb2);
     const int i0 = bb0 * B0 + t / (B1 * B2), i1 = bb1 * B1 + (t / B2) % B1, i2 = bb2 * B2 + t % B2;
     const int blo[3] = {bb0 * B0, bb1 * B1, bb2 * B2};
     const int bhi[3] = {min(blo[0] + B0 - 1, V.m0 - 1), min(blo[1] + B1 - 1, V.m1 - 1), min(blo[2] + B2 - 1, V.m2 - 1)};
@@ -979,13 +973,15 @@ extern "C" int gt_psf_forward(const gt_atoms *atoms, const gt_geom *geom, float 
     const PsfRec *recs = (const PsfRec *)ws;
     prof_begin(st, GT_K_FORWARD);
     if (atoms->dim == 3) {
-        const dim3 grid((unsigned)(((geom->n0 + 15) / 16) * ((n1 + 15) / 16)), (unsigned)geom->nA);
-        hipLaunchKernelGGL(psf_forward_kernel<2>, grid, dim3(256), 0, st, recs, atoms->n, geom->n0, n1, geom->p0,
-                           geom->p1, R);
+        const int e0 = ceil_log2((geom->n0 + 15) / 16), e1 = ceil_log2((n1 + 15) / 16);
+        GT_CHECK_ARG(((unsigned long long)geom->nA << (e0 + e1)) < (1ull << 31), "gt_psf_forward: too many tiles");
+        const dim3 grid((unsigned)((unsigned long long)geom->nA << (e0 + e1)));
+        hipLaunchKernelGGL(psf_forward_kernel<2>, grid, dim3(256), 0, st, recs, atoms->n, geom->nA, geom->n0, n1,
+                           geom->p0, geom->p1, R, e0, e1);
     } else {
         const dim3 grid((unsigned)((geom->n0 + 255) / 256), (unsigned)geom->nA);
-        hipLaunchKernelGGL(psf_forward_kernel<1>, grid, dim3(256), 0, st, recs, atoms->n, geom->n0, 1, geom->p0,
-                           geom->p0, R);
+        hipLaunchKernelGGL(psf_forward_kernel<1>, grid, dim3(256), 0, st, recs, atoms->n, geom->nA, geom->n0, 1,
+                           geom->p0, geom->p0, R, 0, 0);
     }
     prof_end(st, GT_K_FORWARD);
     GT_HIP(hipGetLastError());
@@ -1110,9 +1106,7 @@ extern "C" int gt_psf_volume(const gt_atoms *atoms, const gt_vol *vol, float *u,
         hipLaunchKernelGGL(psf_vol_setup_kernel<2>, sg, dim3(256), 0, st, *atoms, *vol, recs);
         nb[0] = (vol->m0 + 15) / 16, nb[1] = (vol->m1 + 15) / 16, nb[2] = 1;
     }
-    int e[3] = {0, 0, 0};
-    for (int d = 0; d < 3; d++)
-        while ((1 << e[d]) < nb[d]) e[d]++;
+    const int e[3] = {ceil_log2(nb[0]), ceil_log2(nb[1]), ceil_log2(nb[2])};
     GT_CHECK_ARG(e[0] + e[1] + e[2] <= 30, "gt_psf_volume: grid too large");
     const unsigned blocks = 1u << (e[0] + e[1] + e[2]);
     prof_begin(st, GT_K_VOLUME);
