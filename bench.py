@@ -220,9 +220,25 @@ def _timed(torch, fn, reps):
     return (time.perf_counter() - t0) * 1e3 / reps
 
 
+def _kernel_ms(torch, kernel_id, fn, reps):
+    """Average duration (ms) of the launches of one library kernel id over
+    reps calls of fn, from the library's HIP-event hooks."""
+    L = importlib.import_module("2018_dictionary_tomography_amd._lib")
+    torch.cuda.synchronize()
+    L.profile_enable(True)
+    L.profile_read(kernel_id)
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    L.profile_enable(False)
+    tot, cnt = L.profile_read(kernel_id)
+    return tot / max(cnt, 1)
+
+
 def extras_leg(torch, gtp, dev, reps):
     """Rows the reference times nowhere, on the 3D workload's geometry (61
-    angles +-60 deg about y, 128x128): the point-spread projector (forward
+    angles +-60 deg about y, 128x128): the volume render of 1000 atoms drawn
+    as the 3D workload on 128^3, the point-spread projector (forward
     and L2 gradient + Hessian blocks of 1000 atoms drawn as the 3D
     workload), the Fourier-slice planes of 2000 FT atoms and their complex
     L2 derivatives, and the KL initialiser's sinogram moments on a 90 x
@@ -245,6 +261,14 @@ def extras_leg(torch, gtp, dev, reps):
     vol = gtp.VolSpace(box=(-1, 1), shape=64, dim=3)
     u = psf.volume(atoms, vol)
     ms_v = _timed(torch, lambda: psf.volume(atoms, vol, out=u), reps)
+    vol128 = gtp.VolSpace(box=(-1, 1), shape=128, dim=3)
+    u128 = gtp.ops.volume(atoms, vol128)
+    ms_pv = _timed(torch, lambda: gtp.ops.volume(atoms, vol128, out=u128), reps)
+    L = importlib.import_module("2018_dictionary_tomography_amd._lib")
+    k_pv = _kernel_ms(torch, L.GT_K_VOLUME, lambda: gtp.ops.volume(atoms, vol128, out=u128), reps)
+    out["volume"] = {"workload": "VolProj of the same 1000 atoms on 128^3 voxel centres",
+                     "ms": ms_pv, "kernel_ms": k_pv, "note": "ms: wall clock per call (host-bound); kernel_ms: "
+                     "the render kernel alone (HIP events)"}
     out["psf"] = {"workload": "1000 atoms, 61 x 128^2, pixel-integrated; voxel-integrated render on 64^3",
                   "forward_ms": ms_f, "derivs_L2_ms": ms_d, "volume_ms": ms_v,
                   "forward_atom_pixel_per_s": 1000 * rays / (ms_f * 1e-3),

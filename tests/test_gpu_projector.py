@@ -236,6 +236,20 @@ def test_volume_3d_2d(gt, oracle, cuda):
     assert relerr(u2, u264) < TOL
 
 
+def test_volume_3d_many_atoms(gt, oracle, cuda):
+    """300 atoms (several 128-record staging chunks per brick) on a volume
+    whose sizes are off the 4 x 4 x 64 voxel bricks; repeated calls give the
+    same bits."""
+    I, x, r, _, _ = _c2_case(gt, 300, 3, 16, seed=33)
+    vol = gt.VolSpace(box=(-1, 1), shape=(31, 21, 70), dim=3)
+    atoms = _atoms(gt, I, x, r, 3, cuda)
+    first = gt.ops.volume(atoms, vol)
+    u64 = oracle.volproj(f32(I), f32(x), f32(r), [f32(c) for c in vol.coord_vectors])
+    assert relerr(first.cpu().numpy(), u64) < TOL
+    for _ in range(2):
+        assert torch.equal(gt.ops.volume(atoms, vol), first)
+
+
 def test_dropin_module_functions(gt, oracle, cuda):
     """RadProj / L2derivs_RadProj / derivs_RadProj / VolProj with the reference conventions."""
     I, x, r, space, _ = _c2_case(gt, 5, 7, 40, seed=50)
