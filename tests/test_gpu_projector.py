@@ -68,9 +68,9 @@ def test_forward_2p5d_and_nonsquare(gt, oracle, cuda):
 @pytest.mark.parametrize("N,shape", [(1100, (37, 53)), (300, (5, 100)), (300, (130, 7)), (65, (1, 1)),
                                      (513, (64, 200))])
 def test_forward_3d_tile_order_shapes(gt, oracle, cuda, N, shape):
-    """Detectors that are not multiples of the 16 x 64 pixel tiles or their
+    """Detectors that are not multiples of the 8 x 64 pixel tiles or their
     8 x 16 wave rectangles, tile grids that are not powers of two (empty
-    blocks in the centre-first block order), several 512-record staging chunks
+    blocks in the centre-first block order), several 256-record staging chunks
     per tile, a one-pixel detector; repeated calls give the same bits."""
     W = __import__("importlib").import_module("2018_dictionary_tomography_amd.workloads")
     rng = np.random.default_rng(N + shape[0])
