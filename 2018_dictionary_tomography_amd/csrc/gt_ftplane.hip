@@ -434,7 +434,10 @@ __device__ inline float kd_sum(float v)
 
 // One workgroup per atom; wave w takes angles w, w + 16, ...  csum[a] =
 // sum over angle a of |C|^2 (FP64).
-__global__ __launch_bounds__(KD_THREADS) void kd_kernel(gt_atoms A, KpDev<float> P, const float2 *__restrict__ C,
+// At most 128 VGPRs (amdgpu_waves_per_eu(4): two blocks of 8 waves per CU):
+// 2.3 ms against 2.7 ms at the 207 VGPRs of one block per CU, despite 260
+// bytes of spills per lane (the row walk is latency-bound).
+__global__ __launch_bounds__(KD_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void kd_kernel(gt_atoms A, KpDev<float> P, const float2 *__restrict__ C,
                                                         const double *__restrict__ csum, double *__restrict__ fo,
                                                         double *__restrict__ go, double *__restrict__ Ho)
 {
