@@ -814,10 +814,13 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                     // rounded to the FP32 atom storage
                     for (int i = 0; i < 1 + D + NRs; i++) t[i] = (double)(float)t[i];
                     if (S.block) {
-                        // norm(old - T) > tol * norm(old) over the block (NewtonGaussian.py:214)
+                        // norm(old - T) > tol * norm(old) over the block (NewtonGaussian.py:214),
+                        // T the float32 atom entry: I and x before their projection
+                        // (:170-171, :178-180), r after it (:188-199)
                         double dn = 0.0, on = 0.0;
                         for (int i = o; i < o + n; i++) {
-                            const double d = L.th_new[i] - L.th_old[i];
+                            const double Ti = L.c.blk < 2 ? (double)(float)(L.th_old[i] + b[i - o]) : L.th_new[i];
+                            const double d = Ti - L.th_old[i];
                             dn += d * d;
                             on += L.th_old[i] * L.th_old[i];
                         }

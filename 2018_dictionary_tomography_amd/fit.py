@@ -413,7 +413,9 @@ def newton_host(recon, space, data, max_iter, kernels=POINT, reg=None, tol=1e-4,
                         R = Rnew
                         E.append(En)
                         if block:
-                            moved = np.linalg.norm((new - old)[o:o + m]) > tol * np.linalg.norm(old[o:o + m])
+                            # T of NewtonGaussian.py:170-199: I and x unprojected, r projected
+                            T = np.float32(old[o:o + m] + d) if blk < 2 else new[o:o + m]
+                            moved = np.linalg.norm(old[o:o + m] - T) > tol * np.linalg.norm(old[o:o + m])
                         else:
                             moved = np.sqrt(np.sum(dx * dx)) > tol
                         if moved:
@@ -576,10 +578,17 @@ def newton_linesearch_block(recon, data, max_iter, fidelity=None, reg=None, Rado
 # ---------------------------------------------------------------------------
 def gd_linesearch_block(recon, data, max_iter, space, dim="xrI", tol=1e-4, min_iter=1, seed=1, reg=None,
                         return_F=False, kernels=POINT, rs=None, h=None):
-    """Block GD over (x, r, I) of each atom with the reference's step-size rules.
-    Energies (fidelity + regulariser) in FP64.  Divergence from the reference,
-    kept on purpose: on a rejected step the sinogram of the reverted atoms is
-    used again (GD_lib.py:157-160 leaves the rejected sinogram in place).
+    """Block GD over (x, r, I) of each atom with the reference's step-size rules
+    (GD_lib.py:96-166: h from 1e-2, h = min(h, 1/max(|g|, 1e-8)), x1.1 on
+    accept, /2 on reject, floor 1e-8).  Energies (fidelity + regulariser) in
+    FP64.  Divergences from the reference, each a repair:
+    * each block steps along its own gradient, T -= h * g_block.  The
+      reference selects the gradient by the block's position (df[0],
+      df[1:1+D], df[1+D:] for t = 0, 1, 2) while its blocks are ordered
+      x, r, I, and moves the whole block by the scalar h * grad[0]
+      (GD_lib.py:138-151): for dim='xrI' its x moves along (1, .., 1) dF/dI;
+    * on a rejected step the sinogram of the reverted atoms is used again
+      (GD_lib.py:157-160 leaves the rejected sinogram in place).
     Returns (recon, E), or (recon, E, F) with F the fidelity of each proposal."""
     reg = as_regulariser(reg)
     dev = recon.I.device
@@ -651,7 +660,9 @@ def linesearch_block(recon, data, max_iter, fidelity=None, reg=None, Radon=None,
                      tol=1e-4, min_iter=1, **kwargs):
     """Drop-in for GD_lib.linesearch_block (GD_lib.py:82-178): fits `recon`
     in place, returns (recon, E, F); operators and guess as
-    newton_linesearch."""
+    newton_linesearch.  Each block steps along its own gradient (the
+    reference takes the gradient by block position and a scalar step,
+    GD_lib.py:138-151; see gd_linesearch_block)."""
     _check_fidelity(fidelity)
     _check_guess(guess, Radon)
     space = Radon.range if Radon is not None else kwargs["space"]

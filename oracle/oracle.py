@@ -397,7 +397,12 @@ def _projector(kind, n_sub):
     pixel-integrated one (SingleParticleGaussTomo, n_sub nodes per pixel
     and axis), or of the Fourier-slice planes (GaussFTVolume: 'geometry' is
     the sample tuple (k0, k1, w0, w1), values cut at |L k|^2 < 30 and L2
-    derivatives at 60, FT-space atoms)."""
+    derivatives at 60, FT-space atoms).  A pair of callables (radproj(I, x,
+    r, geom), derivs(I, x, r, geom, C, mode, order)) is used as given: the
+    fitters call nothing else, as the reference's fitters only call
+    Radon(.) and Radon.L2_derivs."""
+    if isinstance(kind, tuple):
+        return kind
     if kind == "point":
         return radproj, derivs
     if kind == "psf":
@@ -415,16 +420,40 @@ def _projector(kind, n_sub):
     raise ValueError(kind)
 
 
+def _atom_dim(projector, geom):
+    """Spatial dimension of the atoms: 3 for Fourier-space atoms, else the
+    geometry's (a custom projector's geometry only needs a `dim`)."""
+    return 3 if isinstance(projector, str) and projector == "ft" else geom.dim
+
+
 def gd_linesearch_block(I, x, r, geom, data, max_iter, dim="xrI", tol=1e-4, min_iter=1, seed=1, projector="point",
-                        n_sub=12, guess=None, return_F=False):
+                        n_sub=12, guess=None, return_F=False, trace=None):
     """Block gradient descent (GD_lib.py:82-178), null regulariser, FP64.
 
-    Divergence from the reference, kept deliberately: on reject the sinogram is
-    recomputed from the reverted atoms (GD_lib.py:157-160 leaves the rejected
-    sinogram in place).  guess: the greedy insertion path as in newton_fit
-    (GD_lib.py:113-126).  Returns (I, x, r, E) or, with return_F, also F
-    (the fidelity of every proposal)."""
-    D = 3 if projector == "ft" else geom.dim
+    Per atom j of each outer iteration's permutation and per block t of dim
+    (filtered to the order x, r, I, :84-92): gradient of the block, step size
+    h[t][j] = min(h[t][j], 1/max(|grad|, eps)) with eps = 1e-8 and h starting
+    at 1e-2 (:96-99, :149-150), T -= h grad, full energy of the proposal;
+    reject (E rises): revert, h /= 2; accept: h *= 1.1 and the sweep is not
+    converged if |old - T| > tol |old|; then h = max(h, eps) (:157-166).  No
+    projections.  Exit after an outer iteration with no move once its index
+    exceeds min_iter (:171).
+
+    Divergences from the reference, each a repair:
+    * block t steps along that block's own gradient: I along dF/dI, x along
+      dF/dx, r along dF/dr.  The reference picks the gradient by the block's
+      position (grad = df[0], df[1:1+D], df[1+D:] for t = 0, 1, 2) while its
+      blocks are ordered x, r, I, and then moves every entry of the block by
+      the scalar h*grad[0] (:138-151): for dim='xrI' x moves along
+      (1, .., 1) dF/dI, r along (1, .., 1) dF/dx0, I by dF/dr0;
+    * on reject the sinogram is recomputed from the reverted atoms
+      (:157-160 leave the rejected sinogram in place).
+    guess: the greedy insertion path as in newton_fit (GD_lib.py:113-126).
+    Returns (I, x, r, E) or, with return_F, also F (the fidelity of every
+    proposal).  trace (a list): one dict per block step appended, with the
+    atom j, block name, step size after the update, accept flag and the two
+    BREAK flags after it."""
+    D = _atom_dim(projector, geom)
     I, x = np.array(I, float).reshape(-1), np.array(x, float).reshape(-1, D)
     r = np.array(r, float).reshape(I.size, -1)
     n_all = I.size
@@ -486,7 +515,8 @@ def gd_linesearch_block(I, x, r, geom, data, max_iter, dim="xrI", tol=1e-4, min_
                         Rn = radproj(I[:n], x[:n], r[:n], geom)
                         En = l2_squared_loss(Rn, data)
                         F.append(En)
-                        if En > E[-1]:
+                        acc = not En > E[-1]
+                        if not acc:
                             A[j] = old
                             E.append(E[-1])
                             h[t][j] /= 2
@@ -497,6 +527,9 @@ def gd_linesearch_block(I, x, r, geom, data, max_iter, dim="xrI", tol=1e-4, min_
                             if np.sqrt(np.sum((old - new) ** 2)) > tol * np.sqrt(np.sum(old * old)):
                                 brk = [False, False]
                         h[t][j] = max(h[t][j], eps)
+                        if trace is not None:
+                            trace.append({"j": int(j), "block": name, "h": h[t][j], "accept": acc,
+                                          "brk": tuple(brk)})
                     if brk[1]:
                         break
             if brk[0] and it > min_iter:
@@ -619,11 +652,24 @@ def _project_block(D, iso, t, th_old, d):
 
 
 def newton_fit(I, x, r, geom, data, max_iter, tol=1e-4, min_iter=1, seed=1, reg=None, block=False,
-               projector="point", n_sub=12, guess=None):
+               projector="point", n_sub=12, guess=None, trace=None):
     """NewtonGaussian.linesearch (block=False, :18-109, with GD_lib._step) or
     NewtonGaussian.linesearch_block (block=True, :112-237) with regulariser
     `reg` (a Reg, None = null), FP64.  Energies E = F + reg(all atoms),
     recomputed in full after every update as the reference does.
+
+    Control flow (plain): h starts at 1 (:25); per update dx = solve(H + I/h,
+    -g) (:68-74), the _step projections, then reject (E rises): if h > 2 eps
+    the sweep is not converged, h /= 10; accept: if |dx| > tol the sweep is
+    not converged, h *= 2; then h is clamped to [eps, 1/eps], eps = 1e-4
+    (:81-92).  An inner loop of max_iter[1] updates per atom ends early when
+    its last update left BREAK[1] set (:95-96); the fit ends after an outer
+    iteration that left BREAK[0] set once its 0-based index exceeds min_iter
+    (:97-98).  Block variant: blocks I, x, r in turn with h[t][j], x1.3 on
+    accept (:205-217); the I block is skipped (no energy entry) unless
+    ddf[0,0] > 1e-8 after damping, x and r when their solve fails (:165-187);
+    the convergence test is |old - T| > tol |old| over the block (:214),
+    where T is the unprojected I and x and the projected r (:170-199).
 
     The block variant restates the intent: its blocks are sliced with the
     spatial dimension (the reference slices with the string 'Ixr', :135,163,172,
@@ -640,9 +686,13 @@ def newton_fit(I, x, r, geom, data, max_iter, tol=1e-4, min_iter=1, seed=1, reg=
     update's proposal (also when rejected, as the reference records it).
     projector: 'point' (GaussTomo) or 'psf' (SingleParticleGaussTomo; the
     reference's Newton only calls Radon(.) and Radon.L2_derivs) or 'ft'
-    (GaussFTVolume; geom is then the sample tuple (k0, k1, w0, w1))."""
+    (GaussFTVolume; geom is then the sample tuple (k0, k1, w0, w1)), or a
+    (radproj, derivs) pair of callables.  trace (a list): one dict per
+    update appended (atom j, block t, step size after the update, accept
+    flag, the two BREAK flags after it), and {"j", "t", "skip": True} for a
+    skipped block."""
     radproj, derivs = _projector(projector, n_sub)
-    D = 3 if projector == "ft" else geom.dim
+    D = _atom_dim(projector, geom)
     I, x = np.array(I, float).reshape(-1), np.array(x, float).reshape(-1, D)
     r = np.array(r, float).reshape(I.size, -1)
     iso = r.shape[1] == 1
@@ -712,20 +762,27 @@ def newton_fit(I, x, r, geom, data, max_iter, tol=1e-4, min_iter=1, seed=1, reg=
                             o, m = (0, 1) if t == 0 else ((1, D) if t == 1 else (1 + D, old.size - 1 - D))
                             if t == 0:
                                 if not ddf[0, 0] > 1e-8:
+                                    if trace is not None:
+                                        trace.append({"j": int(j), "t": t, "skip": True})
                                     break
                                 d = np.array([-df[0] / ddf[0, 0]])
                             else:
                                 try:
                                     d = np.linalg.solve(ddf[o:o + m, o:o + m], -df[o:o + m])
                                 except np.linalg.LinAlgError:
+                                    if trace is not None:
+                                        trace.append({"j": int(j), "t": t, "skip": True})
                                     break
                             new = _project_block(D, iso, t, old, d)
+                            # T of :170-199: the I and x blocks unprojected, r projected
+                            moved_to = old[o:o + m] + d if t < 2 else new[o:o + m]
                         put(j, new)
                         Rnew = R + radproj(I[j:j + 1], x[j:j + 1], r[j:j + 1], geom) - Rj
                         Fn = l2_squared_loss(Rnew, data)
                         En = Fn + total_reg(n)
                         F.append(Fn)
-                        if En > E[-1]:
+                        acc = not En > E[-1]
+                        if not acc:
                             put(j, old)
                             E.append(E[-1])
                             if h[t][j] > 2 * eps:
@@ -735,13 +792,15 @@ def newton_fit(I, x, r, geom, data, max_iter, tol=1e-4, min_iter=1, seed=1, reg=
                             R = Rnew
                             E.append(En)
                             if block:
-                                moved = np.linalg.norm((new - old)[o:o + m]) > tol * np.linalg.norm(old[o:o + m])
+                                moved = np.linalg.norm(old[o:o + m] - moved_to) > tol * np.linalg.norm(old[o:o + m])
                             else:
                                 moved = np.sqrt(np.sum(dx * dx)) > tol
                             if moved:
                                 brk = [False, False]
                             h[t][j] *= 1.3 if block else 2
                         h[t][j] = min(max(h[t][j], eps), 1 / eps)
+                        if trace is not None:
+                            trace.append({"j": int(j), "t": t, "h": h[t][j], "accept": acc, "brk": tuple(brk)})
                         if brk[1]:
                             break
             if brk[0] and it > min_iter:

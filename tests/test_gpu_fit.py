@@ -32,6 +32,16 @@ def _oracle_geom(oracle, space):
                            [f32(p) for p in space.detector])
 
 
+def final_steps(trace, n, blocks):
+    """Step sizes after an oracle fit, from its trace: h[t][j] of the last
+    update of block t of atom j (1 for an atom never updated)."""
+    h = np.ones((len(blocks), n))
+    for t in trace:
+        if not t.get("skip"):
+            h[blocks.index(t.get("t", t.get("block"))), t["j"]] = t["h"]
+    return h
+
+
 def test_newton_3d_matches_oracle(gt, oracle, cuda):
     F = importlib.import_module("2018_dictionary_tomography_amd.fit")
     I, x, r, space, rng = _setup3(gt, 6, 11, 40, seed=5)
@@ -53,6 +63,11 @@ def test_newton_3d_matches_oracle(gt, oracle, cuda):
     assert np.all(np.diff(Eg) <= 0)
     assert np.abs(Eg - Eo).max() < 1e-5 * Eo[0]
     assert np.abs(req.x.cpu().numpy() - xo).max() < 1e-4
+    # the same accept/reject decisions: equal update counts and step sizes
+    tr = []
+    oracle.newton_fit(f32(I0), f32(x0), f32(r), og, data.double().cpu().numpy(), [4, 1], tol=1e-6, trace=tr)
+    assert len(tr) == s.jj
+    assert np.array_equal(req.h.cpu().numpy(), final_steps(tr, 6, [0])[0])
     # energy bookkeeping: the tracked E equals 1/2||resid||^2 of the stored residual
     r64 = req.resid.double()
     assert abs(float(0.5 * (r64 * r64).sum()) - s.E) < 1e-9 * s.E
@@ -153,19 +168,30 @@ def test_grouped_launch_rejects_oversubscription(gt, cuda):
 
 
 def test_gd_block_2d_matches_oracle(gt, oracle, cuda):
+    """GD_lib.linesearch_block at the 2D workload's size: 50 isotropic atoms
+    (centres U[-0.8, 0.8]^2, radii U[0.02, 0.08], intensities U[0.5, 1.5]),
+    180 angles over [0, pi), 128-pixel detector, data = forward + 1% noise,
+    start = centres + N(0, 0.02); 3 outer iterations (450 block steps).
+    Energies entry-wise within 1e-5 of E[0], the same step sizes."""
     F = importlib.import_module("2018_dictionary_tomography_amd.fit")
     W = importlib.import_module("2018_dictionary_tomography_amd.workloads")
     rng = np.random.default_rng(21)
-    I, x, r = W.draw_iso2(4, rng, rho=(0.04, 0.08))
-    T, ortho, det = W.fan_2d(36, 64)
+    I, x, r = W.draw_iso2(50, rng)
+    T, ortho, det = W.fan_2d(180, 128)
     space = gt.ProjSpace(orientations=T, ortho=ortho, detector=det)
     aspace = gt.AtomSpace(2, isotropic=True)
     data = gt.ops.forward(gt.AtomElement(aspace, x, r, I, device=cuda), space)
+    data = data + torch.as_tensor(rng.normal(0, 0.01 * float(data.max()), data.shape), dtype=torch.float32,
+                                  device=cuda)
     x0 = x + rng.normal(0, 0.02, x.shape)
     start = gt.AtomElement(aspace, x0, r, I, device=cuda)
-    _, Eg = F.gd_linesearch_block(start, data, [3, 1], space)
+    h = np.full((3, 50), 1e-2)
+    _, Eg = F.gd_linesearch_block(start, data, [3, 1], space, h=h)
+    tr = []
     _, _, _, Eo = oracle.gd_linesearch_block(f32(I), f32(x0), f32(r), _oracle_geom(oracle, space),
-                                             data.double().cpu().numpy(), [3, 1])
+                                             data.double().cpu().numpy(), [3, 1], trace=tr)
     assert np.all(np.diff(Eg) <= 0)
-    n = min(len(Eg), len(Eo))
-    assert np.abs(Eg[:n] - Eo[:n]).max() < 1e-4 * Eo[0]
+    assert len(Eg) == len(Eo) == 1 + 3 * 50 * 3
+    assert np.abs(Eg - Eo).max() < 1e-5 * Eo[0]
+    ho = final_steps(tr, 50, list("xrI"))
+    assert np.allclose(h, ho, rtol=1e-12)

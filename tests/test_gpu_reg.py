@@ -141,9 +141,17 @@ def test_newton_block_matches_oracle(gt, oracle, cuda, iso):
     req, s = F.newton_fit(start, space, data, [3, 1], tol=1e-6, block=True)
     assert s.done == 1
     Eg = req.Ehist[: s.jj + 1].cpu().numpy()
+    tr = []
     Io, xo, ro, Eo, Fo = oracle.newton_fit(f32(I * 0.9), f32(x0), f32(r0), _ogeom(oracle, space),
-                                           data.double().cpu().numpy(), [3, 1], tol=1e-6, block=True)
+                                           data.double().cpu().numpy(), [3, 1], tol=1e-6, block=True, trace=tr)
     assert Eg.size == Eo.size
+    # the same decisions: every block's step size after the fit
+    n = I.size
+    ho = np.ones((3, n))
+    for t in tr:
+        if not t.get("skip"):
+            ho[t["t"], t["j"]] = t["h"]
+    assert np.allclose(req.h.cpu().numpy().reshape(3, n), ho, rtol=1e-12)
     assert np.all(np.diff(Eg) <= 0)
     assert np.abs(Eg - Eo).max() < 1e-5 * Eo[0]
     assert np.abs(req.x.cpu().numpy() - xo).max() < 1e-4
