@@ -74,6 +74,15 @@ inline int ceil_log2(long v)
 // sin(2 pi t) and cos(2 pi t) of a phase t in turns, |t| <= 256: the
 // hardware sine and cosine, whose argument is in turns (quarter-rate, like
 // exp2); measured maximum absolute error 1.25e-7 for |t| <= 1.5 on gfx950.
+// LDS written by some lanes of a wave, then read by others: order the wave's
+// LDS traffic (the lanes of a wave execute together).
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ inline void ft_sincos_turns(float t, float &sn, float &cs)
 {
     sn = __builtin_amdgcn_sinf(t);

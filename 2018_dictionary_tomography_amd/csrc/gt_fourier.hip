@@ -10,8 +10,9 @@
 // pair (xh + xl, ~48 bits) and the kernel forms the phase in turns with an
 // exact FMA product xh * k = p + e, reduces p exactly by rint, and evaluates
 // the hardware sine and cosine (argument in turns) on the small remainder.
-// Per-point complex sums are Kahan sums.
+// Per-point complex sums are taken in FP64.
 #include "gt_common.h"
+#include "gt_ftpatch.h"
 
 namespace gt {
 
@@ -88,15 +89,10 @@ __global__ __launch_bounds__(256) void ft_setup_kernel(gt_atoms A, gt_kgrid Kg, 
             rec.hi[c] = upper_idx(kv[c], m[c], hw[c] + pad);
         }
     }
+    // radius of a ball about k = 0 holding the support's box (+inf: no bound)
+    rec.pad0[0] = ok ? (float)(sqrt(hw[0] * hw[0] + hw[1] * hw[1] + hw[2] * hw[2]) * (1.0 + 1e-5) + 1e-6)
+                     : __builtin_huge_valf();
     recs[j] = rec;
-}
-
-__device__ inline void ckahan(float &s, float &c, float v)
-{
-    const float y = v - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
 }
 
 // phase in turns of x . k for one axis: exact product split, integer part dropped
@@ -108,131 +104,130 @@ __device__ inline float turns(float xh, float xl, float k, float &err)
     return p - rintf(p);
 }
 
-// Block = 16 waves on a brick of FT_ROWS (k0, one row per wave of a group)
-// x 4 (k1, per lane) x 64 (k2, lane) samples.  Every atom's envelope is
-// centred at k = 0, so a brick near the origin lists nearly all atoms and one
-// at the edge few: the block's FT_GROUPS wave groups take every FT_GROUPS-th
-// record of the staged list each, and the groups' compensated sums are added
-// in group order at the end (deterministic).  Blocks are numbered bricks
-// nearest the grid centre first (Morton order over the centre-out brick
-// ranks), and the grid holds several times the blocks that fit at once, so
-// the light edge bricks fill the tail.  (2 rows x 8 groups measured best of
-// 1 x 16, 2 x 8 and 4 x 4.)
-constexpr int FT_ROWS = 2, FT_GROUPS = 16 / FT_ROWS, FT_THREADS = 16 * 64, FCHUNK = 256;
+// ---------------------------------------------------------------------------
+// Every atom's support {|U^T k|^2 < 30} is an ellipsoid about k = 0 (about 1%
+// of a 128^3 grid), so the samples near the origin see nearly every atom and
+// the outer ones few.  The records are first put in order of decreasing
+// radius of a ball about k = 0 holding the support (ft_rank_kernel, ties by
+// atom index); a patch whose nearest sample lies at distance d from the
+// origin then only looks at the leading records of radius >= d.
+constexpr int FT_SORT_MAX = 16384;  // atoms ranked by radius; more are taken in atom order
 
-__global__ __launch_bounds__(FT_THREADS) void ft_grid_kernel(const FtRec *__restrict__ recs, int N, gt_kgrid Kg,
-                                                             float2 *__restrict__ out, int e0, int e1, int e2)
+// One wave per atom j: its rank = the atoms of larger radius, or of equal
+// radius and smaller index; the record is copied to that position.
+__global__ __launch_bounds__(256) void ft_rank_kernel(const FtRec *__restrict__ recs, int N, FtRec *__restrict__ sorted)
 {
-    __shared__ FtRec list[FCHUNK];
-    __shared__ int wcnt[FCHUNK / 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int grp = wave / FT_ROWS, w2 = wave % FT_ROWS;
-    const int nb0 = (Kg.m0 + FT_ROWS - 1) / FT_ROWS, nb1 = (Kg.m1 + 3) / 4, nb2 = (Kg.m2 + 63) / 64;
-    // Morton index -> centre-out ranks (t0, t1, t2); ranks beyond the brick counts are empty blocks
+    const int lane = threadIdx.x & 63;
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= N) return;
+    const float rj = recs[j].pad0[0];
+    int rank = 0;
+    for (int i = lane; i < N; i += 64) {
+        const float ri = recs[i].pad0[0];
+        rank += (ri > rj || (ri == rj && i < j)) ? 1 : 0;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) rank += __shfl_xor(rank, o);
+    // eight lanes copy the 128-byte record, 16 bytes each
+    if (lane < 8)
+        reinterpret_cast<int4 *>(sorted + rank)[lane] = reinterpret_cast<const int4 *>(recs + j)[lane];
+}
+
+// Patch kernel: one block of 4 waves per 8 x 8 (k1, k2) patch at one k0;
+// lane l holds the sample at offset (l / 8, l % 8).  Staging (a thread per
+// record, records of radius >= the patch's distance from the origin only):
+// the record's index box against the patch and, for an ascending grid, the
+// least |U^T k|^2 over the patch against the cutoff (ft_patch_min, with a
+// relative slack of 1e-4 so that no sample the FP32 test takes is lost);
+// records that meet the patch go to LDS in record order.  Wave w evaluates
+// the staged records w, w + 4, ... (branch-free: a sample outside the support
+// adds an exact zero) summing in FP64, and the 4 partial sums are added in
+// wave order: every sample's sum has an order fixed by the record order.
+constexpr int FP_THREADS = 256, FP_WAVES = 4, FP_CHUNK = 256, FP_PATCH = 8;
+
+__global__ __launch_bounds__(FP_THREADS) void ft_patch_kernel(const FtRec *__restrict__ recs, int N, gt_kgrid Kg,
+                                                              float2 *__restrict__ out, int e0, int e1, int e2,
+                                                              int by_radius)
+{
+    __shared__ FtL list[FP_CHUNK];  // also the reduction buffer at the end
+    __shared__ int wcnt[FP_WAVES];
+    static_assert(sizeof(list) >= (size_t)FP_WAVES * 64 * sizeof(double2), "reduction buffer");
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nb1 = (Kg.m1 + FP_PATCH - 1) / FP_PATCH, nb2 = (Kg.m2 + FP_PATCH - 1) / FP_PATCH;
     const int e[3] = {e0, e1, e2};
     int t[3];
     morton_ranks(blockIdx.x, e, t);
-    if (t[0] >= nb0 || t[1] >= nb1 || t[2] >= nb2) return;  // whole block
-    const int bb0 = centre_out(t[0], nb0), bb1 = centre_out(t[1], nb1), bb2 = centre_out(t[2], nb2);
-    const int i0 = bb0 * FT_ROWS + w2, i1b = bb1 * 4, i2 = bb2 * 64 + lane;
-    const int blo[3] = {bb0 * FT_ROWS, i1b, bb2 * 64};
-    const int bhi[3] = {min(bb0 * FT_ROWS + FT_ROWS - 1, Kg.m0 - 1), min(i1b + 3, Kg.m1 - 1), min(bb2 * 64 + 63, Kg.m2 - 1)};
-    const float k0 = Kg.k0[min(i0, Kg.m0 - 1)];
-    const float k2 = Kg.k2[min(i2, Kg.m2 - 1)];
-    float k1[4], sr[4], cr[4], si[4], ci[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        k1[q] = Kg.k1[min(i1b + q, Kg.m1 - 1)];
-        sr[q] = cr[q] = si[q] = ci[q] = 0.0f;
-    }
+    if (t[0] >= Kg.m0 || t[1] >= nb1 || t[2] >= nb2) return;  // whole block: no patch of this rank
+    const int i0 = centre_out(t[0], Kg.m0);
+    const int a1 = centre_out(t[1], nb1) * FP_PATCH, a2 = centre_out(t[2], nb2) * FP_PATCH;
+    const int z1 = min(a1 + FP_PATCH - 1, Kg.m1 - 1), z2 = min(a2 + FP_PATCH - 1, Kg.m2 - 1);
+    const int i1 = a1 + (lane >> 3), i2 = a2 + (lane & 7);
+    const float k0 = Kg.k0[i0], k1 = Kg.k1[min(i1, Kg.m1 - 1)], k2 = Kg.k2[min(i2, Kg.m2 - 1)];
+    const float c1a = Kg.k1[a1], c1b = Kg.k1[z1], c2a = Kg.k2[a2], c2b = Kg.k2[z2];
+    const bool exact = Kg.sorted != 0;
+    // distance from k = 0 to the patch (its samples lie in the coordinate box)
+    const float g1 = (c1a <= 0.0f && c1b >= 0.0f) ? 0.0f : fminf(fabsf(c1a), fabsf(c1b));
+    const float g2 = (c2a <= 0.0f && c2b >= 0.0f) ? 0.0f : fminf(fabsf(c2a), fabsf(c2b));
+    const float dmin = exact ? sqrtf(fmaf(k0, k0, fmaf(g1, g1, g2 * g2))) : 0.0f;
     const float cut = (float)(FT_CUT * HALF_LOG2E_SQRT * HALF_LOG2E_SQRT);
-    for (int base = 0; base < N; base += FCHUNK) {
-        // stage the chunk's records that reach the brick (waves 0..3 scan, atom order kept)
+    const float cut_keep = cut * (1.0f + 1e-4f);
+    double sr = 0.0, si = 0.0;
+    for (int base = 0; base < N; base += FP_CHUNK) {
+        // records in radius order: this and every later one fall short of the patch
+        if (by_radius && recs[base].pad0[0] < dmin) break;
         const int j = base + (int)threadIdx.x;
         bool hit = false;
-        if (threadIdx.x < FCHUNK && j < N) {
-            const FtRec &r = recs[j];
-            hit = r.lo[0] <= bhi[0] && r.hi[0] >= blo[0] && r.lo[1] <= bhi[1] && r.hi[1] >= blo[1] &&
-                  r.lo[2] <= bhi[2] && r.hi[2] >= blo[2];
+        if (j < N) {
+            const FtL &rec = *reinterpret_cast<const FtL *>(recs + j);
+            const int *bx = reinterpret_cast<const int *>(recs + j) + 16;  // lo[3], hi[3]
+            hit = rec.pad[0] >= dmin && bx[0] <= i0 && bx[3] >= i0 && bx[1] <= z1 && bx[4] >= a1 && bx[2] <= z2 &&
+                  bx[5] >= a2 && (!exact || ft_patch_min(rec, k0, c1a, c1b, c2a, c2b) < cut_keep);
         }
-        const unsigned long long msk = __ballot(hit);
-        if (wave < FCHUNK / 64 && lane == 0) wcnt[wave] = __popcll(msk);
+        const unsigned long long m = __ballot(hit);
+        if (lane == 0) wcnt[wave] = __popcll(m);
         __syncthreads();
         int off = 0, total = 0;
 #pragma unroll
-        for (int w = 0; w < FCHUNK / 64; w++) {
+        for (int w = 0; w < FP_WAVES; w++) {
             if (w < wave) off += wcnt[w];
             total += wcnt[w];
         }
-        if (hit) {
-            const int4 *src = reinterpret_cast<const int4 *>(recs + j);
-            int4 *dst = reinterpret_cast<int4 *>(list + off + __popcll(msk & ((1ull << lane) - 1ull)));
-#pragma unroll
-            for (int k = 0; k < 8; k++) dst[k] = src[k];
-        }
+        if (hit) list[off + __popcll(m & ((1ull << lane) - 1ull))] = *reinterpret_cast<const FtL *>(recs + j);
         __syncthreads();
-        for (int e = grp; e < total; e += FT_GROUPS) {
-            const FtRec &r = list[e];
-            if (r.lo[0] > i0 || r.hi[0] < i0) continue;
-            float err0 = 0.0f;
-            const float t02 = turns(r.xh[0], r.xl[0], k0, err0) + turns(r.xh[2], r.xl[2], k2, err0);
+        for (int q = wave; q < total; q += FP_WAVES) {
+            const FtL &r = list[q];
+            float nq;
             if (r.iso != 0.0f) {
-                const float base2 = fmaf(k0, k0, k2 * k2);
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const float nq = fmaf(k1[q], k1[q], base2) * (r.u[0] * r.u[0]);
-                    if (nq < cut) {
-                        float err = err0;
-                        const float tt = t02 + turns(r.xh[1], r.xl[1], k1[q], err) + err;
-                        float sn, cs;
-                        ft_sincos_turns(tt, sn, cs);
-                        const float a = r.amp * __builtin_amdgcn_exp2f(-nq);
-                        ckahan(sr[q], cr[q], a * cs);
-                        ckahan(si[q], ci[q], -a * sn);
-                    }
-                }
+                nq = fmaf(k1, k1, fmaf(k0, k0, k2 * k2)) * (r.u[0] * r.u[0]);
             } else {
-                // (U^T k)_0 = u00 k0, (U^T k)_1 = u01 k0 + u11 k1, (U^T k)_2 = u02 k0 + u12 k1 + u22 k2
                 const float v0 = r.u[0] * k0;
-                const float a1 = r.u[3] * k0;
-                const float a2 = fmaf(r.u[5], k0, r.u[2] * k2);
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const float v1 = fmaf(r.u[1], k1[q], a1);
-                    const float v2 = fmaf(r.u[4], k1[q], a2);
-                    const float nq = fmaf(v0, v0, fmaf(v1, v1, v2 * v2));
-                    if (nq < cut) {
-                        float err = err0;
-                        const float tt = t02 + turns(r.xh[1], r.xl[1], k1[q], err) + err;
-                        float sn, cs;
-                        ft_sincos_turns(tt, sn, cs);
-                        const float a = r.amp * __builtin_amdgcn_exp2f(-nq);
-                        ckahan(sr[q], cr[q], a * cs);
-                        ckahan(si[q], ci[q], -a * sn);
-                    }
-                }
+                const float v1 = fmaf(r.u[1], k1, r.u[3] * k0);
+                const float v2 = fmaf(r.u[4], k1, fmaf(r.u[5], k0, r.u[2] * k2));
+                nq = fmaf(v0, v0, fmaf(v1, v1, v2 * v2));
             }
+            float err = 0.0f;
+            const float t01 = turns(r.xh[0], r.xl[0], k0, err) + turns(r.xh[1], r.xl[1], k1, err);
+            const float tt = (t01 + turns(r.xh[2], r.xl[2], k2, err)) + err;
+            float sn, cs;
+            ft_sincos_turns(tt, sn, cs);
+            const float a = nq < cut ? r.amp * __builtin_amdgcn_exp2f(-nq) : 0.0f;
+            sr += (double)(a * cs);
+            si += (double)(-a * sn);
         }
         __syncthreads();
     }
-    // groups' sums (value minus compensation) through LDS, added in group order
-    float2 *red = reinterpret_cast<float2 *>(list);  // [group][w2][q][lane]: 8 x 2 x 4 x 64 float2 = 32 KB
-    static_assert(sizeof(list) >= FT_GROUPS * FT_ROWS * 4 * 64 * sizeof(float2), "reduction buffer");
-#pragma unroll
-    for (int q = 0; q < 4; q++) red[((grp * FT_ROWS + w2) * 4 + q) * 64 + lane] = make_float2(sr[q] - cr[q], si[q] - ci[q]);
+    double2 *red = reinterpret_cast<double2 *>(list);  // [wave][lane]
+    red[wave * 64 + lane] = make_double2(sr, si);
     __syncthreads();
-    if (grp == 0 && i0 < Kg.m0 && i2 < Kg.m2)
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            if (i1b + q >= Kg.m1) continue;
-            float tr = 0.0f, ctr = 0.0f, ti = 0.0f, cti = 0.0f;
-            for (int g = 0; g < FT_GROUPS; g++) {
-                const float2 v = red[((g * FT_ROWS + w2) * 4 + q) * 64 + lane];
-                ckahan(tr, ctr, v.x);
-                ckahan(ti, cti, v.y);
-            }
-            out[((size_t)i0 * Kg.m1 + i1b + q) * Kg.m2 + i2] = make_float2(tr, ti);
+    if (wave == 0 && i1 < Kg.m1 && i2 < Kg.m2) {
+        double tr = 0.0, ti = 0.0;
+        for (int w = 0; w < FP_WAVES; w++) {
+            const double2 v = red[w * 64 + lane];
+            tr += v.x;
+            ti += v.y;
         }
+        out[((size_t)i0 * Kg.m1 + i1) * Kg.m2 + i2] = make_float2((float)tr, (float)ti);
+    }
 }
 
 __global__ void cfill_kernel(float2 *p, size_t n)
@@ -249,7 +244,9 @@ extern "C" size_t gt_ft_workspace(const gt_atoms *atoms, const gt_kgrid *grid)
 {
     (void)grid;
     if (!atoms) return 0;
-    return align_up((size_t)atoms->n * sizeof(FtRec));
+    // the records, then (for at most FT_SORT_MAX atoms) their copy in radius order
+    const size_t one = align_up((size_t)atoms->n * sizeof(FtRec));
+    return atoms->n <= FT_SORT_MAX ? 2 * one : one;
 }
 
 extern "C" int gt_ft_eval(const gt_atoms *atoms, const gt_kgrid *grid, float *out, void *ws, size_t ws_bytes,
@@ -278,15 +275,23 @@ extern "C" int gt_ft_eval(const gt_atoms *atoms, const gt_kgrid *grid, float *ou
         return 0;
     }
     FtRec *recs = (FtRec *)ws;
-    hipLaunchKernelGGL(ft_setup_kernel, dim3((atoms->n + 255) / 256), dim3(256), 0, st, *atoms, *grid, recs);
-    // bricks of 2 x 4 x 64 samples, numbered by Morton index over 2^e0 x 2^e1 x 2^e2 ranks
-    const int e[3] = {ceil_log2((grid->m0 + FT_ROWS - 1) / FT_ROWS), ceil_log2((grid->m1 + 3) / 4),
-                      ceil_log2((grid->m2 + 63) / 64)};
+    const unsigned sb = (unsigned)((atoms->n + 255) / 256);
+    hipLaunchKernelGGL(ft_setup_kernel, dim3(sb), dim3(256), 0, st, *atoms, *grid, recs);
+    const FtRec *order = recs;
+    if (atoms->n <= FT_SORT_MAX) {
+        FtRec *sorted = (FtRec *)((char *)ws + align_up((size_t)atoms->n * sizeof(FtRec)));
+        hipLaunchKernelGGL(ft_rank_kernel, dim3((unsigned)((atoms->n + 3) / 4)), dim3(256), 0, st, recs, atoms->n,
+                           sorted);
+        order = sorted;
+    }
+    // patches of 1 x 8 x 8 samples, numbered by Morton index over 2^e0 x 2^e1 x 2^e2 centre-out ranks
+    const int e[3] = {ceil_log2(grid->m0), ceil_log2((grid->m1 + FP_PATCH - 1) / FP_PATCH),
+                      ceil_log2((grid->m2 + FP_PATCH - 1) / FP_PATCH)};
     GT_CHECK_ARG(e[0] + e[1] + e[2] <= 30, "gt_ft_eval: grid too large");
     const unsigned blocks = 1u << (e[0] + e[1] + e[2]);
     prof_begin(st, GT_K_FT);
-    hipLaunchKernelGGL(ft_grid_kernel, dim3(blocks), dim3(FT_THREADS), 0, st, recs, atoms->n, *grid, (float2 *)out,
-                       e[0], e[1], e[2]);
+    hipLaunchKernelGGL(ft_patch_kernel, dim3(blocks), dim3(FP_THREADS), 0, st, order, atoms->n, *grid,
+                       (float2 *)out, e[0], e[1], e[2], order != recs ? 1 : 0);
     prof_end(st, GT_K_FT);
     GT_HIP(hipGetLastError());
     return 0;

@@ -184,9 +184,9 @@ __device__ inline void pair_moments4(const PairRec &rec, const float *p0, const 
 struct ColChunk {
     int c, ro, R, Wc;
     bool act;
-    __device__ inline void init(int lane, int W, int cb)
+    __device__ inline void init(int lane, int W, int cb, int maxw = 64)
     {
-        Wc = min(64, W - cb);
+        Wc = min(maxw, W - cb);
         R = 64 / Wc;
         ro = lane / Wc;
         c = lane - ro * Wc;
@@ -194,18 +194,127 @@ struct ColChunk {
     }
 };
 
-// Moments of one 2D-detector pair, separable form: per lane and column,
-// A_i = sum_rows w1 dp0^i and B_i = sum_rows E^2 dp0^i (i <= 4); then
-// S1[i, j] += A_i dp1^j, S2[i, j] += B_i dp1^j.  Same sums as pair_moments.
-// UNR rows are loaded per step (more rows in flight hide load latency when
-// few waves share a SIMD).
+// Slack (in units of the exp2 exponent q = kappa m) by which the per-column
+// row spans below are widened: far above the FP32 rounding of q (|q| is a
+// few tens inside a footprint box), so every pixel whose FP32 q passes the
+// cutoff lies inside its span.
+constexpr float SPAN_SLACK = 1.0e-2f;
+
+// Step dx of a uniform grid p[i] = p[0] + i dx that every one of the n
+// coordinates matches to within dx / 4 (one wave; the value is the same in
+// every lane), or 0 if p is not such a grid.
+__device__ inline float grid_step(const float *p, int n, int lane)
+{
+    if (n < 2) return 0.0f;
+    const float dx = (p[n - 1] - p[0]) / (float)(n - 1);
+    bool ok = dx > 0.0f;
+    for (int i = lane; i < n; i += 64) ok = ok && fabsf(p[i] - fmaf((float)i, dx, p[0])) <= 0.25f * dx;
+    return __ballot(!ok) == 0ull ? dx : 0.0f;
+}
+
+// Row table of one pair over a window of H box rows, in the wave's LDS:
+// tab[r] = (x, x^2, x^3, x^4), x = dp0 of row row0 + r (the same FP32
+// expressions a pixel would evaluate).  Returns the spacing dx of a uniform
+// grid x(r) = x(0) + r dx that every row matches to within dx / 4, or 0 if
+// the rows are not such a grid (then no row span is derived from it).
+__device__ inline float row_table(const PairRec &rec, const float *p0, int row0, int H, int lane, float4 *tab)
+{
+    for (int r = lane; r < H; r += 64) {
+        const float x = (p0[row0 + r] - rec.c_hi[0]) - rec.c_lo[0];
+        const float x2 = x * x;
+        tab[r] = make_float4(x, x2, x2 * x, x2 * x2);
+    }
+    wave_sync();
+    if (H < 2) return 0.0f;
+    const float x0 = tab[0].x, dx = (tab[H - 1].x - x0) / (float)(H - 1);
+    bool ok = dx > 0.0f;
+    for (int r = lane; r < H; r += 64) ok = ok && fabsf(tab[r].x - fmaf((float)r, dx, x0)) <= 0.25f * dx;
+    return __ballot(!ok) == 0ull ? dx : 0.0f;
+}
+
+// Rows [rs, re] (window-relative, inclusive) of one column that can hold
+// pixels of the support {q >= kappa}: the roots of the column's quadratic
+// q(x) = q0 x^2 + rowlin x + colt (q0 < 0) widened by SPAN_SLACK, mapped to
+// rows through the uniform grid and padded by one row.  inv2a = 0.5 / (-q0)
+// and inv_dx = 1 / dx are the caller's (approximate reciprocals do: the
+// padding covers their error).  Without a grid (dx = 0) or for a flat
+// quadratic, the whole window.  rs > re: no support.
+__host__ __device__ inline void column_span(const PairRec &rec, float rowlin, float colt, float x0, float dx,
+                                            float inv_dx, float inv2a, int H, int &rs, int &re)
+{
+    rs = 0;
+    re = H - 1;
+    const float a = -rec.q[0];
+    if (!(dx > 0.0f) || !(a > 0.0f)) return;
+    const float disc = fmaf(rowlin, rowlin, 4.0f * a * (colt - (float)KAPPA + SPAN_SLACK));
+    if (!(disc >= 0.0f)) {
+        rs = 1;
+        re = 0;
+        return;
+    }
+    const float sq = sqrtf(disc);
+    const float ulo = fmaf(rowlin - sq, inv2a, -x0) * inv_dx, uhi = fmaf(rowlin + sq, inv2a, -x0) * inv_dx;
+    const float hf = (float)H;
+    const int a0 = (int)floorf(fminf(fmaxf(ulo, -2.0f), hf)) - 1, a1 = (int)ceilf(fminf(fmaxf(uhi, -2.0f), hf)) + 1;
+    rs = a0 < 0 ? 0 : a0;
+    re = a1 > H - 1 ? H - 1 : a1;
+}
+
+// The reciprocals column_span takes: 0.5 / (-q0) and 1 / dx (0 when unused).
+__host__ __device__ inline float span_inv2a(const PairRec &rec) { return -rec.q[0] > 0.0f ? 0.5f / -rec.q[0] : 0.0f; }
+__host__ __device__ inline float span_inv_dx(float dx) { return dx > 0.0f ? 1.0f / dx : 0.0f; }
+
+__device__ inline int wave_min_i32(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline int wave_max_i32(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Can the support {q >= kappa} of a record meet the rectangle [xa, xb] x
+// [ya, yb] of detector offsets from its centre?  q is concave with its
+// maximum at the centre (the expansion point), so the rectangle is met when
+// it holds the centre, else iff the largest value on its four edges (each a
+// concave quadratic in one variable, maximised at its clamped vertex)
+// reaches the cutoff, widened by SPAN_SLACK.  A record whose quadratic is
+// not strictly concave (rounded) is always kept.  f1, f2: the record's
+// floats 4..11, (c_lo1, q0, q1, q2) and (q3, q4, q5, pad).
+__host__ __device__ inline bool support_meets_rect(const float4 &f1, const float4 &f2, float xa, float xb, float ya, float yb)
+{
+    const float q0 = f1.y, q1 = f1.z, q2 = f1.w, q3 = f2.x, q4 = f2.y, q5 = f2.z;
+    if (!(q0 < 0.0f) || !(q2 < 0.0f) || !(q1 * q1 < 4.0f * q0 * q2)) return true;
+    if (xa <= 0.0f && xb >= 0.0f && ya <= 0.0f && yb >= 0.0f) return true;
+    const float i0 = -0.5f / q0, i2 = -0.5f / q2;
+    float best = -3.0e38f;
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+        // edges x = const: concave in y
+        const float x = e ? xb : xa;
+        const float lin = fmaf(q1, x, q4);
+        const float y = fminf(fmaxf(lin * i2, ya), yb);
+        best = fmaxf(best, fmaf(fmaf(q2, y, lin), y, fmaf(fmaf(q0, x, q3), x, q5)));
+        // edges y = const: concave in x
+        const float yy = e ? yb : ya;
+        const float lin2 = fmaf(q1, yy, q3);
+        const float xx = fminf(fmaxf(lin2 * i0, xa), xb);
+        best = fmaxf(best, fmaf(fmaf(q0, xx, lin2), xx, fmaf(fmaf(q2, yy, q4), yy, q5)));
+    }
+    return best >= (float)KAPPA - SPAN_SLACK;
+}
+
 // One pixel of pair_moments_rows: branch-free cutoff (outside the support
-// E = 0 and every accumulation adds an exact zero, so the sums equal those of
-// skipping the pixel).
+// E = 0 and every accumulation adds an exact zero).  xp = (x, x^2, x^3, x^4).
 template <int MODE>
-__device__ inline void moment_pixel(const PairRec &rec, float cv, float x, float rowlin, float colt, float *A,
+__device__ inline void moment_pixel(const PairRec &rec, float cv, float4 xp, float rowlin, float colt, float *A,
                                     float *B, float *acc)
 {
+    const float x = xp.x;
     const float q = fmaf(fmaf(rec.q[0], x, rowlin), x, colt);
     const bool in = q >= (float)KAPPA;
     const float E = in ? __builtin_amdgcn_exp2f(q) : 0.0f;
@@ -218,71 +327,106 @@ __device__ inline void moment_pixel(const PairRec &rec, float cv, float x, float
     } else {
         w1 = cv * E;
     }
-    const float x2 = x * x;
-    const float xp[5] = {1.0f, x, x2, x2 * x, x2 * x2};
-#pragma unroll
-    for (int i = 0; i < 5; i++) A[i] = fmaf(w1, xp[i], A[i]);
+    A[0] += w1;
+    A[1] = fmaf(w1, xp.x, A[1]);
+    A[2] = fmaf(w1, xp.y, A[2]);
+    A[3] = fmaf(w1, xp.z, A[3]);
+    A[4] = fmaf(w1, xp.w, A[4]);
     if (MODE != GT_MODE_LINEAR) {
         const float w2 = E * E;
-#pragma unroll
-        for (int i = 0; i < 5; i++) B[i] = fmaf(w2, xp[i], B[i]);
+        B[0] += w2;
+        B[1] = fmaf(w2, xp.x, B[1]);
+        B[2] = fmaf(w2, xp.y, B[2]);
+        B[3] = fmaf(w2, xp.z, B[3]);
+        B[4] = fmaf(w2, xp.w, B[4]);
     }
 }
 
-template <int MODE, int UNR = 4>
+// Widest column chunk of the moments walk: with at least two row phases per
+// column, lanes of short and long columns of an ellipse even out better than
+// with one lane per column of a 64-wide chunk (measured on the footprints of
+// the 3D workloads: 1.44 against 1.55 lane-steps per in-support pixel at
+// 128^2, epilogues included).
+constexpr int MOM_CHUNK = 32;
+
+// Moments of one 2D-detector pair, separable form: per lane and column,
+// A_i = sum_rows w1 dp0^i and B_i = sum_rows E^2 dp0^i (i <= 4); then
+// S1[i, j] += A_i dp1^j, S2[i, j] += B_i dp1^j.
+//
+// The box is taken in windows of at most TROWS rows.  Per window the wave
+// writes the row table (tab, TROWS entries of its LDS) and, on a uniform
+// detector, the lanes of a column chunk walk only the rows of the hull of
+// their columns' spans (column_span), each lane its own rows ro, ro + R, ...
+// of it: the pixels left out are outside the support, where the full walk
+// adds exact zeros, so the sums are those of walking the whole box.  UNR rows are loaded per step (more rows in flight
+// hide load latency when few waves share a SIMD).
+template <int MODE, int UNR, int TROWS>
 __device__ inline void pair_moments_rows(const PairRec &rec, const float *p0, const float *p1,
-                                         const float *__restrict__ Ca, int n1, int lane, float *acc)
+                                         const float *__restrict__ Ca, int n1, int lane, float *acc, float4 *tab)
 {
     const int W = rec.hi[1] - rec.lo[1] + 1;
-    const int H = rec.hi[0] - rec.lo[0] + 1;
-    if (W <= 0 || H <= 0) return;
-    // 32-bit byte offsets from the wave-uniform bases (scalar-base loads, one
+    const int Hbox = rec.hi[0] - rec.lo[0] + 1;
+    if (W <= 0 || Hbox <= 0) return;
+    // 32-bit byte offsets from the wave-uniform base (scalar-base loads, one
     // multiply-add per pixel); detector slabs are far below 4 GiB
     const char *cbase = reinterpret_cast<const char *>(Ca);
-    const char *pbase = reinterpret_cast<const char *>(p0);
     const unsigned rowb = 4u * (unsigned)n1;
-    for (int cb = 0; cb < W; cb += 64) {
-        ColChunk cc;
-        cc.init(lane, W, cb);
-        if (!cc.act) continue;
-        const int i1 = rec.lo[1] + cb + cc.c;
-        const float dp1 = (p1[i1] - rec.c_hi[1]) - rec.c_lo[1];
-        const float rowlin = fmaf(rec.q[1], dp1, rec.q[3]);
-        const float colt = fmaf(fmaf(rec.q[2], dp1, rec.q[4]), dp1, rec.q[5]);
-        float A[5] = {0, 0, 0, 0, 0}, B[5] = {0, 0, 0, 0, 0};
-        const unsigned cb0 = 4u * ((unsigned)rec.lo[0] * (unsigned)n1 + (unsigned)i1);
-        const unsigned pb0 = 4u * (unsigned)rec.lo[0];
-        int r0 = cc.ro;
-        // full groups of UNR rows, all inside the box
-        for (; r0 + (UNR - 1) * cc.R < H; r0 += UNR * cc.R) {
-            float cv[UNR], dp0[UNR];
+    for (int w0 = 0; w0 < Hbox; w0 += TROWS) {
+        const int H = min(TROWS, Hbox - w0);
+        const int row0 = rec.lo[0] + w0;
+        if (w0 > 0) wave_sync();  // the previous window's table is no longer read
+        const float dx = row_table(rec, p0, row0, H, lane, tab);
+        const float x0 = tab[0].x, inv_dx = span_inv_dx(dx), inv2a = span_inv2a(rec);
+        for (int cb = 0; cb < W; cb += MOM_CHUNK) {
+            ColChunk cc;
+            cc.init(lane, W, cb, MOM_CHUNK);
+            const int i1 = rec.lo[1] + cb + (cc.act ? cc.c : 0);
+            const float dp1 = (p1[i1] - rec.c_hi[1]) - rec.c_lo[1];
+            const float rowlin = fmaf(rec.q[1], dp1, rec.q[3]);
+            const float colt = fmaf(fmaf(rec.q[2], dp1, rec.q[4]), dp1, rec.q[5]);
+            // rows [hs, he]: the hull of the chunk's column spans; every lane
+            // walks its own rows hs + ro, hs + ro + R, ... of it
+            int rs, re;
+            column_span(rec, rowlin, colt, x0, dx, inv_dx, inv2a, H, rs, re);
+            const bool any = cc.act && rs <= re;
+            const int hs = wave_min_i32(any ? rs : H), he = wave_max_i32(any ? re : -1);
+            if (he < hs || !cc.act) continue;
+            float A[5] = {0, 0, 0, 0, 0}, B[5] = {0, 0, 0, 0, 0};
+            const unsigned cb0 = 4u * ((unsigned)row0 * (unsigned)n1 + (unsigned)i1);
+            const unsigned step = (unsigned)cc.R;
+            int r0 = hs + cc.ro;
+            // full groups of UNR rows, all inside the hull
+            for (; r0 + (UNR - 1) * cc.R <= he; r0 += UNR * cc.R) {
+                float cv[UNR];
+                float4 xp[UNR];
 #pragma unroll
-            for (int u = 0; u < UNR; u++) {
-                const unsigned r = (unsigned)(r0 + u * cc.R);
-                cv[u] = *reinterpret_cast<const float *>(cbase + (cb0 + r * rowb));
-                dp0[u] = (*reinterpret_cast<const float *>(pbase + (pb0 + 4u * r)) - rec.c_hi[0]) - rec.c_lo[0];
+                for (int u = 0; u < UNR; u++) {
+                    const unsigned ru = (unsigned)r0 + (unsigned)u * step;
+                    cv[u] = *reinterpret_cast<const float *>(cbase + (cb0 + ru * rowb));
+                    xp[u] = tab[ru];
+                }
+#pragma unroll
+                for (int u = 0; u < UNR; u++) moment_pixel<MODE>(rec, cv[u], xp[u], rowlin, colt, A, B, acc);
             }
-#pragma unroll
-            for (int u = 0; u < UNR; u++) moment_pixel<MODE>(rec, cv[u], dp0[u], rowlin, colt, A, B, acc);
-        }
-        // remaining rows one at a time (same row order)
-        for (; r0 < H; r0 += cc.R) {
-            const unsigned r = (unsigned)r0;
-            const float cv = *reinterpret_cast<const float *>(cbase + (cb0 + r * rowb));
-            const float x = (*reinterpret_cast<const float *>(pbase + (pb0 + 4u * r)) - rec.c_hi[0]) - rec.c_lo[0];
-            moment_pixel<MODE>(rec, cv, x, rowlin, colt, A, B, acc);
-        }
-        const float y2 = dp1 * dp1;
-        const float yp[5] = {1.0f, dp1, y2, y2 * dp1, y2 * y2};
-#pragma unroll
-        for (int i = 0; i <= 4; i++)
-#pragma unroll
-            for (int j = 0; i + j <= 4; j++) {
-                const int m = (i + j) * (i + j + 1) / 2 + j;
-                acc[m] = fmaf(A[i], yp[j], acc[m]);
-                if (MODE != GT_MODE_LINEAR) acc[15 + m] = fmaf(B[i], yp[j], acc[15 + m]);
+            // remaining rows one at a time (same row order)
+            for (; r0 <= he; r0 += cc.R) {
+                const unsigned r = (unsigned)r0;
+                const float cv = *reinterpret_cast<const float *>(cbase + (cb0 + r * rowb));
+                moment_pixel<MODE>(rec, cv, tab[r], rowlin, colt, A, B, acc);
             }
+            const float y2 = dp1 * dp1;
+            const float yp[5] = {1.0f, dp1, y2, y2 * dp1, y2 * y2};
+#pragma unroll
+            for (int i = 0; i <= 4; i++)
+#pragma unroll
+                for (int j = 0; i + j <= 4; j++) {
+                    const int m = (i + j) * (i + j + 1) / 2 + j;
+                    acc[m] = fmaf(A[i], yp[j], acc[m]);
+                    if (MODE != GT_MODE_LINEAR) acc[15 + m] = fmaf(B[i], yp[j], acc[15 + m]);
+                }
+        }
     }
+    wave_sync();  // tab may be rewritten by the caller's next use
 }
 
 __device__ inline float wave_sum_f32(float v)

@@ -21,6 +21,8 @@ namespace gt {
 constexpr int TILE_ROWS = 8, TILE_COLS = 64;
 constexpr int CHUNK = 256;
 constexpr int MOM_STRIDE = 32;  // floats per (atom, angle) moment record
+constexpr int MOM_TROWS = 512;  // rows of a moments wave's row table
+static_assert(MOM_TROWS * 16 <= 64 * (MOM_STRIDE + 1) * 4, "row table must fit the reduction slab");
 constexpr int MODE_LINEAR = GT_MODE_LINEAR, MODE_L2 = GT_MODE_L2, MODE_RESID = GT_MODE_RESID;
 
 __device__ inline void kahan_add(float &s, float &c, float v)
@@ -114,7 +116,7 @@ static_assert(TILE_ROWS == 8 * ((F3_WAVES + 3) / 4) && TILE_COLS == 16 * (F3_WAV
 
 __device__ inline int stage_rects(const PairRec *__restrict__ ra, int base, int N, int row0, int col0, int thi0,
                                   int thi1, PairRec *list, unsigned short (*wl)[F3_CHUNK], int (*cnt)[F3_WAVES],
-                                  int *wlen)
+                                  int *wlen, const float *rx, const float *cy)
 {
     const int lane = threadIdx.x & 63, hw = threadIdx.x >> 6;
     const int j = base + (int)threadIdx.x;
@@ -129,12 +131,20 @@ __device__ inline int stage_rects(const PairRec *__restrict__ ra, int base, int 
         v3 = src[3];
         // v3 = (lo[0], lo[1], hi[0], hi[1])
         hit = v3.x <= thi0 && v3.z >= row0 && v3.y <= thi1 && v3.w >= col0;
-        if (hit)
+        if (hit) {
+            // f0 = (alpha, c_hi0, c_hi1, c_lo0), f1 = (c_lo1, q0, q1, q2), f2 = (q3, q4, q5, pad)
+            const float4 f0 = *reinterpret_cast<const float4 *>(&v0), f1 = *reinterpret_cast<const float4 *>(&v1);
+            const float4 f2 = *reinterpret_cast<const float4 *>(&v2);
 #pragma unroll
             for (int w = 0; w < F3_WAVES; w++) {
                 const int r = row0 + 8 * (w >> 2), c = col0 + 16 * (w & 3);
-                if (v3.x <= r + 7 && v3.z >= r && v3.y <= c + 15 && v3.w >= c) mask |= 1u << w;
+                if (v3.x <= r + 7 && v3.z >= r && v3.y <= c + 15 && v3.w >= c &&
+                    (!rx || support_meets_rect(f1, f2, (rx[2 * (w >> 2)] - f0.y) - f0.w,
+                                               (rx[2 * (w >> 2) + 1] - f0.y) - f0.w, (cy[2 * (w & 3)] - f0.z) - f1.x,
+                                               (cy[2 * (w & 3) + 1] - f0.z) - f1.x)))
+                    mask |= 1u << w;
             }
+        }
     }
     // cnt[k][hw]: hits of this hardware wave, k = 0 the tile list, k = 1 + w wave w's list
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -179,9 +189,11 @@ __device__ inline int stage_rects(const PairRec *__restrict__ ra, int base, int 
 // One record's contribution to a lane's two pixels.  The record arrives as
 // its first 48 bytes (alpha, c_hi, c_lo, q): f0 = (alpha, c_hi0, c_hi1,
 // c_lo0), f1 = (c_lo1, q0, q1, q2), f2 = (q3, q4, q5, pad).  Branch-free
-// cutoff: every record in a wave's list reaches its rectangle.
+// cutoff: every record in a wave's list reaches its rectangle.  The FP32
+// values are summed in FP64 (one conversion and one add per value, against
+// four FP32 operations for a Kahan sum, with a smaller error).
 __device__ inline void f3_pixels(const float4 &f0, const float4 &f1, const float4 &f2, float pc, float prA,
-                                 float prB, float &sA, float &cA, float &sB, float &cB)
+                                 float prB, double &sA, double &sB)
 {
     const float kap = (float)KAPPA;
     const float dp1 = (pc - f0.z) - f1.x;
@@ -192,14 +204,15 @@ __device__ inline void f3_pixels(const float4 &f0, const float4 &f1, const float
     const float qA = fmaf(fmaf(f1.y, dA, rowlin), dA, col);
     const float qB = fmaf(fmaf(f1.y, dB, rowlin), dB, col);
     const float eA = __builtin_amdgcn_exp2f(qA), eB = __builtin_amdgcn_exp2f(qB);
-    kahan_add(sA, cA, qA >= kap ? f0.x * eA : 0.0f);
-    kahan_add(sB, cB, qB >= kap ? f0.x * eB : 0.0f);
+    sA += (double)(qA >= kap ? f0.x * eA : 0.0f);
+    sB += (double)(qB >= kap ? f0.x * eB : 0.0f);
 }
 
 __global__ __launch_bounds__(F3_WAVES * 64) void forward3_kernel(const PairRec *__restrict__ recs, int N, int nA,
                                                                  int n0, int n1, const float *__restrict__ p0,
                                                                  const float *__restrict__ p1, float *__restrict__ R,
-                                                                 const float *__restrict__ sub, int e0, int e1)
+                                                                 const float *__restrict__ sub, int e0, int e1,
+                                                                 int sorted)
 {
     __shared__ PairRec list[F3_CHUNK];
     __shared__ alignas(16) unsigned short wl[F3_WAVES][F3_CHUNK];  // 8-byte reads of 4 indices
@@ -219,10 +232,23 @@ __global__ __launch_bounds__(F3_WAVES * 64) void forward3_kernel(const PairRec *
     const int thi0 = min(row0 + TILE_ROWS - 1, n0 - 1), thi1 = min(col0 + TILE_COLS - 1, n1 - 1);
     const float pc = p1[min(i1, n1 - 1)];
     const float prA = p0[min(rA, n0 - 1)], prB = p0[min(rB, n0 - 1)];
-    float sA = 0.0f, cA = 0.0f, sB = 0.0f, cB = 0.0f;
+    double sA = 0.0, sB = 0.0;
+    // detector coordinates bounding each wave rectangle (first and last
+    // row / column), for the exact cull of an ascending detector
+    float rx[2 * ((F3_WAVES + 3) / 4)], cy[2 * (F3_WAVES < 4 ? F3_WAVES : 4)];
+#pragma unroll
+    for (int w = 0; w < (F3_WAVES + 3) / 4; w++) {
+        rx[2 * w] = p0[min(row0 + 8 * w, n0 - 1)];
+        rx[2 * w + 1] = p0[min(row0 + 8 * w + 7, n0 - 1)];
+    }
+#pragma unroll
+    for (int w = 0; w < (F3_WAVES < 4 ? F3_WAVES : 4); w++) {
+        cy[2 * w] = p1[min(col0 + 16 * w, n1 - 1)];
+        cy[2 * w + 1] = p1[min(col0 + 16 * w + 15, n1 - 1)];
+    }
     const PairRec *ra = recs + (size_t)a * N;
     for (int base = 0; base < N; base += F3_CHUNK) {
-        stage_rects(ra, base, N, row0, col0, thi0, thi1, list, wl, cnt, wlen);
+        stage_rects(ra, base, N, row0, col0, thi0, thi1, list, wl, cnt, wlen, sorted ? rx : nullptr, cy);
         const int len = wlen[wave];
         const unsigned short *wlw = wl[wave];
         int e = 0;
@@ -241,19 +267,19 @@ __global__ __launch_bounds__(F3_WAVES * 64) void forward3_kernel(const PairRec *
                 f2[u] = rp[2];
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++) f3_pixels(f0[u], f1[u], f2[u], pc, prA, prB, sA, cA, sB, cB);
+            for (int u = 0; u < 4; u++) f3_pixels(f0[u], f1[u], f2[u], pc, prA, prB, sA, sB);
         }
         for (; e < len; e++) {
             const float4 *rp = reinterpret_cast<const float4 *>(list + wlw[e]);
-            f3_pixels(rp[0], rp[1], rp[2], pc, prA, prB, sA, cA, sB, cB);
+            f3_pixels(rp[0], rp[1], rp[2], pc, prA, prB, sA, sB);
         }
         __syncthreads();
     }
     // R = projection, or projection - sub (the residual) when sub is given
     if (i1 < n1) {
         const size_t oA = ((size_t)a * n0 + rA) * n1 + i1, oB = ((size_t)a * n0 + rB) * n1 + i1;
-        if (rA < n0) R[oA] = sub ? sA - sub[oA] : sA;
-        if (rB < n0) R[oB] = sub ? sB - sub[oB] : sB;
+        if (rA < n0) R[oA] = sub ? (float)sA - sub[oA] : (float)sA;
+        if (rB < n0) R[oB] = sub ? (float)sB - sub[oB] : (float)sB;
     }
 }
 
@@ -301,7 +327,9 @@ __global__ __launch_bounds__(256) void moments_kernel(const PairRec *__restrict_
                                                       const float *__restrict__ p1, const float *__restrict__ C,
                                                       float *__restrict__ mom)
 {
-    __shared__ float red[4][64][MOM_STRIDE + 1];
+    // the reduction slab of a wave doubles as its row table while it walks
+    // its pair (MOM_TROWS float4 = 8 KiB of the 8.25 KiB slab)
+    __shared__ alignas(16) float red[4][64][MOM_STRIDE + 1];
     // the wave index is uniform: scalar pair index, record and slab base
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const long pair = (long)blockIdx.x * 4 + wave;
@@ -313,7 +341,8 @@ __global__ __launch_bounds__(256) void moments_kernel(const PairRec *__restrict_
         const PairRec rec = recs[pair];
         const int a = (int)(pair / N);
         if (K == 2)
-            pair_moments_rows<MODE>(rec, p0, p1, C + (size_t)a * n0 * n1, n1, lane, acc);
+            pair_moments_rows<MODE, 4, MOM_TROWS>(rec, p0, p1, C + (size_t)a * n0 * n1, n1, lane, acc,
+                                                  reinterpret_cast<float4 *>(&red[wave][0][0]));
         else
             pair_moments<K, MODE>(rec, p0, p1, C + (size_t)a * n0 * n1, n1, lane, acc);
     }
@@ -535,7 +564,7 @@ int launch_forward(const gt_atoms *A, const gt_geom *G, const PairRec *recs, flo
         GT_CHECK_ARG(blocks < (1ull << 31), "forward: too many detector tiles (%d x %d, %d angles)", G->n0, G->n1,
                      G->nA);
         hipLaunchKernelGGL(forward3_kernel, dim3((unsigned)blocks), dim3(F3_WAVES * 64), 0, st, recs, A->n, G->nA,
-                           G->n0, G->n1, G->p0, G->p1, R, sub, e0, e1);
+                           G->n0, G->n1, G->p0, G->p1, R, sub, e0, e1, G->sorted);
     } else {
         const int tiles = (G->n0 + 255) / 256;
         hipLaunchKernelGGL(forward2_kernel, dim3(tiles, G->nA), dim3(256), 0, st, recs, A->n, G->n0, G->p0, R, sub);

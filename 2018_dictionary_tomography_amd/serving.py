@@ -14,9 +14,9 @@ gradients, Hessians and energy changes are reduced per angle in a fixed
 order, so its result depends neither on its group size nor on what else is
 in the table (tests/test_gpu_serving.py checks this bit for bit).
 
-`plain_fits` runs the same fits one request at a time, each with the whole
-device (its default group), the baseline the served throughput is compared
-with.
+`plain_fits` / `plain_fit_timed` run the same fits one request at a time,
+each with the whole device (`best_group` measures the group size at which a
+fit alone runs fastest), the baseline the served throughput is compared with.
 """
 from __future__ import annotations
 
@@ -119,10 +119,19 @@ def update_cost(spec, group):
     return spec.nA * spec.S * spec.S / group + GROUP_UPDATE_OVERHEAD
 
 
+def useful_group(nA):
+    """Largest group worth giving one fit: one angle per workgroup (a fit
+    alone runs fastest there or within a few percent of it, measured over
+    the serving workload's shapes)."""
+    return max(1, int(nA))
+
+
 def allocate_groups(specs, remaining, capacity):
     """Share `capacity` fit workgroups among the active requests: one each,
     then one more at a time to the request whose remaining work per workgroup
-    is largest, up to its useful group size (fit.default_group)."""
+    is largest, up to its useful group size.  With as many active fits as
+    workgroups (the steady state of an overloaded stream) every fit has one;
+    spare workgroups appear as the stream drains."""
     import heapq
 
     n = len(specs)
@@ -130,13 +139,13 @@ def allocate_groups(specs, remaining, capacity):
     spare = capacity - n
     if spare <= 0:
         return groups
-    heap = [(-remaining[i] * update_cost(specs[i], 1), i) for i in range(n) if F.default_group(specs[i].nA) > 1]
+    heap = [(-remaining[i] * update_cost(specs[i], 1), i) for i in range(n) if useful_group(specs[i].nA) > 1]
     heapq.heapify(heap)
     while spare > 0 and heap:
         _, i = heapq.heappop(heap)
         groups[i] += 1
         spare -= 1
-        if groups[i] < F.default_group(specs[i].nA):
+        if groups[i] < useful_group(specs[i].nA):
             heapq.heappush(heap, (-remaining[i] * update_cost(specs[i], groups[i]), i))
     return groups
 
@@ -151,25 +160,35 @@ class ServeStats:
     admitted_bytes_peak: int = 0
     queued_peak: int = 0
     refused: int = 0
-    failed: int = 0
+    failed: int = 0        # fits whose workgroups timed out at a barrier (done = 3)
+    singular: int = 0      # fits stopped by a singular Newton system (done = 2)
     admit_s: float = 0.0   # host + device time spent admitting requests
     launch_s: float = 0.0  # time from each launch to its completion
     trace: list = field(default_factory=list)  # per launch: (start s, active fits, workgroups used, s, updates)
     results: dict = field(default_factory=dict)
 
     def summary(self):
+        """fits_per_s counts fits that ended normally (done = 1) only; latency
+        is arrival to completion (queueing included), service is admission to
+        completion."""
         lat = np.array(self.latencies) if self.latencies else np.zeros(1)
+        svc = np.array(self.service) if self.service else np.zeros(1)
+        ok = self.completed - self.failed - self.singular
         return {
             "completed": self.completed,
-            "fits_per_s": self.completed / self.makespan if self.makespan > 0 else 0.0,
+            "succeeded": ok,
+            "fits_per_s": ok / self.makespan if self.makespan > 0 else 0.0,
             "makespan_s": self.makespan,
             "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)),
+            "p50_service_s": float(np.percentile(svc, 50)),
+            "p99_service_s": float(np.percentile(svc, 99)),
             "launches": self.launches,
             "peak_admitted_bytes": self.admitted_bytes_peak,
             "peak_queue": self.queued_peak,
             "refused": self.refused,
             "failed": self.failed,
+            "singular": self.singular,
             "admit_s": self.admit_s,
             "launch_s": self.launch_s,
         }
@@ -253,10 +272,12 @@ class FitServer:
         self.used_blocks = b
         return dict(zip(act, groups))
 
-    def run(self, specs, time_scale=1.0, log=None, payloads=None):
+    def run(self, specs, time_scale=1.0, log=None, payloads=None, progress=None, progress_every=20.0):
         """Serve `specs` (sorted by arrival).  time_scale stretches the arrival
         clock (1.0 = real time).  payloads: prepare_payloads() output (else each
-        request's data is generated at admission).  Returns ServeStats."""
+        request's data is generated at admission).  log: called with a line per
+        completed fit; progress: called with a status line every
+        progress_every seconds.  Returns ServeStats."""
         st = ServeStats()
         pending = sorted(specs, key=lambda s: s.arrival)
         queue = []
@@ -264,11 +285,12 @@ class FitServer:
         t0 = time.perf_counter()
         nxt = 0
         active = 0
-        progress = {}
+        progress_map = {}
         # the empty table must not be left with stale slots
         self.table.zero_()
         torch.cuda.synchronize(self.dev)
         dirty = True
+        next_note = progress_every
         while nxt < len(pending) or queue or active:
             now = time.perf_counter() - t0
             while nxt < len(pending) and pending[nxt].arrival * time_scale <= now:
@@ -290,7 +312,7 @@ class FitServer:
                                    stage=spec.N <= self.stage_max_atoms)
                 self._write_slot(i, req.slot)
                 self.slots[i] = (spec, req, time.perf_counter() - t0)
-                progress[i] = 0
+                progress_map[i] = 0
                 reserved += spec.footprint_bytes()
                 active += 1
                 dirty = True
@@ -300,8 +322,12 @@ class FitServer:
                 if nxt < len(pending):
                     time.sleep(max(0.0, min(0.01, pending[nxt].arrival * time_scale - now)))
                 continue
+            if progress is not None and now >= next_note:
+                progress(f"serving: t={now:.0f}s completed={st.completed} active={active} queued={len(queue)} "
+                         f"launches={st.launches}")
+                next_note = now + progress_every
             if dirty:
-                self._plan(progress)
+                self._plan(progress_map)
                 dirty = False
             tl = time.perf_counter()
             self._launch()
@@ -309,11 +335,11 @@ class FitServer:
             torch.cuda.synchronize(self.dev)
             dt = time.perf_counter() - tl
             st.launch_s += dt
-            before = sum(progress.values())
+            before = sum(progress_map.values())
             rows = self._read_active()
             for i, s in rows:
-                progress[i] = s.updates
-            st.trace.append((tl - t0, active, self.used_blocks, dt, sum(progress.values()) - before))
+                progress_map[i] = s.updates
+            st.trace.append((tl - t0, active, self.used_blocks, dt, sum(progress_map.values()) - before))
             for i, s in rows:
                 if not (s.done or not s.active):
                     continue
@@ -323,12 +349,13 @@ class FitServer:
                 st.service.append(t_done - t_admit)
                 st.completed += 1
                 st.failed += int(s.done == 3)
+                st.singular += int(s.done == 2)
                 if self.keep_results:
                     st.results[spec.rid] = (req.I.clone(), req.x.clone(), req.r.clone(), s.E, s.done, s.updates)
                 reserved -= spec.footprint_bytes()
                 active -= 1
                 self.slots[i] = None
-                progress.pop(i, None)
+                progress_map.pop(i, None)
                 self._write_slot(i, F.gt_fit_slot())  # inactive
                 dirty = True
                 if log:
@@ -336,6 +363,67 @@ class FitServer:
                         f"latency={st.latencies[-1]:.2f}s")
         st.makespan = time.perf_counter() - t0
         return st
+
+
+def group_candidates(nA, capacity):
+    """Group sizes tried for a plain fit of nA angles: the default (about one
+    angle per wave), 16, 32, 64 and one angle per workgroup."""
+    c = {F.default_group(nA), 16, 32, 64, int(nA)}
+    return sorted(g for g in c if 1 <= g <= min(int(nA), capacity))
+
+
+def best_group(spec, payload, device, updates=None, candidates=None):
+    """Fastest group size for a plain fit of this request: each candidate
+    advances a fresh copy of the fit by the same number of atom updates (after
+    a short warm-up) and the one with the least time per update wins.
+    Returns (group, {group: seconds per update})."""
+    start, space, data = payload
+    cands = candidates or group_candidates(spec.nA, F.max_blocks())
+    n = updates or max(16, min(spec.N, 96))
+    per = {}
+    for g in cands:
+        req = F.FitRequest(start, space, data, [spec.max_iter, 1], record=False)
+        table = torch.as_tensor(np.frombuffer(req.slot_bytes(), dtype=np.uint8).copy(), device=device)
+        blocks = torch.as_tensor(F.block_table([g]), device=device)
+        F.run_blocks(table, 1, blocks, g, req.dim, 4)
+        torch.cuda.synchronize(device)
+        u0 = F.read_slots(table, 1)[0].updates
+        t0 = time.perf_counter()
+        F.run_blocks(table, 1, blocks, g, req.dim, n)
+        torch.cuda.synchronize(device)
+        dt = time.perf_counter() - t0
+        done = F.read_slots(table, 1)[0].updates - u0
+        per[g] = dt / max(done, 1)
+    return min(per, key=per.get), per
+
+
+def plain_fit_timed(spec, payload, device, group, max_updates=None, work_per_launch=4.0e7):
+    """One plain fit (the request alone on the device, `group` workgroups),
+    stopped after max_updates atom updates if given.  Returns (seconds,
+    FitRequest, final slot)."""
+    start, space, data = payload
+    req = F.FitRequest(start, space, data, [spec.max_iter, 1], record=False)
+    table = torch.as_tensor(np.frombuffer(req.slot_bytes(), dtype=np.uint8).copy(), device=device)
+    blocks = torch.as_tensor(F.block_table([group]), device=device) if group > 1 else None
+    budget = max(1, int(work_per_launch / update_cost(spec, group)))
+    torch.cuda.synchronize(device)
+    t0 = time.perf_counter()
+    while True:
+        left = budget if max_updates is None else min(budget, max_updates - F.read_slots(table, 1)[0].updates)
+        if left <= 0:
+            break
+        if blocks is None:
+            F.run_slots(table, 1, req.dim, left)
+        else:
+            F.run_blocks(table, 1, blocks, group, req.dim, left)
+        s = F.read_slots(table, 1)[0]
+        if s.done or not s.active:
+            break
+    torch.cuda.synchronize(device)
+    dt = time.perf_counter() - t0
+    s = F.read_slots(table, 1)[0]
+    req.final = s
+    return dt, req, s
 
 
 def plain_fits(specs, device=None, keep_results=False, work_per_launch=4.0e7, group=None, payloads=None):

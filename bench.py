@@ -29,8 +29,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 # Frozen algorithmic work per in-support (atom, pixel) pair (FP32 flops, FMA = 2).
-FLOPS_FWD = 11
-FLOPS_L2 = 85
+FLOPS_FWD = 11   # forward: quadratic (Horner) 9, amplitude 1, accumulate 1; + 1 exp
+FLOPS_ADJ = 25   # adjoint (linear functional): value 11 + 7 residual-weighted moment FMAs; + 1 exp
+FLOPS_L2 = 85    # L2 gradient + Hessian: value 11, 7 first moments, 15 + 15 degree-4 moments; + 1 exp
+FLOPS_FT = 15    # Fourier: U~^T k 6, norm 3 (+2), phase 3, complex multiply-add 4; + exp + sincos
+FLOPS_VOL = 20   # volume: y - x 3, U (y - x) 9, squared norm 5, scale 1, amplitude 1, accumulate 1; + 1 exp
 PEAK_FP32_TFLOPS = 157.3      # MI355X published FP32 (vector, packed / matrix)
 PEAK_FP32_NONPACKED = 78.65   # 256 CU x 4 SIMD x 16 lanes x 2 flop x 2.4 GHz
 PEAK_TRANS_PER_S = 256 * 4 * 16 * 2.4e9 / 4  # quarter-rate transcendental
@@ -48,14 +51,18 @@ def parse():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--atoms", type=int, default=1000)
     ap.add_argument("--fit-iters", type=int, default=20, help="Newton outer iterations of the fit leg (0 skips it)")
-    ap.add_argument("--serve-requests", type=int, default=256, help="requests of the serving leg (0 skips it)")
-    ap.add_argument("--serve-nmax", type=int, default=500, help="upper end of the log-uniform atom count")
-    ap.add_argument("--plain-requests", type=int, default=6, help="requests timed one at a time")
+    ap.add_argument("--serve-requests", type=int, default=2000, help="requests of the serving leg (0 skips it)")
+    ap.add_argument("--serve-nmax", type=int, default=5000, help="upper end of the log-uniform atom count")
+    ap.add_argument("--plain-cap", type=float, default=3.0,
+                    help="seconds a sampled plain fit may run before it is cut to a timed prefix")
     ap.add_argument("--autograd-reps", type=int, default=3,
                     help="timed forward+backward passes of the autograd leg (0 skips it)")
     ap.add_argument("--fourier-reps", type=int, default=5, help="timed evaluations of the Fourier leg (0 skips it)")
     ap.add_argument("--extras-reps", type=int, default=5,
                     help="timed calls of the point-spread / Fourier-plane / KL-moment leg (0 skips it)")
+    ap.add_argument("--gd-iters", type=int, default=100, help="outer iterations of the 2D block-GD leg (0 skips it)")
+    ap.add_argument("--gd-cpu-iters", type=int, default=5,
+                    help="outer iterations of the 2D block-GD leg timed on the CPU oracle")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg and print its JSON")
     ap.add_argument("--skip-checks", action="store_true",
                     help="skip the FP64 parity spot check and the in-support pair count (counter runs)")
@@ -87,46 +94,104 @@ def fit_leg(torch, gtp, space, data, start, iters):
     }
 
 
+def stratified_sample(specs, n_edges, s_bins, seed):
+    """One request per (atom-count bin, detector-size bin) cell, drawn with a
+    seeded shuffle: n_edges are the atom-count bin edges, s_bins tuples of
+    detector sides."""
+    order = np.random.default_rng(seed).permutation(len(specs))
+    pick = []
+    for a, b in zip(n_edges[:-1], n_edges[1:]):
+        for sides in s_bins:
+            for i in order:
+                sp = specs[i]
+                if a <= sp.N < b and sp.S in sides:
+                    pick.append(int(i))
+                    break
+    return sorted(pick)
+
+
+def _cost_features(nA, S):
+    nA, S = np.asarray(nA, float), np.asarray(S, float)
+    return np.stack([np.ones_like(nA), nA, S * S, nA * S * S], -1)
+
+
 def serving_leg(torch, args, dev):
     """Poisson stream at 200 req/s with the serving workload's distributions
-    (angles U{30..120} over +-60 deg, detector side from {64,96,128,192,256},
-    30-iteration Newton), atom count log-uniform in [50, serve_nmax].  Served
-    = continuous batching into the slot table with graph-replayed grouped
-    launches; plain = the same fits one request at a time, each with the whole
-    device (its default workgroup group), on a seeded random subset.  Request
-    payloads (start atoms, data) are generated before either is timed."""
+    (N log-uniform in [50, serve_nmax], angles U{30..120} over +-60 deg,
+    detector side from {64,96,128,192,256}, 30-iteration Newton).
+
+    Served = continuous batching into the slot table with graph-replayed
+    grouped launches.  Plain = the same fits one request at a time with the
+    whole device, each at the group size that runs it fastest (best_group).
+    Plain is measured on a stratified sample (atom-count quartiles x three
+    detector-size bins = 12 requests): a sampled fit runs in full when it
+    takes at most plain_cap seconds, else for its first atom updates within
+    that time, and its time per update is kept.  A per-update cost model
+    u(nA, S) = b0 + b1 nA + b2 S^2 + b3 nA S^2 is fitted to the sample (its
+    residual is reported) and applied to every request with its update count
+    from the served run (a fit makes the same updates alone and served, the
+    results being equal): plain fits/s = requests / sum of modelled times.
+    Request payloads (start atoms, data) are generated before either is
+    timed."""
     Sv = importlib.import_module("2018_dictionary_tomography_amd.serving")
     specs = Sv.c5_stream(args.serve_requests, rate=200.0, seed=args.seed, n_range=(50, args.serve_nmax))
-    log = (lambda m: print(m, file=sys.stderr, flush=True)) if args.serve_only else None
+    log = (lambda m: print(m, file=sys.stderr, flush=True))
     t0 = time.perf_counter()
     payloads = Sv.prepare_payloads(specs, dev)
     torch.cuda.synchronize()
     prep_s = time.perf_counter() - t0
-    if log:
-        log(f"payloads of {len(specs)} requests built in {prep_s:.1f}s")
+    log(f"serving: payloads of {len(specs)} requests built in {prep_s:.1f}s")
     server = Sv.FitServer(dev, keep_results=True)
-    st = server.run(specs, log=log, payloads=payloads)
+    st = server.run(specs, log=None, payloads=payloads, progress=log, progress_every=20.0)
     served = st.summary()
-    pick = sorted(np.random.default_rng(args.seed + 1).choice(len(specs), min(args.plain_requests, len(specs)),
-                                                              replace=False).tolist())
-    times, alone = [], {}
+    log(f"serving: served {served['succeeded']} fits in {served['makespan_s']:.1f}s "
+        f"({served['fits_per_s']:.2f} fits/s)")
+    updates = {rid: int(v[5]) for rid, v in st.results.items()}
+    # plain: stratified sample
+    edges = np.exp(np.linspace(np.log(50), np.log(args.serve_nmax + 1), 5))
+    pick = stratified_sample(specs, edges, ((64, 96), (128,), (192, 256)), args.seed + 1)
+    rows, same = [], True
     for i in pick:
-        t, a = Sv.plain_fits([specs[i]], dev, keep_results=True, payloads=payloads)
-        times += t
-        alone.update(a)
-        if log:
-            log(f"plain rid={i} N={specs[i].N} nA={specs[i].nA} S={specs[i].S} service={t[0]:.2f}s")
-    same = all(torch.equal(alone[k][j], st.results[k][j]) for k in alone for j in range(3)) and \
-        all(alone[k][3] == st.results[k][3] for k in alone)
-    plain_fps = 1.0 / float(np.mean(times))
+        sp = specs[i]
+        g, per = Sv.best_group(sp, payloads[sp.rid], dev)
+        total = updates[sp.rid]
+        cap = max(sp.N, int(args.plain_cap / per[g]))
+        dt, req, s = Sv.plain_fit_timed(sp, payloads[sp.rid], dev, g, max_updates=min(total, cap))
+        full = bool(s.done)
+        if full:
+            Im, xm, rm, Em, donem, updm = st.results[sp.rid]
+            same &= (torch.equal(req.I, Im) and torch.equal(req.x, xm) and torch.equal(req.r, rm) and s.E == Em
+                     and s.updates == updm)
+        rows.append({"rid": int(sp.rid), "N": sp.N, "nA": sp.nA, "S": sp.S, "group": int(g),
+                     "updates_timed": int(s.updates), "updates_total": total, "seconds_timed": dt,
+                     "us_per_update": dt / max(s.updates, 1) * 1e6, "full_fit": full,
+                     "group_us_per_update": {str(k): round(v * 1e6, 1) for k, v in per.items()}})
+        log(f"plain rid={sp.rid} N={sp.N} nA={sp.nA} S={sp.S} G={g} {s.updates}/{total} updates in {dt:.2f}s")
+    u = np.array([r["us_per_update"] for r in rows]) * 1e-6
+    X = _cost_features([r["nA"] for r in rows], [r["S"] for r in rows])
+    beta = np.linalg.lstsq(X / u[:, None], np.ones_like(u), rcond=None)[0]   # relative least squares
+    resid = X @ beta / u - 1.0
+    allX = _cost_features([sp.nA for sp in specs], [sp.S for sp in specs])
+    per_fit = np.array([updates[sp.rid] for sp in specs]) * (allX @ beta)
+    plain_fps = len(specs) / float(per_fit.sum())
+    for r, x in zip(rows, X):
+        r["model_seconds_full"] = float(r["updates_total"] * (x @ beta))
     return {
         "requests": len(specs), "rate_per_s": 200.0, "atoms_range": [50, args.serve_nmax],
-        "iterations": 30, "served": served, "client_payload_s": prep_s,
-        "plain": {"requests_timed": pick, "service_s": times, "fits_per_s": plain_fps,
-                  "note": "same device fitter, one request at a time, whole device (default group)"},
+        "angles_range": [30, 120], "detector_sides": [64, 96, 128, 192, 256], "iterations": 30,
+        "served": served, "client_payload_s": prep_s,
+        "plain": {
+            "fits_per_s": plain_fps,
+            "p50_service_s": float(np.percentile(per_fit, 50)), "p99_service_s": float(np.percentile(per_fit, 99)),
+            "sample": rows,
+            "cost_model": {"form": "us/update = b0 + b1 nA + b2 S^2 + b3 nA S^2", "coef_s": beta.tolist(),
+                           "rms_rel_residual": float(np.sqrt(np.mean(resid ** 2))),
+                           "max_rel_residual": float(np.abs(resid).max())},
+            "note": "same device fitter, one request at a time with the whole device at its fastest group size; "
+                    "service times = served update counts x modelled time per update"},
         "speedup_fits_per_s": served["fits_per_s"] / plain_fps,
         "alone_equals_mixed": bool(same),
-        "max_residual_drift": max(a[6] for a in alone.values()),
+        "alone_equals_mixed_checked": int(sum(r["full_fit"] for r in rows)),
     }
 
 
@@ -163,9 +228,17 @@ def autograd_leg(torch, gtp, dev, reps):
         loss = one()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / reps
+    L = importlib.import_module("2018_dictionary_tomography_amd._lib")
+    kt = _kernel_table(torch, one, reps, {"setup": L.GT_K_SETUP, "forward": L.GT_K_FORWARD,
+                                          "moments": L.GT_K_MOMENTS, "finish": L.GT_K_FINISH})
+    P_in = support_pairs_columns(torch, t[2].detach().cpu().numpy(), t[1].detach().cpu().numpy(), T, ortho[0],
+                                 ortho[1], det[0], det[1])
     return {"workload": "5000 atoms, 90 angles over [0, pi) about a random axis, 256x256 detector",
             "ms_per_forward_backward": ms, "evals_per_s": 2.0 * N * nA * S * S / (ms * 1e-3),
-            "loss": float(loss.detach()), "reps": reps}
+            "loss": float(loss.detach()), "reps": reps, "kernel_ms": kt,
+            "roofline": {"forward": roof_entry("forward3_kernel", P_in, kt["forward"], FLOPS_FWD, 1),
+                         "adjoint": roof_entry("moments_kernel<2,LINEAR> (hand-written backward)", P_in,
+                                               kt["moments"], FLOPS_ADJ, 1)}}
 
 
 def fourier_leg(torch, gtp, dev, reps):
@@ -206,8 +279,15 @@ def fourier_leg(torch, gtp, dev, reps):
     got = out[idx[:, 0], idx[:, 1], idx[:, 2]].to(torch.complex128)
     err = float((got - ref).abs().max() / ref.abs().max())
     pairs = 2000.0 * 128 ** 3
+    L = importlib.import_module("2018_dictionary_tomography_amd._lib")
+    kms = _kernel_table(torch, lambda: Fo.evaluate(atoms, grid, out=out), reps, {"ft": L.GT_K_FT})["ft"]
+    Uc = U.cpu().numpy()
+    P_in = ellipsoid_lattice_count(torch, Uc @ Uc.transpose(0, 2, 1), np.zeros((len(nI), 3)), k[0], k[1], k[2],
+                                   30.0, strict=True)
     return {"workload": "2000 atoms on the integer grid [-64, 64)^3, complex64", "ms_per_evaluation": ms,
-            "atom_k_pairs_per_s": pairs / (ms * 1e-3), "rel_err_vs_fp64_sampled": err, "reps": reps}
+            "atom_k_pairs_per_s": pairs / (ms * 1e-3), "rel_err_vs_fp64_sampled": err, "reps": reps,
+            "roofline": roof_entry("ft_patch_kernel", P_in, kms, FLOPS_FT, 2,
+                                   "in-support = (atom, k) pairs with |U~^T k|^2 < 30")}
 
 
 def _timed(torch, fn, reps):
@@ -218,6 +298,25 @@ def _timed(torch, fn, reps):
         fn()
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) * 1e3 / reps
+
+
+def _kernel_table(torch, fn, reps, ids):
+    """Average launch duration (ms) of several library kernels over reps
+    calls of fn, from the library's HIP-event hooks (on each kernel's stream)."""
+    L = importlib.import_module("2018_dictionary_tomography_amd._lib")
+    torch.cuda.synchronize()
+    L.profile_enable(True)
+    for k in ids.values():
+        L.profile_read(k)
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    L.profile_enable(False)
+    out = {}
+    for name, k in ids.items():
+        tot, cnt = L.profile_read(k)
+        out[name] = tot / max(cnt, 1)
+    return out
 
 
 def _kernel_ms(torch, kernel_id, fn, reps):
@@ -266,9 +365,18 @@ def extras_leg(torch, gtp, dev, reps):
     ms_pv = _timed(torch, lambda: gtp.ops.volume(atoms, vol128, out=u128), reps)
     L = importlib.import_module("2018_dictionary_tomography_amd._lib")
     k_pv = _kernel_ms(torch, L.GT_K_VOLUME, lambda: gtp.ops.volume(atoms, vol128, out=u128), reps)
+    rc = np.asarray(r, dtype=np.float32).astype(np.float64)
+    Uv = np.zeros((rc.shape[0], 3, 3))
+    for k_, (i_, j_) in enumerate([(0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (0, 2)]):
+        Uv[:, i_, j_] = rc[:, k_]
+    cv = vol128.coord_vectors
+    P_vol = ellipsoid_lattice_count(torch, Uv.transpose(0, 2, 1) @ Uv, np.float32(x).astype(np.float64), cv[0],
+                                    cv[1], cv[2], 1.0, strict=False)
     out["volume"] = {"workload": "VolProj of the same 1000 atoms on 128^3 voxel centres",
                      "ms": ms_pv, "kernel_ms": k_pv, "note": "ms: wall clock per call (host-bound); kernel_ms: "
-                     "the render kernel alone (HIP events)"}
+                     "the render kernel alone (HIP events)",
+                     "roofline": roof_entry("volume3_kernel", P_vol, k_pv, FLOPS_VOL, 1,
+                                            "in-support = (atom, voxel) pairs with |U(y - x)|^2 <= 1")}
     out["psf"] = {"workload": "1000 atoms, 61 x 128^2, pixel-integrated; voxel-integrated render on 64^3",
                   "forward_ms": ms_f, "derivs_L2_ms": ms_d, "volume_ms": ms_v,
                   "forward_atom_pixel_per_s": 1000 * rays / (ms_f * 1e-3),
@@ -332,29 +440,182 @@ def support_pairs(torch, I, x, r, T, w0, w1, p0, p1):
     return total, R64
 
 
+def _count_in(lo, hi, grid):
+    """Number of ascending grid values in [lo, hi] (elementwise)."""
+    import torch
+    return (torch.searchsorted(grid, hi, right=True) - torch.searchsorted(grid, lo, right=False)).clamp_min(0)
+
+
+def support_pairs_columns(torch, r, x, T, w0, w1, p0, p1, chunk=1024):
+    """Exact number of (atom, pixel) pairs with m <= 1 on a 2D detector,
+    counted per detector column (FP64 torch, independent of the library):
+    along a column p1 the ray offset is G p - x = p0 w0 + d, d = p1 w1 - x,
+    so m = a p0^2 + b p0 + c with a = w0^T Q w0, b = 2 w0^T Q d, c = d^T Q d
+    (Q = U^T (Id - t t^T/|t|^2) U, t = U T), and the rows inside are the
+    detector coordinates between the roots of m = 1."""
+    dev = torch.device("cuda" if torch.cuda.is_available() else "cpu")
+    d64 = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64), device=dev)
+    r, x = d64(r), d64(x)
+    N = r.shape[0]
+    U = torch.zeros((N, 3, 3), dtype=torch.float64, device=dev)
+    for k, (i, j) in enumerate([(0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (0, 2)]):
+        U[:, i, j] = r[:, k]
+    P = U.transpose(1, 2) @ U
+    g0, g1 = d64(p0), d64(p1)
+    total = 0
+    for a in range(T.shape[0]):
+        Ta, v0, v1 = d64(T[a]), d64(w0[a]), d64(w1[a])
+        z = P @ Ta
+        Q = P - z[:, :, None] * z[:, None, :] / (z @ Ta)[:, None, None]
+        Qw0 = Q @ v0
+        A = Qw0 @ v0
+        for c in range(0, N, chunk):
+            D = g1[None, :, None] * v1[None, None, :] - x[c:c + chunk, None, :]
+            QD = torch.matmul(D, Q[c:c + chunk].transpose(1, 2).contiguous())
+            B = 2.0 * (D * Qw0[c:c + chunk, None, :]).sum(-1)
+            C = (D * QD).sum(-1) - 1.0
+            Ac = A[c:c + chunk, None]
+            disc = B * B - 4.0 * Ac * C
+            ok = (disc >= 0) & (Ac > 0)
+            sq = disc.clamp_min(0).sqrt()
+            lo, hi = (-B - sq) / (2 * Ac), (-B + sq) / (2 * Ac)
+            total += int((_count_in(lo.contiguous(), hi.contiguous(), g0) * ok).sum())
+    return total
+
+
+def ellipsoid_lattice_count(torch, M, centre, k0, k1, k2, bound, strict, chunk=256):
+    """Number of grid points y = (k0[i], k1[j], k2[l]) with (y - c)^T M (y - c)
+    below `bound` (strictly if `strict`), per atom M (n, 3, 3) and centre c
+    (n, 3), summed over atoms: per (k0, k1) column the inequality is a
+    quadratic in the third coordinate, whose roots bound the points inside."""
+    dev = torch.device("cuda" if torch.cuda.is_available() else "cpu")
+    d64 = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64), device=dev)
+    M, centre = d64(M), d64(centre)
+    a0, a1, a2 = d64(k0), d64(k1), d64(k2)
+    Y0, Y1 = torch.meshgrid(a0, a1, indexing="ij")
+    total = 0
+    for c in range(0, M.shape[0], chunk):
+        m = M[c:c + chunk]
+        e0 = Y0[None] - centre[c:c + chunk, 0, None, None]
+        e1 = Y1[None] - centre[c:c + chunk, 1, None, None]
+        g = lambda i, j: m[:, i, j, None, None]
+        A = g(2, 2).expand_as(e0)
+        B = 2.0 * (g(0, 2) * e0 + g(1, 2) * e1)
+        C = g(0, 0) * e0 * e0 + 2.0 * g(0, 1) * e0 * e1 + g(1, 1) * e1 * e1 - bound
+        disc = B * B - 4.0 * A * C
+        ok = disc > 0 if strict else disc >= 0
+        sq = disc.clamp_min(0).sqrt()
+        lo = (-B - sq) / (2 * A) + centre[c:c + chunk, 2, None, None]
+        hi = (-B + sq) / (2 * A) + centre[c:c + chunk, 2, None, None]
+        if strict:
+            n = (torch.searchsorted(a2, hi.reshape(-1).contiguous(), right=False)
+                 - torch.searchsorted(a2, lo.reshape(-1).contiguous(), right=True)).clamp_min(0)
+        else:
+            n = _count_in(lo.reshape(-1).contiguous(), hi.reshape(-1).contiguous(), a2)
+        total += int((n * ok.reshape(-1)).sum())
+    return total
+
+
+def roof_entry(kernel, pairs, ms, flops, trans, note=None):
+    """Roofline block of one kernel: algorithmic FP32 flops and transcendentals
+    per in-support pair times the pairs of one launch, over its average
+    launch time (HIP events on its stream)."""
+    ach = pairs * flops / (ms * 1e-3) / 1e12
+    out = {"bound": "mfma", "engine": "fp32-valu", "kernel": kernel, "in_support_pairs": int(pairs),
+           "flops_per_pair": flops, "transcendentals_per_pair": trans, "avg_kernel_ms": round(ms, 5),
+           "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+           "frac": round(ach / PEAK_FP32_TFLOPS, 4), "frac_nonpacked": round(ach / PEAK_FP32_NONPACKED, 4),
+           "trans_frac": round(pairs * trans / (ms * 1e-3) / PEAK_TRANS_PER_S, 4)}
+    if note:
+        out["note"] = note
+    return out
+
+
 def cpu_baseline(I, x, r, T, ortho, det, data, n_atoms):
-    """Oracle (FP64 C restatement of the reference's dense loops, OpenMP) on a
-    bounded sample: forward + L2 derivatives of the first n_atoms atoms over the
-    full geometry.  Rate in dense atom.ray evaluations per second."""
+    """The reference's dense loops on the host cores (one ray x all atoms for
+    the forward, one atom x all rays for the L2 gradient and Hessian,
+    NewtonBases.py:729-870), restated in C (oracle/gt_oracle.c) and compiled
+    here with gcc -O3 -march=native -fopenmp in FP32 (the reference's float32)
+    and FP64, on a bounded sample: the first n_atoms atoms over the full
+    geometry.  value = the FP32 rate in dense atom.ray evaluations per second."""
+    import tempfile
     from oracle import oracle as O
 
-    O.build()
     g = O.Geometry(T, ortho, det)
     rays = T.shape[0] * det[0].size * det[1].size
     sl = slice(0, n_atoms)
-    t0 = time.perf_counter()
-    R = O.radproj(I[sl], x[sl], r[sl], g)
-    t1 = time.perf_counter()
-    O.derivs(I[sl], x[sl], r[sl], g, data - R, "L2", 2)
-    t2 = time.perf_counter()
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for bits in (32, 64):
+            cb = O.CpuBaseline(tmp, bits)
+            t0 = time.perf_counter()
+            R = cb.radproj(I[sl], x[sl], r[sl], g)
+            t1 = time.perf_counter()
+            cb.derivs(I[sl], x[sl], r[sl], g, data - R, "L2", 2)
+            t2 = time.perf_counter()
+            out[f"fp{bits}"] = {"value": 2.0 * n_atoms * rays / (t2 - t0), "forward_s": t1 - t0,
+                                "derivs_s": t2 - t1, "threads": cb.threads(), "flags": cb.flags}
     return {
-        "value": 2.0 * n_atoms * rays / (t2 - t0),
+        "value": out["fp32"]["value"],
         "unit": "atom-ray evals/s",
-        "cores": O.num_threads(),
+        "cores": out["fp32"]["threads"],
         "kind": "port",
-        "sample": f"{n_atoms} of the atoms, full {T.shape[0]}x{det[0].size}x{det[1].size} geometry: "
-                  f"dense FP64 forward ({t1 - t0:.2f} s) + L2 gradient/Hessian ({t2 - t1:.2f} s)",
+        "sample": f"{n_atoms} of the atoms, full {T.shape[0]}x{det[0].size}x{det[1].size} geometry: dense forward "
+                  f"+ L2 gradient/Hessian (the reference's loops in C, gcc -O3 -march=native, OpenMP); value is "
+                  f"FP32, fp64 beside it",
+        "fp32": out["fp32"], "fp64": out["fp64"],
+        "cpu": _cpu_model(),
     }
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def gd2d_leg(torch, gtp, dev, iters, cpu_iters):
+    """The 2D configuration: 50 isotropic atoms (centres U[-0.8, 0.8]^2,
+    radii U[0.02, 0.08], intensities U[0.5, 1.5]), 180 angles over [0, pi),
+    128-pixel detector on [-1, 1], data = forward + 1% noise, block gradient
+    descent (GD_lib.linesearch_block) from centres + N(0, 0.02).  GPU: the
+    drop-in fitter for `iters` outer iterations.  CPU: the FP64 oracle's
+    restatement (Python loop over the C projector, OpenMP) for cpu_iters
+    outer iterations, a bounded sample of the same run."""
+    from oracle import oracle as O
+    F = importlib.import_module("2018_dictionary_tomography_amd.fit")
+    W = importlib.import_module("2018_dictionary_tomography_amd.workloads")
+    rng = np.random.default_rng(7)
+    I, x, r = W.draw_iso2(50, rng)
+    T, ortho, det = W.fan_2d(180, 128)
+    space = gtp.ProjSpace(orientations=T, ortho=ortho, detector=det)
+    aspace = gtp.AtomSpace(2, isotropic=True)
+    data = gtp.ops.forward(gtp.AtomElement(aspace, x, r, I, device=dev), space)
+    data = data + torch.as_tensor(rng.normal(0, 0.01 * float(data.max()), data.shape), dtype=torch.float32,
+                                  device=dev)
+    x0 = x + rng.normal(0, 0.02, x.shape)
+    start = gtp.AtomElement(aspace, x0, r, I, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, Eg = F.gd_linesearch_block(start, data, [iters, 1], space)
+    torch.cuda.synchronize()
+    tg = time.perf_counter() - t0
+    f32 = lambda a: np.float32(a).astype(np.float64)
+    g = O.Geometry(f32(T), tuple(f32(o) for o in ortho), [f32(p) for p in det])
+    t0 = time.perf_counter()
+    _, _, _, Eo = O.gd_linesearch_block(f32(I), f32(x0), f32(r), g, data.double().cpu().numpy(), [cpu_iters, 1])
+    tc = time.perf_counter() - t0
+    n = min(len(Eg), len(Eo))
+    return {"workload": "50 isotropic 2D atoms, 180 angles over [0, pi), 128-pixel detector, block GD",
+            "gpu": {"outer_iterations": iters, "block_steps": len(Eg) - 1, "seconds": tg,
+                    "s_per_iteration": tg / iters, "E0": float(Eg[0]), "E_final": float(Eg[-1])},
+            "cpu_fp64_oracle": {"outer_iterations": cpu_iters, "seconds": tc, "s_per_iteration": tc / cpu_iters,
+                                "threads": O.num_threads()},
+            "energy_match_rel_E0": float(np.abs(Eg[:n] - Eo[:n]).max() / Eo[0])}
 
 
 def main():
@@ -458,6 +719,7 @@ def main():
     autograd = autograd_leg(torch, gtp, dev, args.autograd_reps) if args.autograd_reps > 0 else None
     fourier = fourier_leg(torch, gtp, dev, args.fourier_reps) if args.fourier_reps > 0 else None
     extras = extras_leg(torch, gtp, dev, args.extras_reps) if args.extras_reps > 0 else None
+    gd2d = gd2d_leg(torch, gtp, dev, args.gd_iters, args.gd_cpu_iters) if args.gd_iters > 0 else None
     cpu = None
     if args.cpu_atoms > 0:
         cpu = cpu_baseline(np.float32(I).astype(float), np.float32(x_cur).astype(float),
@@ -489,6 +751,7 @@ def main():
         "autograd": autograd,
         "fourier": fourier,
         "extras": extras,
+        "gd_2d": gd2d,
     }
     print(json.dumps(out))
     return 0

@@ -43,9 +43,18 @@
  * and ddR in the reference (the dg == 0 branch, NewtonBases.py:375-380); the
  * loops below skip the derivative algebra for them, which changes nothing.
  */
-#include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <tgmath.h>
+#undef I /* complex.h's imaginary unit; I is the atom intensity here */
+
+/* Arithmetic type: double for the checker.  The CPU baseline also builds this
+ * file with -DGT_REAL=float -fsingle-precision-constant, the reference's own
+ * precision (float32 kernels, NewtonBases.py:729-1069). */
+#ifndef GT_REAL
+#define GT_REAL double
+#endif
+typedef GT_REAL real;
 
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
@@ -55,9 +64,9 @@
 static const int RIDX3[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {1, 2}, {0, 2}};
 static const int RIDX2[3][2] = {{0, 0}, {1, 1}, {0, 1}};
 
-static void fill_U(int D, const double *r, double U[3][3])
+static void fill_U(int D, const real *r, real U[3][3])
 {
-    memset(U, 0, sizeof(double) * 9);
+    memset(U, 0, sizeof(real) * 9);
     int nr = D * (D + 1) / 2;
     for (int k = 0; k < nr; k++) {
         int a = (D == 3) ? RIDX3[k][0] : RIDX2[k][0];
@@ -67,15 +76,15 @@ static void fill_U(int D, const double *r, double U[3][3])
 }
 
 /* __g: NewtonBases.py:40-46 */
-static inline double prof_g(double n) { return n > 1.0 ? 0.0 : exp(-30.0 * n) * sqrt(30.0 / M_PI); }
+static inline real prof_g(real n) { return n > 1.0 ? 0.0 : exp(-30.0 * n) * sqrt(30.0 / M_PI); }
 
 /* __f: NewtonBases.py:237-257 (D=3), NewtonBases2D.py:16-34 (D=2) */
-static double atom_f(int D, double I, const double *x, const double U[3][3], const double *y)
+static real atom_f(int D, real I, const real *x, const real U[3][3], const real *y)
 {
-    double Y[3], n = 0.0;
+    real Y[3], n = 0.0;
     for (int i = 0; i < D; i++) Y[i] = y[i] - x[i];
     for (int i = 0; i < D; i++) {
-        double s = 0.0;
+        real s = 0.0;
         for (int j = i; j < D; j++) s += U[i][j] * Y[j];
         n += s * s;
     }
@@ -85,19 +94,19 @@ static double atom_f(int D, double I, const double *x, const double U[3][3], con
 /* Shared front half of __Rf and __dRf: Y, rY = U Y, rT = U T / |U T|, n = 1/|U T|,
  * IP = rT.rY, MY = rY - rT IP, m = |MY|^2.  NewtonBases.py:267-308. */
 struct ray_geom {
-    double Y[3], T[3], rY[3], rT[3], MY[3], n, IP, m;
+    real Y[3], T[3], rY[3], rT[3], MY[3], n, IP, m;
 };
 
-static void ray_setup(int D, const double *x, const double U[3][3], const double *y,
-                      const double *T, struct ray_geom *g)
+static void ray_setup(int D, const real *x, const real U[3][3], const real *y,
+                      const real *T, struct ray_geom *g)
 {
     for (int i = 0; i < 3; i++) {
         g->Y[i] = (i < D) ? y[i] - x[i] : 0.0;
         g->T[i] = (i < D) ? T[i] : 0.0;
     }
-    double nn = 0.0;
+    real nn = 0.0;
     for (int i = 0; i < D; i++) {
-        double sy = 0.0, st = 0.0;
+        real sy = 0.0, st = 0.0;
         for (int j = i; j < D; j++) {
             sy += U[i][j] * g->Y[j];
             st += U[i][j] * g->T[j];
@@ -120,8 +129,8 @@ static void ray_setup(int D, const double *x, const double U[3][3], const double
 }
 
 /* __Rf: I * Rg(m) * n.  NewtonBases.py:260-310, NewtonBases2D.py:38-72 */
-static double atom_Rf(int D, double I, const double *x, const double U[3][3], const double *y,
-                      const double *T)
+static real atom_Rf(int D, real I, const real *x, const real U[3][3], const real *y,
+                      const real *T)
 {
     struct ray_geom g;
     ray_setup(D, x, U, y, T, &g);
@@ -131,9 +140,9 @@ static double atom_Rf(int D, double I, const double *x, const double U[3][3], co
 /* __dRf_aniso / __dRf: value, gradient and Hessian of R w.r.t. [I, x, r].
  * NewtonBases.py:313-527, NewtonBases2D.py:76-242.  Returns 0 (and leaves the
  * outputs untouched) if the ray is outside the cutoff. */
-static int atom_dRf(int D, double I, const double *x, const double *r, const double U[3][3],
-                    const double *y, const double *T, int order, double *R, double *dR,
-                    double *ddR /* P x P row-major */)
+static int atom_dRf(int D, real I, const real *x, const real *r, const real U[3][3],
+                    const real *y, const real *T, int order, real *R, real *dR,
+                    real *ddR /* P x P row-major */)
 {
     const int nr = D * (D + 1) / 2, P = 1 + D + nr;
     struct ray_geom g;
@@ -141,12 +150,12 @@ static int atom_dRf(int D, double I, const double *x, const double *r, const dou
     if (g.m > 1.0) return 0;
     (void)r;
 
-    const double *MY = g.MY, *rT = g.rT, *Y = g.Y, *Tv = g.T;
-    const double n = g.n, IP = g.IP;
-    double dg0 = exp(-30.0 * g.m), dg1 = -30.0 * dg0, dg2 = 900.0 * dg0; /* __dRg :59-71 */
+    const real *MY = g.MY, *rT = g.rT, *Y = g.Y, *Tv = g.T;
+    const real n = g.n, IP = g.IP;
+    real dg0 = exp(-30.0 * g.m), dg1 = -30.0 * dg0, dg2 = 900.0 * dg0; /* __dRg :59-71 */
 
-    double dMydT[3][3], dJdy[3], dJdT[3];
-    double ddJdyy[3][3], ddJdyT[3][3], ddJdTT[3][3];
+    real dMydT[3][3], dJdy[3], dJdT[3];
+    real ddJdyy[3][3], ddJdyT[3][3], ddJdTT[3][3];
     for (int i = 0; i < D; i++)
         for (int j = 0; j < D; j++)
             dMydT[i][j] = rT[i] * (IP * rT[j] - MY[j]) - (i == j ? IP : 0.0);
@@ -156,7 +165,7 @@ static int atom_dRf(int D, double I, const double *x, const double *r, const dou
     }
     for (int i = 0; i < D; i++)
         for (int j = 0; j < D; j++) {
-            double d = (i == j) ? 1.0 : 0.0;
+            real d = (i == j) ? 1.0 : 0.0;
             ddJdyy[i][j] = 2.0 * n * (2.0 * MY[i] * MY[j] * dg2 + (d - rT[i] * rT[j]) * dg1);
             ddJdyT[i][j] = 2.0 * n * n * ((dMydT[i][j] - MY[i] * rT[j]) * dg1 -
                                           2.0 * IP * MY[i] * MY[j] * dg2);
@@ -164,8 +173,8 @@ static int atom_dRf(int D, double I, const double *x, const double *r, const dou
     /* ddJdTT: the reference evaluates its (i<=j) formula and mirrors it. */
     for (int i = 0; i < D; i++)
         for (int j = i; j < D; j++) {
-            double d = (i == j) ? 1.0 : 0.0;
-            double v = 2.0 * n * n * n *
+            real d = (i == j) ? 1.0 : 0.0;
+            real v = 2.0 * n * n * n *
                        ((1.5 * rT[i] * rT[j] - 0.5 * d) * dg0 +
                         (2.0 * IP * MY[i] * rT[j] + IP * rT[i] * MY[j] - IP * dMydT[i][j] -
                          MY[i] * MY[j]) * dg1 +
@@ -175,14 +184,14 @@ static int atom_dRf(int D, double I, const double *x, const double *r, const dou
         }
 
     const int (*RIDX)[2] = (D == 3) ? RIDX3 : RIDX2;
-    memset(dR, 0, sizeof(double) * P);
-    memset(ddR, 0, sizeof(double) * P * P);
+    memset(dR, 0, sizeof(real) * P);
+    memset(ddR, 0, sizeof(real) * P * P);
 
     *R = I * dg0 * n;
     dR[0] = dg0 * n;
     /* d/dI d/dx: -sum_l U[l][a] dJdy[l] */
     for (int a = 0; a < D; a++) {
-        double s = 0.0;
+        real s = 0.0;
         for (int l = 0; l < D; l++) s += U[l][a] * dJdy[l];
         ddR[0 * P + 1 + a] = -s;
     }
@@ -208,10 +217,10 @@ static int atom_dRf(int D, double I, const double *x, const double *r, const dou
         for (int k = 0; k < nr; k++) {
             int i0 = RIDX[k][0], i1 = RIDX[k][1];
             for (int a = 0; a < D; a++) {
-                double s = 0.0;
+                real s = 0.0;
                 for (int l = 0; l < D; l++)
                     s += (ddJdyy[i0][l] * Y[i1] + ddJdyT[l][i0] * Tv[i1]) * U[l][a];
-                double v = -I * s;
+                real v = -I * s;
                 if (a == i1) v -= I * dJdy[i0];
                 ddR[(1 + a) * P + 1 + D + k] = v;
             }
@@ -219,7 +228,7 @@ static int atom_dRf(int D, double I, const double *x, const double *r, const dou
         /* x-x block */
         for (int a = 0; a < D; a++)
             for (int b = a; b < D; b++) {
-                double s = 0.0;
+                real s = 0.0;
                 for (int k = 0; k < D; k++)
                     for (int l = 0; l < D; l++) s += ddJdyy[k][l] * U[k][a] * U[l][b];
                 ddR[(1 + a) * P + 1 + b] = I * s;
@@ -234,8 +243,8 @@ static int atom_dRf(int D, double I, const double *x, const double *r, const dou
  * kind 3: y = p0 w0[a] + p1 w1[a]                (NewtonBases.py:676-680)
  * kind 25: y = (p0 w0[a,0], p0 w0[a,1], p1)      (NewtonBases.py:684-688)
  * kind 2: y = p0 w0[a]                            (NewtonBases2D.py:246-249) */
-static inline void tovec(int kind, double p0, double p1, const double *w0, const double *w1,
-                         double *y)
+static inline void tovec(int kind, real p0, real p1, const real *w0, const real *w1,
+                         real *y)
 {
     if (kind == 3) {
         for (int i = 0; i < 3; i++) y[i] = p0 * w0[i] + p1 * w1[i];
@@ -257,22 +266,22 @@ static int geom_kind(int D, int lenT) { return D == 2 ? 2 : (lenT == 3 ? 3 : 25)
  * D: 2 or 3.  r: N x (D(D+1)/2) anisotropic slots.  T, w0: nA x lenT, w1: nA x 3 (3D only).
  * n1 = 1 for D = 2.
  */
-void or_radproj(int D, int N, const double *I, const double *x, const double *r, int nA, int lenT,
-                const double *T, const double *w0, const double *w1, int n0, const double *p0,
-                int n1, const double *p1, double *R)
+void or_radproj(int D, int N, const real *I, const real *x, const real *r, int nA, int lenT,
+                const real *T, const real *w0, const real *w1, int n0, const real *p0,
+                int n1, const real *p1, real *R)
 {
     const int nr = D * (D + 1) / 2, kind = geom_kind(D, lenT);
-    double(*Us)[3][3] = malloc(sizeof(double[3][3]) * (N > 0 ? N : 1));
+    real(*Us)[3][3] = malloc(sizeof(real[3][3]) * (N > 0 ? N : 1));
     for (int j = 0; j < N; j++) fill_U(D, r + (size_t)j * nr, Us[j]);
 #pragma omp parallel for collapse(2) schedule(static)
     for (int a = 0; a < nA; a++)
         for (int k0 = 0; k0 < n0; k0++) {
-            double T3[3] = {T[a * lenT], T[a * lenT + 1], lenT == 3 ? T[a * lenT + 2] : 0.0};
+            real T3[3] = {T[a * lenT], T[a * lenT + 1], lenT == 3 ? T[a * lenT + 2] : 0.0};
             for (int k1 = 0; k1 < n1; k1++) {
-                double y[3];
+                real y[3];
                 tovec(kind, p0[k0], D == 3 ? p1[k1] : 0.0, w0 + (size_t)a * lenT,
                       w1 ? w1 + (size_t)a * 3 : NULL, y);
-                double s = 0.0;
+                real s = 0.0;
                 for (int j = 0; j < N; j++) s += atom_Rf(D, I[j], x + (size_t)j * D, Us[j], y, T3);
                 R[((size_t)a * n0 + k0) * n1 + k1] = s;
             }
@@ -287,26 +296,26 @@ void or_radproj(int D, int N, const double *I, const double *x, const double *r,
  * NewtonBases.py:792-870 (L2), 993-1069 (linear); 2D at NewtonBases2D.py:329-401, 444-515.
  * Outputs f[N], Df[N*P], DDf[N*P*P].
  */
-void or_derivs(int D, int mode, int order, int N, const double *I, const double *x, const double *r,
-               int nA, int lenT, const double *T, const double *w0, const double *w1, int n0,
-               const double *p0, int n1, const double *p1, const double *C, double *f, double *Df,
-               double *DDf)
+void or_derivs(int D, int mode, int order, int N, const real *I, const real *x, const real *r,
+               int nA, int lenT, const real *T, const real *w0, const real *w1, int n0,
+               const real *p0, int n1, const real *p1, const real *C, real *f, real *Df,
+               real *DDf)
 {
     const int nr = D * (D + 1) / 2, P = 1 + D + nr, kind = geom_kind(D, lenT);
 #pragma omp parallel for schedule(dynamic, 1)
     for (int j = 0; j < N; j++) {
-        double U[3][3];
+        real U[3][3];
         fill_U(D, r + (size_t)j * nr, U);
-        double F = 0.0, dR[10], ddR[100];
-        double *G = Df + (size_t)j * P, *H = DDf + (size_t)j * P * P;
-        memset(G, 0, sizeof(double) * P);
-        memset(H, 0, sizeof(double) * P * P);
+        real F = 0.0, dR[10], ddR[100];
+        real *G = Df + (size_t)j * P, *H = DDf + (size_t)j * P * P;
+        memset(G, 0, sizeof(real) * P);
+        memset(H, 0, sizeof(real) * P * P);
         for (int a = 0; a < nA; a++) {
-            double T3[3] = {T[a * lenT], T[a * lenT + 1], lenT == 3 ? T[a * lenT + 2] : 0.0};
+            real T3[3] = {T[a * lenT], T[a * lenT + 1], lenT == 3 ? T[a * lenT + 2] : 0.0};
             for (int k0 = 0; k0 < n0; k0++)
                 for (int k1 = 0; k1 < n1; k1++) {
-                    double y[3], Rv;
-                    double c = C[((size_t)a * n0 + k0) * n1 + k1];
+                    real y[3], Rv;
+                    real c = C[((size_t)a * n0 + k0) * n1 + k1];
                     tovec(kind, p0[k0], D == 3 ? p1[k1] : 0.0, w0 + (size_t)a * lenT,
                           w1 ? w1 + (size_t)a * 3 : NULL, y);
                     int in = atom_dRf(D, I[j], x + (size_t)j * D, r + (size_t)j * nr, U, y, T3,
@@ -316,7 +325,7 @@ void or_derivs(int D, int mode, int order, int N, const double *I, const double 
                             F += c * c;
                             continue;
                         }
-                        double res = Rv - c;
+                        real res = Rv - c;
                         F += res * res;
                         for (int p = 0; p < P; p++) {
                             G[p] += res * dR[p];
@@ -339,19 +348,19 @@ void or_derivs(int D, int mode, int order, int N, const double *I, const double 
 }
 
 /* Volume render u[j0,j1(,j2)] = sum_atoms __f.  NewtonBases.py:698-711, NewtonBases2D.py:259-271 */
-void or_volproj(int D, int N, const double *I, const double *x, const double *r, int m0,
-                const double *y0, int m1, const double *y1, int m2, const double *y2, double *u)
+void or_volproj(int D, int N, const real *I, const real *x, const real *r, int m0,
+                const real *y0, int m1, const real *y1, int m2, const real *y2, real *u)
 {
     const int nr = D * (D + 1) / 2;
     if (D == 2) m2 = 1;
-    double(*Us)[3][3] = malloc(sizeof(double[3][3]) * (N > 0 ? N : 1));
+    real(*Us)[3][3] = malloc(sizeof(real[3][3]) * (N > 0 ? N : 1));
     for (int j = 0; j < N; j++) fill_U(D, r + (size_t)j * nr, Us[j]);
 #pragma omp parallel for collapse(2) schedule(static)
     for (int a = 0; a < m0; a++)
         for (int b = 0; b < m1; b++)
             for (int c = 0; c < m2; c++) {
-                double y[3] = {y0[a], y1[b], D == 3 ? y2[c] : 0.0};
-                double s = 0.0;
+                real y[3] = {y0[a], y1[b], D == 3 ? y2[c] : 0.0};
+                real s = 0.0;
                 for (int j = 0; j < N; j++) s += atom_f(D, I[j], x + (size_t)j * D, Us[j], y);
                 u[((size_t)a * m1 + b) * m2 + c] = s;
             }
@@ -368,36 +377,36 @@ void or_volproj(int D, int N, const double *I, const double *x, const double *r,
  * iso != 0 selects the isotropic form (r has one slot per atom).
  * out is interleaved complex (re, im).
  */
-void or_ft_eval(int D, int iso, int N, const double *I, const double *x, const double *r,
-                long npts, const double *K, double *out)
+void or_ft_eval(int D, int iso, int N, const real *I, const real *x, const real *r,
+                long npts, const real *K, real *out)
 {
     const int nr = iso ? 1 : D * (D + 1) / 2;
-    const double cut = (D == 3) ? 30.0 : 60.0;
+    const real cut = (D == 3) ? 30.0 : 60.0;
 #pragma omp parallel for schedule(static)
     for (long p = 0; p < npts; p++) {
-        const double *k = K + p * D;
-        double re = 0.0, im = 0.0;
+        const real *k = K + p * D;
+        real re = 0.0, im = 0.0;
         for (int j = 0; j < N; j++) {
-            const double *rr = r + (size_t)j * nr;
-            double norm;
+            const real *rr = r + (size_t)j * nr;
+            real norm;
             if (iso) {
-                double k2 = 0.0;
+                real k2 = 0.0;
                 for (int i = 0; i < D; i++) k2 += k[i] * k[i];
                 norm = (D == 3) ? k2 / (rr[0] * rr[0]) : rr[0] * rr[0] * k2;
             } else {
-                double U[3][3];
+                real U[3][3];
                 fill_U(D, rr, U);
                 norm = 0.0;
                 for (int i = 0; i < D; i++) { /* (U^T k)_i = sum_{l<=i} U[l][i] k_l */
-                    double s = 0.0;
+                    real s = 0.0;
                     for (int l = 0; l <= i; l++) s += U[l][i] * k[l];
                     norm += s * s;
                 }
             }
             if (norm < cut) {
-                double ip = 0.0;
+                real ip = 0.0;
                 for (int i = 0; i < D; i++) ip += x[(size_t)j * D + i] * k[i];
-                double amp = I[j] * exp(-0.5 * norm);
+                real amp = I[j] * exp(-0.5 * norm);
                 re += amp * cos(ip);
                 im -= amp * sin(ip);
             }

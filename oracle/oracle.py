@@ -67,6 +67,71 @@ def num_threads() -> int:
     return int(lib().or_num_threads())
 
 
+class CpuBaseline:
+    """The dense reference-semantics loops of gt_oracle.c compiled for the
+    host it runs on (gcc -O3 -march=native -fopenmp), in FP64 or in FP32 (the
+    reference kernels' float32, with single-precision constants), into
+    `dest_dir`.  Timed by the benchmark as the CPU baseline; the checker
+    build above stays FP64 at -O2."""
+
+    def __init__(self, dest_dir, bits=64):
+        src = os.path.join(_HERE, "gt_oracle.c")
+        self.bits = int(bits)
+        self.path = os.path.join(dest_dir, f"libgt_cpu{self.bits}.so")
+        flags = ["-O3", "-march=native", "-std=c11", "-fopenmp", "-fPIC", "-shared"]
+        if self.bits == 32:
+            flags += ["-DGT_REAL=float", "-fsingle-precision-constant"]
+        res = subprocess.run(["gcc", *flags, "-o", self.path, src, "-lm"], capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(res.stderr)
+        self.flags = " ".join(flags)
+        self.lib = ctypes.CDLL(self.path)
+        self.dt = np.float32 if self.bits == 32 else np.float64
+        rp = ctypes.POINTER(ctypes.c_float if self.bits == 32 else ctypes.c_double)
+        ci = ctypes.c_int
+        self.lib.or_radproj.argtypes = [ci, ci, rp, rp, rp, ci, ci, rp, rp, rp, ci, rp, ci, rp, rp]
+        self.lib.or_derivs.argtypes = [ci, ci, ci, ci, rp, rp, rp, ci, ci, rp, rp, rp, ci, rp, ci, rp, rp, rp, rp, rp]
+        self.lib.or_num_threads.restype = ci
+        self._rp = rp
+
+    def threads(self):
+        return int(self.lib.or_num_threads())
+
+    def _a(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dt)
+        return a, a.ctypes.data_as(self._rp)
+
+    def _geom(self, g):
+        T, w0 = self._a(g.orientations), self._a(g.ortho[0])
+        w1 = self._a(g.ortho[1]) if len(g.ortho) > 1 else (None, None)
+        p0 = self._a(g.detector[0])
+        p1 = self._a(g.detector[1]) if len(g.detector) > 1 else self._a(np.zeros(1))
+        return T, w0, w1, p0, p1
+
+    def radproj(self, I, x, r, g):
+        D = g.dim
+        (I_, pI), (x_, px), (r_, pr) = self._a(I), self._a(x), self._a(r)
+        T, w0, w1, p0, p1 = self._geom(g)
+        nA, lenT = T[0].shape
+        n0, n1 = p0[0].size, (p1[0].size if D == 3 else 1)
+        R, pR = self._a(np.zeros((nA, n0, n1)))
+        self.lib.or_radproj(D, I_.size, pI, px, pr, nA, lenT, T[1], w0[1], w1[1], n0, p0[1], n1, p1[1], pR)
+        return R
+
+    def derivs(self, I, x, r, g, C, mode="L2", order=2):
+        D = g.dim
+        (I_, pI), (x_, px), (r_, pr) = self._a(I), self._a(x), self._a(r)
+        T, w0, w1, p0, p1 = self._geom(g)
+        nA, lenT = T[0].shape
+        n0, n1 = p0[0].size, (p1[0].size if D == 3 else 1)
+        N, P = I_.size, 1 + D + r_.size // max(I_.size, 1)
+        C_, pC = self._a(C)
+        (f, pf), (G, pG), (H, pH) = self._a(np.zeros(N)), self._a(np.zeros((N, P))), self._a(np.zeros((N, P, P)))
+        self.lib.or_derivs(D, 1 if mode == "L2" else 0, order, N, pI, px, pr, nA, lenT, T[1], w0[1], w1[1], n0, p0[1],
+                           n1, p1[1], pC, pf, pG, pH)
+        return f, G, H
+
+
 def _p(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 

@@ -82,3 +82,25 @@ def test_fourier_vs_fft_of_render(gt, cuda, c4):
     nI, nx, nr = Fo.analytic_gauss_fourier(I, x, rU)
     ana = Fo.evaluate(Fo.FTAtoms(nI, nx, nr, cuda), [m.astype(float)] * 3).cpu().numpy()
     assert np.abs(Fk - ana).max() < 2e-3 * np.abs(ana).max()
+
+
+def test_fourier_many_atoms_in_atom_order(gt, oracle, cuda):
+    """More atoms than the evaluator ranks by support radius (16384): the
+    records stay in atom order and every patch scans them all; same values
+    as the FP64 oracle.  A non-ascending grid vector (no per-patch culling)
+    gives the same values as the ascending one, reordered."""
+    Fo = importlib.import_module("2018_dictionary_tomography_amd.fourier")
+    W = importlib.import_module("2018_dictionary_tomography_amd.workloads")
+    rng = np.random.default_rng(12)
+    I, x, r, _ = W.draw_aniso3(17000, rng)
+    nI, nx, nr = Fo.analytic_gauss_fourier(I, x, r * np.sqrt(60.0))
+    k = [np.arange(-12, 12, dtype=np.float64), np.arange(-8, 9, dtype=np.float64), np.arange(-10, 10, dtype=np.float64)]
+    atoms = Fo.FTAtoms(nI, nx, nr, cuda)
+    out = Fo.evaluate(atoms, k).cpu().numpy()
+    mesh = np.stack(np.meshgrid(*[f32(v) for v in k], indexing="ij"), -1).reshape(-1, 3)
+    ref = oracle.ft_points(f32(nI), f32(nx), f32(nr), mesh).reshape(out.shape)
+    assert np.abs(out - ref).max() < 1e-5 * np.abs(ref).max()
+    perm = rng.permutation(k[2].size)
+    out_p = Fo.evaluate(Fo.FTAtoms(nI[:3000], nx[:3000], nr[:3000], cuda), [k[0], k[1], k[2][perm]]).cpu().numpy()
+    out_s = Fo.evaluate(Fo.FTAtoms(nI[:3000], nx[:3000], nr[:3000], cuda), k).cpu().numpy()
+    assert np.abs(out_p - out_s[:, :, perm]).max() < 1e-5 * np.abs(out_s).max()
