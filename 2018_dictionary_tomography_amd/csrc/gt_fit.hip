@@ -43,6 +43,8 @@
 // the moments of its angle as soon as it has reduced them (the FP64 work of
 // one pair spread over the wave's lanes, see prep1..prep3), so the latency of
 // that FP64 chain overlaps the pixel work of the other waves.
+#include <mutex>
+
 #include "gt_pixel.h"
 #include "gt_reg.h"
 
@@ -1023,11 +1025,16 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
     }
 }
 
+// Fit workgroups the current device holds at once, queried once per device.
+// The per-device cache is written under a mutex: host threads may launch
+// fits concurrently.
 int max_blocks_cached()
 {
+    static std::mutex mu;
     static int cache[64];
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
+    std::lock_guard<std::mutex> lock(mu);
     if (cache[dev] > 0) return cache[dev];
     int cus = 0, per3 = 0, per2 = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;

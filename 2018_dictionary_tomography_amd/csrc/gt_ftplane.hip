@@ -432,7 +432,7 @@ __device__ inline float kd_sum(float v)
     return v;
 }
 
-// One workgroup per atom; wave w takes angles w, w + 16, ...  csum[a] =
+// One workgroup per atom; wave w takes angles w, w + KD_WAVES (8), ...  csum[a] =
 // sum over angle a of |C|^2 (FP64).
 // At most 128 VGPRs (amdgpu_waves_per_eu(4): two blocks of 8 waves per CU):
 // 2.3 ms against 2.7 ms at the 207 VGPRs of one block per CU, despite 260

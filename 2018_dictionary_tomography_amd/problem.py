@@ -80,23 +80,31 @@ class UniformPartition:
 
 
 def _padshape(arr, shape):
-    """GaussDictCode/__init__.py:223-244: broadcast a scalar / per-axis value
-    to `shape` (an int, or a list of ints for nested values)."""
-    if np.isscalar(arr):
-        arr = [arr]
-    if np.isscalar(shape):
-        shape = [shape]
-    if len(shape) == 1:
-        if len(arr) == 1:
-            return (arr[0],) * shape[0]
-        if len(arr) == shape[0]:
-            return arr
-        raise ValueError("If <shape> is scalar/length 1 then <arr> must be scalar, length 1 or length <shape>.")
-    if len(arr) == 1:
-        return [(arr[0],) * s for s in shape]
-    if len(arr) == len(shape):
-        return [_padshape(arr[i], shape[i]) for i in range(len(arr))]
-    raise ValueError("If <shape> is a vector then either <arr> is scalar or the same length of <shape>.")
+    """Broadcast a value over a shape, the rule the problem factory applies to
+    its box and size arguments (GaussDictCode/__init__.py:223-244):
+
+    * shape a single length n (an int or a one-entry list): a value with one
+      entry (or a scalar) becomes n copies of it; a value with n entries is
+      kept as given; any other length is an error;
+    * shape a list of lengths [n0, n1, ...]: a value with one entry becomes
+      one tuple of n_i copies per length; a value with one entry per length
+      is broadcast entry by entry; any other length is an error."""
+    lengths = [shape] if np.isscalar(shape) else list(shape)
+    entries = [arr] if np.isscalar(arr) else list(arr)
+    one = len(entries) == 1
+    if len(lengths) == 1:
+        n = lengths[0]
+        if one:
+            return (entries[0],) * n
+        if len(entries) != n:
+            raise ValueError(f"cannot broadcast {len(entries)} values to length {n}: give one value or {n}")
+        return arr
+    if one:
+        return [(entries[0],) * n for n in lengths]
+    if len(entries) != len(lengths):
+        raise ValueError(f"cannot broadcast {len(entries)} values over {len(lengths)} axes: give one value or one "
+                         "per axis")
+    return [_padshape(v, n) for v, n in zip(entries, lengths)]
 
 
 def getGeom(vol_box, vol_size, gt, dim, angles, angle_num, angle_range, det_box, det_size, data):

@@ -56,19 +56,26 @@ def test_argument_validation_errors(gt):
 
 def test_atom_space_random_matches_reference_draw_order():
     """AtomSpace.random draws r, then x, then I from numpy's legacy generator
-    (dictionary_def.py:130-139)."""
+    (dictionary_def.py:130-139): r ~ U[1, 2) with the off-diagonal slots
+    divided by 4, x ~ U[-0.8, 0.8), I ~ U[0, 2); the seed-3 stream written
+    out by hand below."""
     sp = importlib.import_module("2018_dictionary_tomography_amd.spaces")
-    rs = np.random.RandomState(3)
-    r = rs.rand(4, 6) + 1
-    r[:, 3:] /= 4
-    x = 0.8 * (2 * rs.rand(4, 3) - 1)
-    I = 2 * rs.rand(4)
-    space = sp.AtomSpace(3, isotropic=False)
-    # reproduce the draw on the host only (no device tensors)
-    rs2 = np.random.RandomState(3)
-    r2 = rs2.rand(4, space.nr) + 1
-    r2[:, space.dim:] /= 4
-    assert np.allclose(r, r2) and np.allclose(x, 0.8 * (2 * rs2.rand(4, 3) - 1)) and np.allclose(I, 2 * rs2.rand(4))
+    torch = importlib.import_module("torch")
+    for iso, D in ((False, 3), (True, 3), (False, 2), (True, 2)):
+        space = sp.AtomSpace(D, isotropic=iso)
+        a = space.random(4, seed=3, device=torch.device("cpu"))
+        rs = np.random.RandomState(3)
+        nr = 1 if iso else D * (D + 1) // 2
+        r = rs.rand(4, nr) + 1
+        if not iso:
+            r[:, D:] /= 4
+        x = 0.8 * (2 * rs.rand(4, D) - 1)
+        I = 2 * rs.rand(4)
+        assert a.r.shape == (4, nr) and a.x.shape == (4, D) and a.I.shape == (4,)
+        assert np.array_equal(a.r.numpy(), np.float32(r)) and np.array_equal(a.x.numpy(), np.float32(x))
+        assert np.array_equal(a.I.numpy(), np.float32(I))
+    # the first draws of the stream itself (numpy legacy MT19937, seed 3)
+    assert np.allclose(np.random.RandomState(3).rand(3), [0.5507979, 0.70814782, 0.29090474])
 
 
 def test_geometry_helpers():
@@ -84,6 +91,23 @@ def test_geometry_helpers():
     assert ps.shape == [7, 5, 3] and ps.dim == 3 and ps.orientations.shape == (7, 2)
     ps2 = sp.ProjSpace(angles=np.linspace(0, np.pi, 7), detector=np.linspace(-1, 1, 5))
     assert ps2.shape == [7, 5] and ps2.dim == 2
+
+
+def test_product_geometry_equals_pinned_oracle_geometry(oracle):
+    """The product's theta_to_vec / perp (what every GPU test and the drop-in
+    ProjSpace feed the kernels) equal the oracle's restatements, which
+    tests/test_oracle.py pins (orthonormal frames, the explicit forms of
+    atomFuncs.py:13-74), for one angle vector (2D) and two (3D Euler pairs)."""
+    sp = importlib.import_module("2018_dictionary_tomography_amd.spaces")
+    rng = np.random.default_rng(2)
+    for theta in ([rng.uniform(-3, 3, 11)], [rng.uniform(-3, 3, 5), rng.uniform(-1.5, 1.5, 4)]):
+        T = sp.theta_to_vec(theta)
+        To = oracle.theta_to_vec(theta)
+        assert np.allclose(np.asarray(T).reshape(To.shape), To, rtol=0, atol=1e-15)
+        W, Wo = sp.perp(theta), oracle.perp(theta)
+        assert len(W) == len(Wo)
+        for w, wo in zip(W, Wo):
+            assert np.allclose(np.asarray(w).reshape(wo.shape), wo, rtol=0, atol=1e-15)
 
 
 def test_workload_cholesky_convention():
