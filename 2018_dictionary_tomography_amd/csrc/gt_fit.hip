@@ -53,11 +53,13 @@ namespace gt {
 constexpr int FIT_THREADS = 1024;
 constexpr int FIT_WAVES = FIT_THREADS / 64;
 constexpr int NAMAX = GT_FIT_MAX_ANGLES;
+static_assert(NAMAX % 64 == 0, "per-angle sums take whole waves of angles");
 constexpr int DETMAX = GT_FIT_MAX_DETECTOR;
 constexpr int NVS = 65;                          // stride of a per-angle gradient/Hessian record
 constexpr int FIT_UNR = 8;                       // detector rows in flight per lane in the pixel passes
 constexpr int FIT_TROWS = 256;                   // rows of a wave's row table (moments pass)
 constexpr double FIT_EPS = 1e-4;                 // NewtonGaussian.py:21
+constexpr double GD_EPS = 1e-8;                  // GD_lib.py:96
 constexpr long long FIT_WAIT_TICKS = 200000000;  // barrier wait limit: 2 s of the 100 MHz clock
 
 // Slot scratch layout (bytes): per-angle records double[nA][NVS] | per-angle
@@ -538,18 +540,30 @@ __device__ inline bool group_barrier(unsigned *bar, int G, unsigned &gen)
 // (NewtonGaussian.py:50-98, :131-222).
 __device__ inline void fit_advance(FitCtl &c, const gt_fit_slot &S, bool skipped)
 {
-    bool next = skipped;
-    if (!skipped) {
+    if (S.block == 2) {
+        // block GD: an inner iteration takes every block of the atom in turn
+        // (GD_lib.py:133-170); BREAK[1] holds across its blocks
+        if (++c.blk < S.gd_nblk) return;
+        c.blk = 0;
         c.inner++;
-        next = c.brk1 || c.inner >= S.max_inner;
-    }
-    if (next) {
-        c.inner = 0;
-        if (S.block && ++c.blk < 3) {
-            // next block of the same atom
-        } else {
-            c.blk = 0;
+        if (c.brk1 || c.inner >= S.max_inner) {
+            c.inner = 0;
             c.pos++;
+        }
+    } else {
+        bool next = skipped;
+        if (!skipped) {
+            c.inner++;
+            next = c.brk1 || c.inner >= S.max_inner;
+        }
+        if (next) {
+            c.inner = 0;
+            if (S.block && ++c.blk < 3) {
+                // next block of the same atom
+            } else {
+                c.blk = 0;
+                c.pos++;
+            }
         }
     }
     c.brk1 = 1;  // BREAK[1] = True at the start of every inner iteration
@@ -739,8 +753,9 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         }
         GT_FIT_TICK(1);
         GT_FIT_MEET();
-        // sum the per-angle records over all angles in one fixed order (angle
-        // a and a + 64 on lane a, then the wave butterfly); all loads first
+        // sum the per-angle records over all angles in one fixed order (angles
+        // a, a + 64, a + 128, a + 192 on lane a, then the wave butterfly); all
+        // loads first
         {
             constexpr int NE = (NV + FIT_WAVES - 1) / FIT_WAVES;
             double v[NE];
@@ -749,8 +764,9 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                 const int e = wave + q * FIT_WAVES;
                 double t = 0.0;
                 if (e < NV) {
-                    if (lane < nA) t = xget(prec + (size_t)lane * NVS + e);
-                    if (lane + 64 < nA) t += xget(prec + (size_t)(lane + 64) * NVS + e);
+#pragma unroll
+                    for (int u = 0; u < NAMAX / 64; u++)
+                        if (lane + 64 * u < nA) t += xget(prec + (size_t)(lane + 64 * u) * NVS + e);
                 }
                 v[q] = t;
             }
@@ -802,7 +818,31 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
             L.b[tid] = -gv;
         }
         __syncthreads();
-        if (wave == 0) {
+        if (wave == 0 && S.block == 2) {
+            // block gradient descent (GD_lib.py:135-166): block t of the
+            // dims order, its gradient g_t (+ the regulariser's), step size
+            // h = min(h, 1 / max(|g_t|, 1e-8)), T = old - h g_t, no projection
+            const int t = S.gd_blocks[L.c.blk];
+            const int o = t == 0 ? 0 : (t == 1 ? 1 : 1 + D), n = t == 0 ? 1 : (t == 1 ? D : PI - 1 - D);
+            if (lane == 0) {
+                double g2 = 0.0;
+                for (int i = 0; i < n; i++) g2 += L.b[o + i] * L.b[o + i];
+                const double h = fmin(L.hj, 1.0 / fmax(sqrt(g2), GD_EPS));
+                L.hj = h;
+                double *th = L.th_new;
+                for (int i = 0; i < 1 + D + NRs; i++) th[i] = L.th_old[i];
+                for (int i = 0; i < n; i++) th[o + i] = (double)(float)(L.th_old[o + i] + h * L.b[o + i]);
+                // norm(old - T) > tol * norm(old) over the block (GD_lib.py:163)
+                double dn = 0.0, on = 0.0;
+                for (int i = o; i < o + n; i++) {
+                    const double d = th[i] - L.th_old[i];
+                    dn += d * d;
+                    on += L.th_old[i] * L.th_old[i];
+                }
+                L.moved = sqrt(dn) > S.tol * sqrt(on);
+                L.rv_new = rk != GT_REG_NULL ? reg_eval<D>(rk, S.reg_p, iso, th, nullptr, nullptr) : 0.0;
+            }
+        } else if (wave == 0) {
             // the parameters this update moves: all, or block blk = I | x | r
             int o = 0, n = PI;
             if (S.block) {
@@ -923,8 +963,9 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         // ---- phase G: accept / reject, step size, flags -------------------
         if (wave == 0) {
             double dE = 0.0;
-            if (lane < nA) dE = xget(dEa + lane);
-            if (lane + 64 < nA) dE += xget(dEa + lane + 64);
+#pragma unroll
+            for (int u = 0; u < NAMAX / 64; u++)
+                if (lane + 64 * u < nA) dE += xget(dEa + lane + 64 * u);
             dE = wave_sum_f64(dE);
             if (lane == 0) {
                 FitCtl &c = L.c;
@@ -944,11 +985,15 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                 if (S.Fhist && member == 0) S.Fhist[c.jj + 1] = c.F + dE;
                 double h = L.hj;
                 if (!accept) {
-                    if (h > 2 * FIT_EPS) {
-                        c.brk0 = 0;
-                        c.brk1 = 0;
+                    if (S.block == 2) {
+                        h /= 2.0;  // block GD: no effect on the BREAK flags (GD_lib.py:157-160)
+                    } else {
+                        if (h > 2 * FIT_EPS) {
+                            c.brk0 = 0;
+                            c.brk1 = 0;
+                        }
+                        h /= 10.0;
                     }
-                    h /= 10.0;
                 } else {
                     c.F += dE;
                     if (rk != GT_REG_NULL) {
@@ -962,7 +1007,7 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                         c.brk0 = 0;
                         c.brk1 = 0;
                     }
-                    h *= S.block ? 1.3 : 2.0;
+                    h *= S.block == 2 ? 1.1 : (S.block ? 1.3 : 2.0);
                     // every member writes the same values: its own later reads
                     // of atom j then never depend on another member's timing
                     S.I[j] = (float)L.th_new[0];
@@ -970,7 +1015,7 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                     for (int k = 0; k < NRs; k++) S.r[(size_t)j * NRs + k] = (float)L.th_new[1 + D + k];
                     c.accepted++;
                 }
-                S.h[(size_t)c.blk * N + j] = fmin(fmax(h, FIT_EPS), 1.0 / FIT_EPS);
+                S.h[(size_t)c.blk * N + j] = S.block == 2 ? fmax(h, GD_EPS) : fmin(fmax(h, FIT_EPS), 1.0 / FIT_EPS);
                 c.jj++;
                 if (S.Ehist && member == 0) S.Ehist[c.jj] = c.E;
                 c.updates++;

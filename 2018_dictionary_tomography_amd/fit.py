@@ -9,8 +9,9 @@ newton_linesearch_block  NewtonGaussian.linesearch_block (NewtonGaussian.py:
                          112-237): the same kernel updating I, x and r of each
                          atom in turn.
 gd_linesearch_block      GD_lib.linesearch_block (GD_lib.py:82-178): block
-                         gradient descent driven from the host over the batched
-                         forward and derivative kernels.
+                         gradient descent; device-resident for the point
+                         projector (the same kernel, block = 2), driven from
+                         the host over the operator's kernels otherwise.
 newton_host              the Newton fitters driven from the host over an
                          operator's forward and derivative kernels: what the
                          drop-ins run for SingleParticleGaussTomo, whose
@@ -51,11 +52,12 @@ class gt_fit_slot(ctypes.Structure):
         ("block", ctypes.c_int), ("reg_kind", ctypes.c_int), ("reg_p", ctypes.c_double * 5),
         ("Fhist", ctypes.c_void_p),
         ("blk", ctypes.c_int), ("ninf", ctypes.c_int), ("F", ctypes.c_double), ("Rsum", ctypes.c_double),
+        ("gd_nblk", ctypes.c_int), ("gd_blocks", ctypes.c_int * 3),
     ]
 
 
 SLOT_BYTES = ctypes.sizeof(gt_fit_slot)
-MAX_ANGLES = 128
+MAX_ANGLES = 256
 MAX_DETECTOR = 1024
 
 
@@ -108,20 +110,28 @@ class FitRequest:
     """Device state of one Newton fit: atoms, residual, step sizes, history.
 
     reg: a regularisation.Regulariser (None: null); block: update I, x and r
-    of each atom in turn (NewtonGaussian.linesearch_block); perm: the atom
+    of each atom in turn (NewtonGaussian.linesearch_block); gd_dims (a
+    string over "xrI" in the reference's dims order): block gradient descent
+    over those blocks instead (GD_lib.linesearch_block); perm: the atom
     order of each outer iteration (default: permutations(N, max_outer,
-    seed)); h: starting step sizes ([3][N] for the block variant, else [N];
-    default 1); stage: give the fit a staging slab (the energy pass then
+    seed)); h: starting step sizes ([blocks][N]; default 1, 1e-2 for gradient
+    descent); stage: give the fit a staging slab (the energy pass then
     commits at once and a rejected update is undone from the slab; without
     it an accepted update is committed by a second pass).  Same results
     either way."""
 
     def __init__(self, atoms, space, data, max_iter, tol=1e-4, min_iter=1, seed=1, record=True, device=None,
-                 reg=None, block=False, perm=None, h=None, stage=True):
+                 reg=None, block=False, perm=None, h=None, stage=True, gd_dims=None):
         dev = device or atoms.I.device
         self.reg = as_regulariser(reg)
+        # variant: 0 Newton, 1 block Newton, 2 block gradient descent over gd_dims
         self.block = bool(block)
-        nblk = 3 if self.block else 1
+        self.gd_blocks = [] if gd_dims is None else [{"I": 0, "x": 1, "r": 2}[c] for c in gd_dims]
+        if gd_dims is not None and not 1 <= len(self.gd_blocks) <= 3:
+            raise ValueError("gd_dims must name one to three of the blocks x, r, I")
+        self.mode = 2 if gd_dims is not None else int(self.block)
+        nblk = len(self.gd_blocks) if self.mode == 2 else (3 if self.block else 1)
+        self.nblk = nblk
         if not hasattr(max_iter, "__iter__"):
             max_iter = [max_iter, 1]
         max_iter = list(max_iter) + [1] * (2 - len(max_iter))
@@ -158,7 +168,8 @@ class FitRequest:
             raise ValueError("perm must hold max_outer permutations of range(N)")
         self.perm = torch.as_tensor(perm[: self.max_outer].copy(), device=dev).contiguous()
         if h is None:
-            self.h = torch.ones(nblk * self.N, dtype=torch.float64, device=dev)
+            h0 = 1e-2 if self.mode == 2 else 1.0  # GD_lib.py:99, NewtonGaussian.py:25,119
+            self.h = torch.full((nblk * self.N,), h0, dtype=torch.float64, device=dev)
         else:
             h = np.asarray(h, dtype=np.float64).reshape(-1)
             if h.size != nblk * self.N:
@@ -184,9 +195,12 @@ class FitRequest:
             h=self.h.data_ptr(), Ehist=self.Ehist.data_ptr() if record else None, scratch=self.scratch.data_ptr(),
             stage=self.stage.data_ptr() if stage else None,
             it=0, pos=0, inner=0, jj=0, brk0=1, brk1=1, done=0 if self.N > 0 else 1, pad1=0, E=self.E0,
-            updates=0, accepted=0, block=int(self.block), reg_kind=int(self.reg.kind),
-            Fhist=self.Fhist.data_ptr() if record else None, blk=0, ninf=self.ninf0, F=self.F0, Rsum=self.Rsum0)
+            updates=0, accepted=0, block=self.mode, reg_kind=int(self.reg.kind),
+            Fhist=self.Fhist.data_ptr() if record else None, blk=0, ninf=self.ninf0, F=self.F0, Rsum=self.Rsum0,
+            gd_nblk=len(self.gd_blocks))
         self.slot.reg_p[:] = [float(v) for v in self.reg.params()]
+        for i, b in enumerate(self.gd_blocks):
+            self.slot.gd_blocks[i] = b
 
     def slot_bytes(self):
         return bytes(self.slot)
@@ -211,16 +225,17 @@ def run_blocks(dev_table, nslots, dev_blocks, nblocks, dim, budget):
 
 
 def newton_fit(atoms, space, data, max_iter, tol=1e-4, min_iter=1, seed=1, budget=None, record=True, group=None,
-               reg=None, block=False, perm=None, h=None):
-    """Run one Newton fit to completion on the device; returns (request, final
-    slot).  group: workgroups advancing the fit together (None: default_group
-    of the angle count; 1: a single workgroup).  The result does not depend on
-    the group size."""
+               reg=None, block=False, perm=None, h=None, gd_dims=None):
+    """Run one Newton fit (or, with gd_dims, one block gradient descent) to
+    completion on the device; returns (request, final slot).  group:
+    workgroups advancing the fit together (None: default_group of the angle
+    count; 1: a single workgroup).  The result does not depend on the group
+    size."""
     req = FitRequest(atoms, space, data, max_iter, tol, min_iter, seed, record, reg=reg, block=block, perm=perm,
-                     h=h)
+                     h=h, gd_dims=gd_dims)
     dev = req.I.device
     table = torch.as_tensor(np.frombuffer(req.slot_bytes(), dtype=np.uint8).copy(), device=dev)
-    budget = budget or max(req.N * req.max_inner * (3 if req.block else 1), 1)
+    budget = budget or max(req.N * req.max_inner * req.nblk, 1)
     group = default_group(req.geom.nA) if group is None else int(group)
     group = max(1, min(group, max_blocks()))
     blocks = torch.as_tensor(block_table([group]), device=dev) if group > 1 else None
@@ -601,6 +616,8 @@ def gd_linesearch_block(recon, data, max_iter, space, dim="xrI", tol=1e-4, min_i
     n = recon.I.numel()
     if h is None:
         h = np.full((len(dims), n), 1e-2)
+    if kernels is POINT and n > 0:
+        return _gd_device(recon, data, max_iter, space, dims, tol, min_iter, seed, reg, return_F, rs, h)
     h = {c: h[t] for t, c in enumerate(dims)}  # rows of the caller's array, updated in place
     data = _device_data(data, dev)
     R = kernels.forward(recon, space)
@@ -654,6 +671,32 @@ def gd_linesearch_block(recon, data, max_iter, space, dim="xrI", tol=1e-4, min_i
     if return_F:
         return recon, np.array(E), np.array(F)
     return recon, np.array(E)
+
+
+def _gd_device(recon, data, max_iter, space, dims, tol, min_iter, seed, reg, return_F, rs, h):
+    """gd_linesearch_block on the device fitter (fit_kernel, block = 2): the
+    atom order of each outer iteration drawn from rs (or RandomState(seed)) as
+    the reference's loop draws it, rs left advanced by the outer iterations the
+    fit ran and the caller's step sizes h[t][j] updated, as the host loop
+    does."""
+    n = recon.I.numel()
+    rs = np.random.RandomState(seed) if rs is None else rs
+    state = rs.get_state()
+    perm = np.stack([rs.permutation(np.arange(n)) for _ in range(max_iter[0])])
+    dev_data = _device_data(data, recon.I.device)
+    req, s = newton_fit(recon, space, dev_data, max_iter, tol, min_iter, reg=reg, perm=perm, h=h, gd_dims=dims)
+    used = max_iter[0] if s.it >= max_iter[0] else s.it + 1
+    rs.set_state(state)
+    for _ in range(used):
+        rs.permutation(np.arange(n))
+    h[...] = req.h.reshape(len(dims), n).cpu().numpy()
+    recon.I.copy_(req.I)
+    recon.x.copy_(req.x)
+    recon.r.copy_(req.r)
+    E = req.Ehist[: s.jj + 1].cpu().numpy()
+    if return_F:
+        return recon, E, req.Fhist[: s.jj + 1].cpu().numpy()
+    return recon, E
 
 
 def linesearch_block(recon, data, max_iter, fidelity=None, reg=None, Radon=None, view=None, dim="xrI", guess=None,

@@ -318,7 +318,10 @@ typedef struct gt_fit_slot {
     /* variant and regulariser (read-only) */
     int block;               /* 0: one Newton step in all parameters per update (linesearch);
                                 1: I, then x, then r of the atom, each with its own step size,
-                                projection and inner loop (linesearch_block) */
+                                projection and inner loop (linesearch_block);
+                                2: block gradient descent (GD_lib.linesearch_block): the
+                                gd_nblk blocks gd_blocks[] in turn inside each inner iteration,
+                                h[gd_nblk][N] from 1e-2 */
     int reg_kind;            /* GT_REG_* */
     double reg_p[GT_REG_NPARAM];
     double *Fhist;           /* like Ehist: fidelity of each update's proposal (also when
@@ -328,10 +331,14 @@ typedef struct gt_fit_slot {
     int ninf;                /* atoms whose regulariser term is not finite */
     double F;                /* fidelity 1/2 ||resid||^2 */
     double Rsum;             /* sum of the finite regulariser terms */
+    /* block gradient descent (block = 2, read-only): the parameter blocks in
+       the order of the reference's dims string, 0 = I, 1 = x, 2 = r */
+    int gd_nblk;
+    int gd_blocks[3];
 } gt_fit_slot;
 
 /* nA must be <= GT_FIT_MAX_ANGLES and n0, n1 <= GT_FIT_MAX_DETECTOR. dim 2 or 3. */
-#define GT_FIT_MAX_ANGLES 128
+#define GT_FIT_MAX_ANGLES 256
 #define GT_FIT_MAX_DETECTOR 1024
 size_t gt_fit_scratch_bytes(int nA);
 int gt_fit_run(gt_fit_slot *slots, int nslots, int dim, int budget, void *stream);
