@@ -71,6 +71,8 @@ SIGNATURES = {
     "gt_derivs_workspace": (_sz, [_P(gt_atoms), _P(gt_geom), _ci]),
     "gt_derivs": (_ci, [_P(gt_atoms), _P(gt_geom), _vp, _ci, _ci, _vp, _vp, _vp, _vp, _sz, _vp]),
     "gt_forward_derivs": (_ci, [_P(gt_atoms), _P(gt_geom), _vp, _ci, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "gt_noalloc_scratch_bytes": (_sz, []),
+    "gt_forward_derivs_noalloc": (_ci, [_P(gt_atoms), _P(gt_geom), _vp, _ci, _vp, _vp, _vp, _vp, _vp]),
     "gt_volume_workspace": (_sz, [_P(gt_atoms), _P(gt_vol)]),
     "gt_volume": (_ci, [_P(gt_atoms), _P(gt_vol), _vp, _vp, _sz, _vp]),
     "gt_ft_workspace": (_sz, [_P(gt_atoms), _P(gt_kgrid)]),

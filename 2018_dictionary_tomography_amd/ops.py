@@ -182,6 +182,34 @@ def forward_derivs(atoms, space, data, order=2, out=None, want_f=True):
     return out, f, g, H
 
 
+def forward_derivs_noalloc(atoms, space, data, order=2, out=None, want_f=True):
+    """forward_derivs through gt_forward_derivs_noalloc: the library takes
+    the workspace from its per-GPU scratch array (at most
+    gt_noalloc_scratch_bytes()) instead of a caller's buffer, so the library
+    call allocates no device memory (this wrapper makes f, g, H, and the
+    residual when `out` is omitted, as forward_derivs does).  Calls that
+    overlap on one GPU race on the scratch array."""
+    lib = load()
+    dev = atoms.I.device
+    geom = device_geometry(space, dev)
+    a = atoms_struct(atoms)
+    data = data.data if hasattr(data, "data") and not isinstance(data, torch.Tensor) else data
+    data = torch.as_tensor(data, device=dev).to(torch.float32).contiguous()
+    if tuple(data.shape) != tuple(geom.shape):
+        raise ValueError(f"data must have shape {geom.shape}, got {tuple(data.shape)}")
+    if out is None:
+        out = torch.empty(geom.shape, dtype=torch.float32, device=dev)
+    if tuple(out.shape) != tuple(geom.shape) or out.dtype != torch.float32 or not out.is_contiguous():
+        raise ValueError(f"out must be a contiguous float32 tensor of shape {geom.shape}")
+    P = n_params(a.dim, a.iso)
+    f = torch.zeros((), dtype=torch.float64, device=dev) if want_f else None
+    g = torch.empty((a.n, P), dtype=torch.float64, device=dev)
+    H = torch.empty((a.n, P, P), dtype=torch.float64, device=dev) if order > 1 else None
+    check(lib.gt_forward_derivs_noalloc(ctypes.byref(a), ctypes.byref(geom.struct), _ptr(data), order, _ptr(out),
+                                        _ptr(f), _ptr(g), _ptr(H), stream_handle(dev)), "gt_forward_derivs_noalloc")
+    return out, f, g, H
+
+
 class DeviceVolume:
     def __init__(self, space, device):
         cv = space.coord_vectors

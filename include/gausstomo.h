@@ -130,6 +130,19 @@ int gt_derivs(const gt_atoms *atoms, const gt_geom *geom, const float *C, int mo
 int gt_forward_derivs(const gt_atoms *atoms, const gt_geom *geom, const float *data, int order, float *R,
                       double *f, double *g, double *H, void *ws, size_t ws_bytes, void *stream);
 
+/* No counterpart in the reference: gt_forward_derivs without a workspace, for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The workspace is a
+ * file-scope device array of GT_NOALLOC_SCRATCH_BYTES that each GPU the
+ * library runs on holds until the process ends; the call allocates nothing
+ * and clears nothing.  A sweep whose workspace (gt_derivs_workspace(..,
+ * GT_MODE_RESID)) exceeds it returns GT_EWORKSPACE.  Calls that overlap on
+ * one GPU (other streams, other host threads) race on the array. */
+#define GT_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+size_t gt_noalloc_scratch_bytes(void);
+int gt_forward_derivs_noalloc(const gt_atoms *atoms, const gt_geom *geom, const float *data, int order, float *R,
+                              double *f, double *g, double *H, void *stream);
+
 /* VolProj: u[j0][j1][j2] = sum over atoms of I sqrt(30/pi) exp(-30|U(y-x)|^2), cut at 1. */
 size_t gt_volume_workspace(const gt_atoms *atoms, const gt_vol *vol);
 int gt_volume(const gt_atoms *atoms, const gt_vol *vol, float *u, void *ws, size_t ws_bytes, void *stream);

@@ -50,6 +50,13 @@ def test_argument_validation_errors(gt):
     assert need >= 5 * 4 * 64
     assert lib.gt_forward(ctypes.byref(a), ctypes.byref(g), 1, 256, need - 1, None) == -3
     assert b"workspace" in lib.gt_last_error()
+    # the no-allocation step refuses a sweep beyond its scratch array before any device call
+    assert lib.gt_noalloc_scratch_bytes() == 128 << 20
+    big = L.gt_atoms(3, 20000, 0, 1, 1, 1)
+    gbig = L.gt_geom(3, 3, 120, 16, 16, 1, 1, 1, 1, 1, 1)
+    assert lib.gt_derivs_workspace(ctypes.byref(big), ctypes.byref(gbig), 2) > 128 << 20
+    assert lib.gt_forward_derivs_noalloc(ctypes.byref(big), ctypes.byref(gbig), 1, 2, 1, None, 1, None, None) == -3
+    assert b"scratch" in lib.gt_last_error()
     with pytest.raises(L.GTError):
         L.check(-1, "probe")
 
