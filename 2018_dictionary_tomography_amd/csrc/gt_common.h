@@ -74,6 +74,14 @@ inline int ceil_log2(long v)
 // sin(2 pi t) and cos(2 pi t) of a phase t in turns, |t| <= 256: the
 // hardware sine and cosine, whose argument is in turns (quarter-rate, like
 // exp2); measured maximum absolute error 1.25e-7 for |t| <= 1.5 on gfx950.
+// T in global memory.  Loads and stores through a gmem<T> * are global_*
+// instructions (scalar base plus 32-bit offset, counted by the vector-memory
+// counter only).  Through a plain pointer the compiler cannot trace to a
+// kernel argument (one read from a struct in memory) they would be flat
+// ones: 64-bit address arithmetic per access, and counted by the LDS counter
+// too, so that every wait for an LDS read also waits for them.
+template <typename T> using gmem = __attribute__((address_space(1))) T;
+
 // LDS written by some lanes of a wave, then read by others: order the wave's
 // LDS traffic (the lanes of a wave execute together).
 __device__ inline void wave_sync()

@@ -57,7 +57,6 @@ static_assert(NAMAX % 64 == 0, "per-angle sums take whole waves of angles");
 constexpr int DETMAX = GT_FIT_MAX_DETECTOR;
 constexpr int NVS = 65;                          // stride of a per-angle gradient/Hessian record
 constexpr int FIT_UNR = 8;                       // detector rows in flight per lane in the pixel passes
-constexpr int FIT_TROWS = 256;                   // rows of a wave's row table (moments pass)
 constexpr double FIT_EPS = 1e-4;                 // NewtonGaussian.py:21
 constexpr double GD_EPS = 1e-8;                  // GD_lib.py:96
 constexpr long long FIT_WAIT_TICKS = 200000000;  // barrier wait limit: 2 s of the 100 MHz clock
@@ -87,7 +86,6 @@ template <int D> struct FitLDS {
     PairRec rec_old[NAMAX];  // indexed by the member's own angle number
     PairRec rec_new[NAMAX];
     WaveC<D> wc[FIT_WAVES];
-    float4 tab[FIT_WAVES][FIT_TROWS];  // per-wave row tables of the moments pass
     float p0[DETMAX], p1[DETMAX];
     double th_old[10], th_new[10];
     double g[10], H[100];
@@ -97,7 +95,6 @@ template <int D> struct FitLDS {
     double rv_old, rv_new;    // regulariser terms of the current atom and of the proposal
     double dxnorm, hj;
     FitCtl c;
-    float dx0;  // grid step of the detector's first axis, 0 if it is not a uniform grid
     unsigned gen;
     int accept, j, stop, abort_, skip, moved;
 };
@@ -164,55 +161,19 @@ __device__ inline float pair_value(const PairRec &rec, float p0v, float p1v)
     return (q >= (float)KAPPA) ? rec.alpha * __builtin_amdgcn_exp2f(q) : 0.0f;
 }
 
-// Rows [rs, re] of one column of the union box (rows from lo0, H of them)
-// that can hold pixels of either support: the hull of the two records'
-// column spans (column_span; dx the detector's grid step, 0 if none).
-__device__ inline void union_span(const PairRec &ro, const PairRec &rn, float rlo, float clo, float rln, float cln,
-                                  float p0lo, float dx, int H, int &rs, int &re)
-{
-    int a0, a1, b0, b1;
-    const float idx = span_inv_dx(dx);
-    column_span(ro, rlo, clo, (p0lo - ro.c_hi[0]) - ro.c_lo[0], dx, idx, span_inv2a(ro), H, a0, a1);
-    column_span(rn, rln, cln, (p0lo - rn.c_hi[0]) - rn.c_lo[0], dx, idx, span_inv2a(rn), H, b0, b1);
-    if (a0 > a1) {
-        rs = b0;
-        re = b1;
-    } else if (b0 > b1) {
-        rs = a0;
-        re = a1;
-    } else {
-        rs = min(a0, b0);
-        re = max(a1, b1);
-    }
-}
-
-// Per-column terms of both records at column i1 of the union box.
-struct ColPair {
-    float rlo, clo, rln, cln;
-    __device__ inline void init(const PairRec &ro, const PairRec &rn, float pv1)
-    {
-        const float d1o = (pv1 - ro.c_hi[1]) - ro.c_lo[1], d1n = (pv1 - rn.c_hi[1]) - rn.c_lo[1];
-        rlo = fmaf(ro.q[1], d1o, ro.q[3]);
-        rln = fmaf(rn.q[1], d1n, rn.q[3]);
-        clo = fmaf(fmaf(ro.q[2], d1o, ro.q[4]), d1o, ro.q[5]);
-        cln = fmaf(fmaf(rn.q[2], d1n, rn.q[4]), d1n, rn.q[5]);
-    }
-};
-
 // Replacing rec_old by rec_new over the union box, both values evaluated per
 // pixel with the same FP32 expressions in every mode; returns the lane's FP64
-// energy-change partial (modes 0 and 2).  On a uniform 2D detector (dx > 0)
-// each column is walked over its union_span only: the pixels left out lie
-// outside both supports, where dR = 0 and nothing is written or summed.
+// energy-change partial (modes 0 and 2): the sum of (new - old)(new + old),
+// each term exact in FP64, halved once at the end (the same bits as halving
+// every term: scaling by 2 commutes with rounding).
 //   mode 0  energy change only (residual untouched)
 //   mode 1  commit: residual += new - old
 //   mode 2  speculative commit: energy change, residual updated in place, and
-//           the old residual of every walked pixel saved in Sa (same layout)
-//           so that pair_restore (which walks the same pixels) can undo a
-//           rejected proposal exactly
+//           the old residual of every union-box pixel saved in Sa (same layout)
+//           so that pair_restore can undo a rejected proposal exactly
 template <int K, int mode>
 __device__ inline double pair_delta4(const PairRec &ro, const PairRec &rn, const float *p0, const float *p1,
-                                     float dx, float *__restrict__ Ra, float *__restrict__ Sa, int n1, int lane)
+                                     gmem<float> *__restrict__ Ra, gmem<float> *__restrict__ Sa, int n1, int lane)
 {
     int lo[2], hi[2];
     box_union(ro, rn, K, lo, hi);
@@ -230,21 +191,20 @@ __device__ inline double pair_delta4(const PairRec &ro, const PairRec &rn, const
             cc.init(lane, W, cb);
             if (!cc.act) continue;
             const int i1 = lo[1] + cb + cc.c;
-            ColPair cp;
-            cp.init(ro, rn, p1[i1]);
-            const float rlo = cp.rlo, clo = cp.clo, rln = cp.rln, cln = cp.cln;
-            int rs, re;
-            union_span(ro, rn, rlo, clo, rln, cln, p0[lo[0]], dx, H, rs, re);
-            const int first = rs + ((cc.ro - rs) % cc.R + cc.R) % cc.R;
+            const float pv1 = p1[i1];
+            const float d1o = (pv1 - ro.c_hi[1]) - ro.c_lo[1], d1n = (pv1 - rn.c_hi[1]) - rn.c_lo[1];
+            const float rlo = fmaf(ro.q[1], d1o, ro.q[3]), rln = fmaf(rn.q[1], d1n, rn.q[3]);
+            const float clo = fmaf(fmaf(ro.q[2], d1o, ro.q[4]), d1o, ro.q[5]);
+            const float cln = fmaf(fmaf(rn.q[2], d1n, rn.q[4]), d1n, rn.q[5]);
             const unsigned off0 = (unsigned)lo[0] * (unsigned)n1 + (unsigned)i1;
-            for (int r0 = first; r0 <= re; r0 += FIT_UNR * cc.R) {
+            for (int r0 = cc.ro; r0 < H; r0 += FIT_UNR * cc.R) {
                 float old[FIT_UNR], pv0[FIT_UNR];
                 bool ok[FIT_UNR];
                 unsigned off[FIT_UNR];
 #pragma unroll
                 for (int u = 0; u < FIT_UNR; u++) {
                     const int r = r0 + u * cc.R;
-                    ok[u] = r <= re;
+                    ok[u] = r < H;
                     const unsigned rr = ok[u] ? (unsigned)r : 0u;
                     off[u] = off0 + rr * (unsigned)n1;
                     old[u] = Ra[off[u]];
@@ -263,12 +223,12 @@ __device__ inline double pair_delta4(const PairRec &ro, const PairRec &rn, const
                     if (ok[u] && dR != 0.0f) {
                         const float nw = old[u] + dR;
                         if (mode != 0) Ra[off[u]] = nw;
-                        if (mode != 1) acc += 0.5 * ((double)nw - (double)old[u]) * ((double)nw + (double)old[u]);
+                        if (mode != 1) acc += ((double)nw - (double)old[u]) * ((double)nw + (double)old[u]);
                     }
                 }
             }
         }
-        return acc;
+        return 0.5 * acc;
     }
     Track4 tr;
     tr.init(lane, W);
@@ -293,44 +253,24 @@ __device__ inline double pair_delta4(const PairRec &ro, const PairRec &rn, const
             if (ok[u] && dR != 0.0f) {
                 const float nw = old[u] + dR;
                 if (mode != 0) Ra[idx[u]] = nw;
-                if (mode != 1) acc += 0.5 * ((double)nw - (double)old[u]) * ((double)nw + (double)old[u]);
+                if (mode != 1) acc += ((double)nw - (double)old[u]) * ((double)nw + (double)old[u]);
             }
         }
         tr.step();
     }
-    return acc;
+    return 0.5 * acc;
 }
 
-// Undo a speculative commit: the pixels pair_delta4 walked (the union box,
-// or on a uniform 2D detector the union_span of each of its columns) back
-// from Sa.
+// Undo a speculative commit: the union box of the residual back from Sa.
 template <int K>
-__device__ inline void pair_restore(const PairRec &ro, const PairRec &rn, const float *p0, const float *p1, float dx,
-                                    float *__restrict__ Ra, const float *__restrict__ Sa, int n1, int lane)
+__device__ inline void pair_restore(const PairRec &ro, const PairRec &rn, gmem<float> *__restrict__ Ra,
+                                    const gmem<float> *__restrict__ Sa, int n1, int lane)
 {
     int lo[2], hi[2];
     box_union(ro, rn, K, lo, hi);
     const int W = (K == 2) ? hi[1] - lo[1] + 1 : 1;
     const int H = hi[0] - lo[0] + 1;
     if (W <= 0 || H <= 0) return;
-    if (K == 2) {
-        for (int cb = 0; cb < W; cb += 64) {
-            ColChunk cc;
-            cc.init(lane, W, cb);
-            if (!cc.act) continue;
-            const int i1 = lo[1] + cb + cc.c;
-            ColPair cp;
-            cp.init(ro, rn, p1[i1]);
-            int rs, re;
-            union_span(ro, rn, cp.rlo, cp.clo, cp.rln, cp.cln, p0[lo[0]], dx, H, rs, re);
-            const int first = rs + ((cc.ro - rs) % cc.R + cc.R) % cc.R;
-            for (int r = first; r <= re; r += cc.R) {
-                const unsigned off = (unsigned)(lo[0] + r) * (unsigned)n1 + (unsigned)i1;
-                Ra[off] = Sa[off];
-            }
-        }
-        return;
-    }
     for (int k = lane; k < W * H; k += 64) {
         const int r = k / W, c = k - (k / W) * W;
         const unsigned off = (unsigned)(lo[0] + r) * (unsigned)n1 + (unsigned)(K == 2 ? lo[1] + c : 0);
@@ -616,18 +556,13 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
     // The slab costs a store of every pixel of the union box; which is faster
     // depends on the load (the host decides per slot).  Both paths add the
     // same dR to the same old value, so results do not depend on it.
-    float *stage = S.stage;
+    gmem<float> *stage = (gmem<float> *)S.stage;
     double *dEa = (double *)(scr + fit_off_dE(nA));
     unsigned *bar = (unsigned *)(scr + fit_off_bar(nA));
     const int nOwn = member < nA ? (nA - member + G - 1) / G : 0;
     for (int i = tid; i < S.n0; i += FIT_THREADS) L.p0[i] = S.p0[i];
     if (K == 2)
         for (int i = tid; i < S.n1; i += FIT_THREADS) L.p1[i] = S.p1[i];
-    __syncthreads();
-    if (wave == 0) {
-        const float dx = grid_step(L.p0, S.n0, lane);
-        if (lane == 0) L.dx0 = dx;
-    }
     if (tid == 0) {
         FitCtl &c = L.c;
         c.it = S.it;
@@ -711,12 +646,11 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                 float acc[MOMW];
 #pragma unroll
                 for (int v = 0; v < MOMW; v++) acc[v] = 0.0f;
-                const float *Ra = S.resid + (size_t)a * slab;
+                const gmem<float> *Ra = (gmem<float> *)S.resid + (size_t)a * slab;
                 if (K == 2)
-                    pair_moments_rows<GT_MODE_RESID, FIT_UNR, FIT_TROWS>(rec, L.p0, L.p1, Ra, n1, lane, acc,
-                                                                         L.tab[wave]);
+                    pair_moments_box<GT_MODE_RESID, FIT_UNR>(rec, L.p0, L.p1, Ra, n1, lane, acc);
                 else
-                    pair_moments4<K, GT_MODE_RESID>(rec, L.p0, L.p1, Ra, n1, lane, acc);
+                    pair_moments4<K, GT_MODE_RESID>(rec, L.p0, L.p1, (const float *)Ra, n1, lane, acc);
                 float mine = 0.0f;
 #pragma unroll
                 for (int v = 0; v < 30; v++) {
@@ -951,10 +885,10 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         // ---- phase F: per-angle FP64 energy changes -----------------------
         for (int k = wave; k < nOwn; k += FIT_WAVES) {
             const int a = member + G * k;
-            float *Ra = S.resid + (size_t)a * slab;
-            double acc = stage ? pair_delta4<K, 2>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, L.dx0, Ra,
+            gmem<float> *Ra = (gmem<float> *)S.resid + (size_t)a * slab;
+            double acc = stage ? pair_delta4<K, 2>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, Ra,
                                                    stage + (size_t)a * slab, n1, lane)
-                               : pair_delta4<K, 0>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, L.dx0, Ra, nullptr, n1, lane);
+                               : pair_delta4<K, 0>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, Ra, nullptr, n1, lane);
             acc = wave_sum_f64(acc);
             if (lane == 0) xput(dEa + a, acc);
         }
@@ -1030,12 +964,11 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         if (stage ? !L.accept : L.accept) {
             for (int k = wave; k < nOwn; k += FIT_WAVES) {
                 const int a = member + G * k;
-                float *Ra = S.resid + (size_t)a * slab;
+                gmem<float> *Ra = (gmem<float> *)S.resid + (size_t)a * slab;
                 if (stage)
-                    pair_restore<K>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, L.dx0, Ra, stage + (size_t)a * slab, n1,
-                                    lane);
+                    pair_restore<K>(L.rec_old[k], L.rec_new[k], Ra, stage + (size_t)a * slab, n1, lane);
                 else
-                    pair_delta4<K, 1>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, L.dx0, Ra, nullptr, n1, lane);
+                    pair_delta4<K, 1>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, Ra, nullptr, n1, lane);
             }
         }
         // ---- loop control (NewtonGaussian.py:50-98, :131-222) ---------------

@@ -201,7 +201,7 @@ class FitServer:
     always has at least one workgroup."""
 
     def __init__(self, device=None, max_slots=None, hbm_budget=64 << 30, work_per_launch=4.0e7, use_graph=True,
-                 keep_results=False, grouped=True, stage_max_atoms=500):
+                 keep_results=False, grouped=True, stage_max_atoms=0):
         self.dev = device or torch.device("cuda", torch.cuda.current_device())
         self.capacity = F.max_blocks()
         self.max_slots = min(max_slots or self.capacity, self.capacity)
@@ -209,9 +209,8 @@ class FitServer:
         self.work_per_launch = work_per_launch
         self.use_graph = use_graph
         self.keep_results = keep_results
-        # fits of at most this many atoms get a staging slab (see FitRequest);
-        # measured on the serving workloads, a mix of larger fits serves more
-        # fits/s without it
+        # fits of at most this many atoms get a staging slab (see FitRequest;
+        # none by default: measured slower at every size of the serving mix)
         self.stage_max_atoms = stage_max_atoms
         self.grouped = grouped
         self.table = torch.zeros(self.max_slots * F.SLOT_BYTES, dtype=torch.uint8, device=self.dev)
