@@ -4,9 +4,12 @@
 //   setup    one thread per (atom, angle): FP64 detector Gaussian -> PairRec
 //   forward  one block per (angle, 8x64 pixel tile), centre tiles first:
 //            stages the angle's records that touch the tile in LDS; each of
-//            4 waves owns an 8x16 rectangle (2 pixels per lane), Kahan sums
+//            4 waves owns an 8x16 rectangle (2 pixels per lane) and walks
+//            the records whose support meets it; FP64 pixel sums of FP32
+//            values taken four records at a time
 //   moments  one wave per (atom, angle): FP32 separable pixel moments over
-//            the footprint box, LDS-transposed wave reduction, no atomics
+//            the rows of each column chunk that the support can reach,
+//            LDS-transposed wave reduction, no atomics
 //   finish   one wave per atom, one lane per angle: FP64 moment -> (f, g, H)
 //            contraction (gt_pair.h), FP64 wave reduction over angles
 //
@@ -107,6 +110,9 @@ __device__ inline int stage_chunk(const PairRec *__restrict__ ra, int base, int 
 constexpr int F3_WAVES = 4, F3_CHUNK = F3_WAVES * 64;
 static_assert(TILE_ROWS == 8 * ((F3_WAVES + 3) / 4) && TILE_COLS == 16 * (F3_WAVES < 4 ? F3_WAVES : 4),
               "the tile is the union of the waves' 8 x 16 rectangles");
+// first and last detector coordinate of each row group / column group of the
+// waves' rectangles (held in registers: indexed only by unrolled constants)
+constexpr int F3_RX = 2 * ((F3_WAVES + 3) / 4), F3_CY = 2 * (F3_WAVES < 4 ? F3_WAVES : 4);
 
 // The tiles at the detector centre carry several times the records of those
 // at its edges, and the grid holds more blocks than fit on the device at once:
@@ -116,7 +122,7 @@ static_assert(TILE_ROWS == 8 * ((F3_WAVES + 3) / 4) && TILE_COLS == 16 * (F3_WAV
 
 __device__ inline int stage_rects(const PairRec *__restrict__ ra, int base, int N, int row0, int col0, int thi0,
                                   int thi1, PairRec *list, unsigned short (*wl)[F3_CHUNK], int (*cnt)[F3_WAVES],
-                                  int *wlen, const float *rx, const float *cy)
+                                  int *wlen, bool exact, const float (&rx)[F3_RX], const float (&cy)[F3_CY])
 {
     const int lane = threadIdx.x & 63, hw = threadIdx.x >> 6;
     const int j = base + (int)threadIdx.x;
@@ -139,7 +145,7 @@ __device__ inline int stage_rects(const PairRec *__restrict__ ra, int base, int 
             for (int w = 0; w < F3_WAVES; w++) {
                 const int r = row0 + 8 * (w >> 2), c = col0 + 16 * (w & 3);
                 if (v3.x <= r + 7 && v3.z >= r && v3.y <= c + 15 && v3.w >= c &&
-                    (!rx || support_meets_rect(f1, f2, (rx[2 * (w >> 2)] - f0.y) - f0.w,
+                    (!exact || support_meets_rect(f1, f2, (rx[2 * (w >> 2)] - f0.y) - f0.w,
                                                (rx[2 * (w >> 2) + 1] - f0.y) - f0.w, (cy[2 * (w & 3)] - f0.z) - f1.x,
                                                (cy[2 * (w & 3) + 1] - f0.z) - f1.x)))
                     mask |= 1u << w;
@@ -186,14 +192,12 @@ __device__ inline int stage_rects(const PairRec *__restrict__ ra, int base, int 
     return total;
 }
 
-// One record's contribution to a lane's two pixels.  The record arrives as
-// its first 48 bytes (alpha, c_hi, c_lo, q): f0 = (alpha, c_hi0, c_hi1,
-// c_lo0), f1 = (c_lo1, q0, q1, q2), f2 = (q3, q4, q5, pad).  Branch-free
-// cutoff: every record in a wave's list reaches its rectangle.  The FP32
-// values are summed in FP64 (one conversion and one add per value, against
-// four FP32 operations for a Kahan sum, with a smaller error).
-__device__ inline void f3_pixels(const float4 &f0, const float4 &f1, const float4 &f2, float pc, float prA,
-                                 float prB, double &sA, double &sB)
+// One record's values at a lane's two pixels.  The record arrives as its
+// first 48 bytes (alpha, c_hi, c_lo, q): f0 = (alpha, c_hi0, c_hi1, c_lo0),
+// f1 = (c_lo1, q0, q1, q2), f2 = (q3, q4, q5, pad).  Branch-free cutoff:
+// every record in a wave's list reaches its rectangle.
+__device__ inline void f3_values(const float4 &f0, const float4 &f1, const float4 &f2, float pc, float prA,
+                                 float prB, float &vA, float &vB)
 {
     const float kap = (float)KAPPA;
     const float dp1 = (pc - f0.z) - f1.x;
@@ -204,8 +208,8 @@ __device__ inline void f3_pixels(const float4 &f0, const float4 &f1, const float
     const float qA = fmaf(fmaf(f1.y, dA, rowlin), dA, col);
     const float qB = fmaf(fmaf(f1.y, dB, rowlin), dB, col);
     const float eA = __builtin_amdgcn_exp2f(qA), eB = __builtin_amdgcn_exp2f(qB);
-    sA += (double)(qA >= kap ? f0.x * eA : 0.0f);
-    sB += (double)(qB >= kap ? f0.x * eB : 0.0f);
+    vA = qA >= kap ? f0.x * eA : 0.0f;
+    vB = qB >= kap ? f0.x * eB : 0.0f;
 }
 
 __global__ __launch_bounds__(F3_WAVES * 64) void forward3_kernel(const PairRec *__restrict__ recs, int N, int nA,
@@ -235,7 +239,7 @@ __global__ __launch_bounds__(F3_WAVES * 64) void forward3_kernel(const PairRec *
     double sA = 0.0, sB = 0.0;
     // detector coordinates bounding each wave rectangle (first and last
     // row / column), for the exact cull of an ascending detector
-    float rx[2 * ((F3_WAVES + 3) / 4)], cy[2 * (F3_WAVES < 4 ? F3_WAVES : 4)];
+    float rx[F3_RX], cy[F3_CY];
 #pragma unroll
     for (int w = 0; w < (F3_WAVES + 3) / 4; w++) {
         rx[2 * w] = p0[min(row0 + 8 * w, n0 - 1)];
@@ -248,7 +252,7 @@ __global__ __launch_bounds__(F3_WAVES * 64) void forward3_kernel(const PairRec *
     }
     const PairRec *ra = recs + (size_t)a * N;
     for (int base = 0; base < N; base += F3_CHUNK) {
-        stage_rects(ra, base, N, row0, col0, thi0, thi1, list, wl, cnt, wlen, sorted ? rx : nullptr, cy);
+        stage_rects(ra, base, N, row0, col0, thi0, thi1, list, wl, cnt, wlen, sorted != 0, rx, cy);
         const int len = wlen[wave];
         const unsigned short *wlw = wl[wave];
         int e = 0;
@@ -266,12 +270,22 @@ __global__ __launch_bounds__(F3_WAVES * 64) void forward3_kernel(const PairRec *
                 f1[u] = rp[1];
                 f2[u] = rp[2];
             }
+            // the four values of a pixel are summed in FP32 in list order and
+            // that sum is added to the pixel's FP64 total: an error of at most
+            // 3u times the sum of the four magnitudes, for one conversion and
+            // one FP64 add per four records
+            float vA[4], vB[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) f3_pixels(f0[u], f1[u], f2[u], pc, prA, prB, sA, sB);
+            for (int u = 0; u < 4; u++) f3_values(f0[u], f1[u], f2[u], pc, prA, prB, vA[u], vB[u]);
+            sA += (double)(((vA[0] + vA[1]) + vA[2]) + vA[3]);
+            sB += (double)(((vB[0] + vB[1]) + vB[2]) + vB[3]);
         }
         for (; e < len; e++) {
             const float4 *rp = reinterpret_cast<const float4 *>(list + wlw[e]);
-            f3_pixels(rp[0], rp[1], rp[2], pc, prA, prB, sA, sB);
+            float vA, vB;
+            f3_values(rp[0], rp[1], rp[2], pc, prA, prB, vA, vB);
+            sA += (double)vA;
+            sB += (double)vB;
         }
         __syncthreads();
     }
