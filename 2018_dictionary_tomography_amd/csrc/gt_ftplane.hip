@@ -434,37 +434,55 @@ __device__ inline float kd_sum(float v)
 
 // One workgroup per atom; wave w takes angles w, w + KD_WAVES (8), ...  csum[a] =
 // sum over angle a of |C|^2 (FP64).
-// At most 128 VGPRs (amdgpu_waves_per_eu(4): two blocks of 8 waves per CU):
-// 2.3 ms against 2.7 ms at the 207 VGPRs of one block per CU, despite 260
-// bytes of spills per lane (the row walk is latency-bound).
+// At most 128 VGPRs (amdgpu_waves_per_eu(4): two blocks of 8 waves per CU).
+// The FP64 state that the row walk does not use (the atom's shape, the
+// angle's frame and L G, the wave-summed moments) is kept in LDS rather than
+// in registers across the walk, so that the walk runs without spills.
 __global__ __launch_bounds__(KD_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void kd_kernel(gt_atoms A, KpDev<float> P, const float2 *__restrict__ C,
                                                         const double *__restrict__ csum, double *__restrict__ fo,
                                                         double *__restrict__ go, double *__restrict__ Ho)
 {
     __shared__ double part[KD_WAVES][KD_NV];
+    __shared__ double sx[3], srs[6], sL[3][3];                 // the atom: centre, shape slots, L
+    __shared__ double sG[KD_WAVES][3][2], sM[KD_WAVES][3][2];  // each wave's angle: G, L G
+    __shared__ KdMom sS[KD_WAVES];                             // each wave's moments (wave sums)
     const int j = blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int iso = A.iso;
     const int nr = iso ? 1 : 6;
     const double I = A.I[j];
-    double xj[3], L[3][3];
-    for (int i = 0; i < 3; i++) xj[i] = A.x[(size_t)j * 3 + i];
-    kp_shape(3, iso, A.r + (size_t)j * nr, L);
-    // r as the aniso slots of L (iso: Id / r) for the pair setup
-    double rs[6] = {L[0][0], L[1][1], L[2][2], L[1][0], L[2][1], L[2][0]};
-    double acc0 = 0.0, acc1 = 0.0;  // entries lane and lane + 64
+    if (threadIdx.x == 0) {
+        double L[3][3];
+        for (int i = 0; i < 3; i++) sx[i] = A.x[(size_t)j * 3 + i];
+        kp_shape(3, iso, A.r + (size_t)j * nr, L);
+        for (int i = 0; i < 3; i++)
+            for (int k = 0; k < 3; k++) sL[i][k] = L[i][k];
+        // r as the aniso slots of L (iso: Id / r) for the pair setup
+        srs[0] = L[0][0];
+        srs[1] = L[1][1];
+        srs[2] = L[2][2];
+        srs[3] = L[1][0];
+        srs[4] = L[2][1];
+        srs[5] = L[2][0];
+    }
+    for (int e = lane; e < KD_NV; e += 64) part[wave][e] = 0.0;
+    __syncthreads();
     const size_t plane = (size_t)P.m0 * P.m1;
     for (int a = wave; a < P.nA; a += KD_WAVES) {
-        double G[3][2], M[3][2];
-        kp_frame(P, a, G);
-        for (int i = 0; i < 3; i++)
-            for (int c = 0; c < 2; c++) {
-                double s = 0.0;
-                for (int k = 0; k < 3; k++) s += L[i][k] * G[k][c];
-                M[i][c] = s;
-            }
         KpRec<float> rec;
-        kp_pair(3, 0, I, xj, rs, G, P, KD_CUT, rec);
+        {
+            double G[3][2];
+            kp_frame(P, a, G);
+            if (lane == 0)
+                for (int i = 0; i < 3; i++)
+                    for (int c = 0; c < 2; c++) {
+                        double s = 0.0;
+                        for (int k = 0; k < 3; k++) s += sL[i][k] * G[k][c];
+                        sM[wave][i][c] = s;
+                        sG[wave][i][c] = G[i][c];
+                    }
+            kp_pair(3, 0, I, sx, srs, G, P, KD_CUT, rec);
+        }
         const float cutlog = (float)(-0.5 * KP_LOG2E * KD_CUT);
         const float amp = (float)I;
         float Sre[9], Sim[6], Sv[9];
@@ -530,19 +548,24 @@ __global__ __launch_bounds__(KD_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
                     }
                 }
         }
-        KdMom S;
+        KdMom &S = sS[wave];
 #pragma unroll
         for (int q = 0; q < 9; q++) {
-            S.re[q] = (double)kd_sum(Sre[q]);
-            S.v[q] = (double)kd_sum(Sv[q]);
+            const float re = kd_sum(Sre[q]), v = kd_sum(Sv[q]);
+            if (lane == 0) {
+                S.re[q] = (double)re;
+                S.v[q] = (double)v;
+            }
         }
 #pragma unroll
-        for (int q = 0; q < 6; q++) S.im[q] = (double)kd_sum(Sim[q]);
-        acc0 += kd_entry(lane, I, G, M, S);
-        if (lane + 64 < KD_NV) acc1 += kd_entry(lane + 64, I, G, M, S);
+        for (int q = 0; q < 6; q++) {
+            const float im = kd_sum(Sim[q]);
+            if (lane == 0) S.im[q] = (double)im;
+        }
+        wave_sync();  // S, sG, sM written by lane 0, read by every lane
+        for (int e = lane; e < KD_NV; e += 64) part[wave][e] += kd_entry(e, I, sG[wave], sM[wave], S);
+        wave_sync();  // S, sG and sM are rewritten for the wave's next angle
     }
-    part[wave][lane] = acc0;
-    if (lane + 64 < KD_NV) part[wave][lane + 64] = acc1;
     __syncthreads();
     __shared__ double tot[KD_NV];
     if (threadIdx.x < KD_NV) {
