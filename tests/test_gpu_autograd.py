@@ -63,6 +63,8 @@ def test_autograd_matches_analytic_and_fd(gt, oracle, cuda, c3):
     vals = np.array([G[p // 10, p % 10] for p in picks])
     refs = np.array([ref[p // 10][p % 10] for p in picks])
     scale = np.abs(refs).max()
+    print(f"autograd at 5000 atoms x 90 x 256^2: max |gpu - fp64| / max |fp64| = "
+          f"{np.abs(vals - refs).max() / scale:.3e} over {len(picks)} parameters")
     assert np.abs(vals - refs).max() < 1e-5 * scale
 
     # footprint-local central differences in FP64

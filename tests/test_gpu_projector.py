@@ -139,6 +139,9 @@ def _derivs_case(gt, oracle, cuda, space, I, x, r, mode, dim):
     else:
         fo, go, Ho = oracle.derivs(f32(I), f32(x), f32(r), og, C, omode, 2)
     g, H = g.cpu().numpy(), H.cpu().numpy()
+    print(f"derivs {mode} {dim}D: worst per-atom relative error gradient "
+          f"{max(relerr(g[j], go[j]) for j in range(len(I))):.3e}, Hessian "
+          f"{max(relerr(H[j], Ho[j]) for j in range(len(I))):.3e}")
     for j in range(len(I)):
         assert relerr(g[j], go[j]) < TOL, (j, relerr(g[j], go[j]))
         assert relerr(H[j], Ho[j]) < TOL, (j, relerr(H[j], Ho[j]))
