@@ -98,6 +98,7 @@ SIGNATURES = {
     "gt_fit_run_blocks": (_ci, [_vp, _ci, _vp, _ci, _ci, _ci, _vp]),
     "gt_profile_enable": (_ci, [_ci]),
     "gt_profile_read": (_ci, [_ci, _P(ctypes.c_double), _P(ctypes.c_long)]),
+    "gt_profile_select": (_ci, [ctypes.c_uint]),
 }
 
 GT_K_SETUP, GT_K_FORWARD, GT_K_MOMENTS, GT_K_FINISH, GT_K_VOLUME, GT_K_FT, GT_K_FIT = range(7)
@@ -105,6 +106,12 @@ GT_K_SETUP, GT_K_FORWARD, GT_K_MOMENTS, GT_K_FINISH, GT_K_VOLUME, GT_K_FT, GT_K_
 
 def profile_enable(on=True):
     load().gt_profile_enable(1 if on else 0)
+
+
+def profile_select(kernel_ids=None):
+    """Bracket only these kernel ids with timing events (None: all)."""
+    mask = 0xFFFFFFFF if kernel_ids is None else sum(1 << int(k) for k in kernel_ids)
+    load().gt_profile_select(mask)
 
 
 def profile_read(kernel_id):

@@ -1,6 +1,7 @@
 // gt_common.h -- shared host/device helpers of the HIP library.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -15,6 +16,21 @@ void set_error(const char *fmt, ...);
 // Optional per-launch event timing (gt_profile_enable / gt_profile_read).
 void prof_begin(hipStream_t st, int id);
 void prof_end(hipStream_t st, int id);
+// The same for a launch timed by its own dispatch: events a, b for
+// hipExtLaunchKernelGGL, which records the kernel's start and end in the
+// kernel's packet (no packets between launches); false, and both null, when
+// the kernel id is not being timed.
+bool prof_ext_begin(int id, hipEvent_t *a, hipEvent_t *b);
+void prof_ext_end(int id, hipEvent_t a, hipEvent_t b);
+
+template <typename F, typename... Args>
+inline void launch_timed(int id, F kernel, dim3 grid, dim3 block, unsigned shmem, hipStream_t st, Args... args)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    const bool timed = prof_ext_begin(id, &a, &b);
+    hipExtLaunchKernelGGL(kernel, grid, block, shmem, st, a, b, 0u, args...);
+    if (timed) prof_ext_end(id, a, b);
+}
 
 #define GT_CHECK_ARG(cond, ...)          \
     do {                                 \

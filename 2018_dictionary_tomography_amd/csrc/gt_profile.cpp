@@ -18,6 +18,13 @@ struct Rec {
     hipEvent_t a, b;
 };
 std::atomic<int> g_on{0};
+std::atomic<unsigned> g_mask{~0u};  // kernel ids bracketed while enabled
+
+bool selected(int id)
+{
+    return g_on.load(std::memory_order_relaxed) && id >= 0 && id < 32 &&
+           ((g_mask.load(std::memory_order_relaxed) >> id) & 1u);
+}
 std::mutex g_mu;
 std::vector<Rec> g_done;                 // ended, not read
 std::vector<hipEvent_t> g_pool;
@@ -40,7 +47,7 @@ hipEvent_t get_event()
 
 void prof_begin(hipStream_t st, int id)
 {
-    if (!g_on.load(std::memory_order_relaxed)) return;
+    if (!selected(id)) return;
     hipEvent_t a = get_event(), b = get_event();
     if (!a || !b) return;
     hipEventRecord(a, st);
@@ -49,13 +56,30 @@ void prof_begin(hipStream_t st, int id)
 
 void prof_end(hipStream_t st, int id)
 {
-    if (!g_on.load(std::memory_order_relaxed) || t_stack.empty()) return;
+    if (!selected(id) || t_stack.empty()) return;
     Rec r = t_stack.back();
     t_stack.pop_back();
     if (r.id != id) return;
     hipEventRecord(r.b, st);
     std::lock_guard<std::mutex> l(g_mu);
     g_done.push_back(r);
+}
+
+bool prof_ext_begin(int id, hipEvent_t *a, hipEvent_t *b)
+{
+    *a = *b = nullptr;
+    if (!selected(id)) return false;
+    hipEvent_t ea = get_event(), eb = get_event();
+    if (!ea || !eb) return false;
+    *a = ea;
+    *b = eb;
+    return true;
+}
+
+void prof_ext_end(int id, hipEvent_t a, hipEvent_t b)
+{
+    std::lock_guard<std::mutex> l(g_mu);
+    g_done.push_back({id, a, b});
 }
 
 }  // namespace gt
@@ -65,6 +89,12 @@ using namespace gt;
 extern "C" int gt_profile_enable(int on)
 {
     g_on.store(on ? 1 : 0);
+    return 0;
+}
+
+extern "C" int gt_profile_select(unsigned mask)
+{
+    g_mask.store(mask);
     return 0;
 }
 

@@ -531,12 +531,10 @@ int launch_setup(const gt_atoms *A, const gt_geom *G, PairRec *recs, hipStream_t
     const long total = (long)A->n * G->nA;
     if (total == 0) return 0;
     const unsigned blocks = (unsigned)((total + 255) / 256);
-    prof_begin(st, GT_K_SETUP);
     if (G->dim == 3)
-        hipLaunchKernelGGL(setup_kernel<3>, dim3(blocks), dim3(256), 0, st, *A, *G, recs);
+        launch_timed(GT_K_SETUP, setup_kernel<3>, dim3(blocks), dim3(256), 0, st, *A, *G, recs);
     else
-        hipLaunchKernelGGL(setup_kernel<2>, dim3(blocks), dim3(256), 0, st, *A, *G, recs);
-    prof_end(st, GT_K_SETUP);
+        launch_timed(GT_K_SETUP, setup_kernel<2>, dim3(blocks), dim3(256), 0, st, *A, *G, recs);
     GT_HIP(hipGetLastError());
     return 0;
 }
@@ -571,19 +569,18 @@ int launch_forward(const gt_atoms *A, const gt_geom *G, const PairRec *recs, flo
         GT_HIP(hipGetLastError());
         return 0;
     }
-    prof_begin(st, GT_K_FORWARD);
     if (G->dim == 3) {
         const int e0 = ceil_log2((G->n0 + TILE_ROWS - 1) / TILE_ROWS), e1 = ceil_log2((G->n1 + TILE_COLS - 1) / TILE_COLS);
         const unsigned long long blocks = (unsigned long long)G->nA << (e0 + e1);
         GT_CHECK_ARG(blocks < (1ull << 31), "forward: too many detector tiles (%d x %d, %d angles)", G->n0, G->n1,
                      G->nA);
-        hipLaunchKernelGGL(forward3_kernel, dim3((unsigned)blocks), dim3(F3_WAVES * 64), 0, st, recs, A->n, G->nA,
-                           G->n0, G->n1, G->p0, G->p1, R, sub, e0, e1, G->sorted);
+        launch_timed(GT_K_FORWARD, forward3_kernel, dim3((unsigned)blocks), dim3(F3_WAVES * 64), 0, st, recs, A->n,
+                     G->nA, G->n0, G->n1, G->p0, G->p1, R, sub, e0, e1, G->sorted);
     } else {
         const int tiles = (G->n0 + 255) / 256;
-        hipLaunchKernelGGL(forward2_kernel, dim3(tiles, G->nA), dim3(256), 0, st, recs, A->n, G->n0, G->p0, R, sub);
+        launch_timed(GT_K_FORWARD, forward2_kernel, dim3(tiles, G->nA), dim3(256), 0, st, recs, A->n, G->n0, G->p0, R,
+                     sub);
     }
-    prof_end(st, GT_K_FORWARD);
     GT_HIP(hipGetLastError());
     return 0;
 }
@@ -595,14 +592,10 @@ static void launch_deriv_kernels(const gt_atoms *A, const gt_geom *G, const Pair
 {
     constexpr int K = D - 1;
     const long pairs = (long)A->n * G->nA;
-    prof_begin(st, GT_K_MOMENTS);
-    hipLaunchKernelGGL((moments_kernel<K, MODE>), dim3((unsigned)((pairs + 3) / 4)), dim3(256), 0, st, recs, A->n,
-                       G->nA, G->n0, G->n1, G->p0, G->p1, C, mom);
-    prof_end(st, GT_K_MOMENTS);
-    prof_begin(st, GT_K_FINISH);
-    hipLaunchKernelGGL((finish_kernel<D, MODE>), dim3(A->n), dim3(64), 0, st, *A, *G, recs, mom, sumC2, want_hess,
-                       f, g, H);
-    prof_end(st, GT_K_FINISH);
+    launch_timed(GT_K_MOMENTS, moments_kernel<K, MODE>, dim3((unsigned)((pairs + 3) / 4)), dim3(256), 0, st, recs,
+                 A->n, G->nA, G->n0, G->n1, G->p0, G->p1, C, mom);
+    launch_timed(GT_K_FINISH, finish_kernel<D, MODE>, dim3(A->n), dim3(64), 0, st, *A, *G, recs, mom, sumC2, want_hess,
+                 f, g, H);
 }
 
 int launch_derivs(const gt_atoms *A, const gt_geom *G, const PairRec *recs, float *mom, double *sumC2,

@@ -102,8 +102,13 @@ const char *gt_version(void);
 /* Timing support (no counterpart in the reference, which only prints wall
  * time): when enabled, each kernel launch is bracketed by HIP events on its
  * stream; gt_profile_read waits for the recorded launches of one kernel id,
- * returns their summed duration in ms and their count, and forgets them. */
+ * returns their summed duration in ms and their count, and forgets them.
+ * gt_profile_select limits the bracketing to the kernel ids whose bit
+ * (1u << id) is set in mask (all of them by default): each event pair adds
+ * a few microseconds between launches, so a timed run brackets only the
+ * kernel it measures. */
 int gt_profile_enable(int on);
+int gt_profile_select(unsigned mask);
 int gt_profile_read(int kernel_id, double *total_ms, long *count);
 
 /* RadProj: R[a][i0][i1] = sum over atoms of the line integral (written, not accumulated). */
