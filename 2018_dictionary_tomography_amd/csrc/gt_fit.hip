@@ -57,6 +57,7 @@ static_assert(NAMAX % 64 == 0, "per-angle sums take whole waves of angles");
 constexpr int DETMAX = GT_FIT_MAX_DETECTOR;
 constexpr int NVS = 65;                          // stride of a per-angle gradient/Hessian record
 constexpr int FIT_UNR = 8;                       // detector rows in flight per lane in the pixel passes
+constexpr int FIT_TROWS = 256;                   // rows of a wave's row table (moments, energy and commit passes)
 constexpr double FIT_EPS = 1e-4;                 // NewtonGaussian.py:21
 constexpr double GD_EPS = 1e-8;                  // GD_lib.py:96
 constexpr long long FIT_WAIT_TICKS = 200000000;  // barrier wait limit: 2 s of the 100 MHz clock
@@ -87,6 +88,7 @@ template <int D> struct FitLDS {
     PairRec rec_new[NAMAX];
     WaveC<D> wc[FIT_WAVES];
     float p0[DETMAX], p1[DETMAX];
+    float4 tab[FIT_WAVES][FIT_TROWS];  // per-wave row tables of the pixel passes
     double th_old[10], th_new[10];
     double g[10], H[100];
     double A[100], b[10], x[10];
@@ -171,9 +173,14 @@ __device__ inline float pair_value(const PairRec &rec, float p0v, float p1v)
 //   mode 2  speculative commit: energy change, residual updated in place, and
 //           the old residual of every union-box pixel saved in Sa (same layout)
 //           so that pair_restore can undo a rejected proposal exactly
+// On a 2D detector the union box is taken in windows of at most FIT_TROWS
+// rows; per window the wave first writes the row table dt[r] = (dp0 of the
+// old record, dp0 of the new one) of its rows (the expressions a pixel would
+// evaluate), so a pixel reads its row terms instead of forming them.
 template <int K, int mode>
 __device__ inline double pair_delta4(const PairRec &ro, const PairRec &rn, const float *p0, const float *p1,
-                                     gmem<float> *__restrict__ Ra, gmem<float> *__restrict__ Sa, int n1, int lane)
+                                     gmem<float> *__restrict__ Ra, gmem<float> *__restrict__ Sa, int n1, int lane,
+                                     float2 *dt)
 {
     int lo[2], hi[2];
     box_union(ro, rn, K, lo, hi);
@@ -186,48 +193,58 @@ __device__ inline double pair_delta4(const PairRec &ro, const PairRec &rn, const
         // column-owning walk (ColChunk): per-column terms of both Gaussians once
         // per lane; 32-bit element offsets within the angle's slab
         const float kap = (float)KAPPA;
-        for (int cb = 0; cb < W; cb += 64) {
-            ColChunk cc;
-            cc.init(lane, W, cb);
-            if (!cc.act) continue;
-            const int i1 = lo[1] + cb + cc.c;
-            const float pv1 = p1[i1];
-            const float d1o = (pv1 - ro.c_hi[1]) - ro.c_lo[1], d1n = (pv1 - rn.c_hi[1]) - rn.c_lo[1];
-            const float rlo = fmaf(ro.q[1], d1o, ro.q[3]), rln = fmaf(rn.q[1], d1n, rn.q[3]);
-            const float clo = fmaf(fmaf(ro.q[2], d1o, ro.q[4]), d1o, ro.q[5]);
-            const float cln = fmaf(fmaf(rn.q[2], d1n, rn.q[4]), d1n, rn.q[5]);
-            const unsigned off0 = (unsigned)lo[0] * (unsigned)n1 + (unsigned)i1;
-            for (int r0 = cc.ro; r0 < H; r0 += FIT_UNR * cc.R) {
-                float old[FIT_UNR], pv0[FIT_UNR];
-                bool ok[FIT_UNR];
-                unsigned off[FIT_UNR];
+        for (int w0 = 0; w0 < H; w0 += FIT_TROWS) {
+            const int Hw = min(FIT_TROWS, H - w0);
+            if (w0 > 0) wave_sync();  // the previous window's table is no longer read
+            for (int r = lane; r < Hw; r += 64) {
+                const float pv0 = p0[lo[0] + w0 + r];
+                dt[r] = make_float2((pv0 - ro.c_hi[0]) - ro.c_lo[0], (pv0 - rn.c_hi[0]) - rn.c_lo[0]);
+            }
+            wave_sync();
+            for (int cb = 0; cb < W; cb += 64) {
+                ColChunk cc;
+                cc.init(lane, W, cb);
+                if (!cc.act) continue;
+                const int i1 = lo[1] + cb + cc.c;
+                const float pv1 = p1[i1];
+                const float d1o = (pv1 - ro.c_hi[1]) - ro.c_lo[1], d1n = (pv1 - rn.c_hi[1]) - rn.c_lo[1];
+                const float rlo = fmaf(ro.q[1], d1o, ro.q[3]), rln = fmaf(rn.q[1], d1n, rn.q[3]);
+                const float clo = fmaf(fmaf(ro.q[2], d1o, ro.q[4]), d1o, ro.q[5]);
+                const float cln = fmaf(fmaf(rn.q[2], d1n, rn.q[4]), d1n, rn.q[5]);
+                const unsigned off0 = (unsigned)(lo[0] + w0) * (unsigned)n1 + (unsigned)i1;
+                for (int r0 = cc.ro; r0 < Hw; r0 += FIT_UNR * cc.R) {
+                    float old[FIT_UNR];
+                    float2 d0[FIT_UNR];
+                    bool ok[FIT_UNR];
+                    unsigned off[FIT_UNR];
 #pragma unroll
-                for (int u = 0; u < FIT_UNR; u++) {
-                    const int r = r0 + u * cc.R;
-                    ok[u] = r < H;
-                    const unsigned rr = ok[u] ? (unsigned)r : 0u;
-                    off[u] = off0 + rr * (unsigned)n1;
-                    old[u] = Ra[off[u]];
-                    pv0[u] = p0[lo[0] + (int)rr];
-                }
+                    for (int u = 0; u < FIT_UNR; u++) {
+                        const int r = r0 + u * cc.R;
+                        ok[u] = r < Hw;
+                        const unsigned rr = ok[u] ? (unsigned)r : 0u;
+                        off[u] = off0 + rr * (unsigned)n1;
+                        old[u] = Ra[off[u]];
+                        d0[u] = dt[rr];
+                    }
 #pragma unroll
-                for (int u = 0; u < FIT_UNR; u++) {
-                    const float d0o = (pv0[u] - ro.c_hi[0]) - ro.c_lo[0];
-                    const float d0n = (pv0[u] - rn.c_hi[0]) - rn.c_lo[0];
-                    const float qo = fmaf(fmaf(ro.q[0], d0o, rlo), d0o, clo);
-                    const float qn = fmaf(fmaf(rn.q[0], d0n, rln), d0n, cln);
-                    const float vo = (qo >= kap) ? ro.alpha * __builtin_amdgcn_exp2f(qo) : 0.0f;
-                    const float vn = (qn >= kap) ? rn.alpha * __builtin_amdgcn_exp2f(qn) : 0.0f;
-                    const float dR = vn - vo;
-                    if (mode == 2 && ok[u]) Sa[off[u]] = old[u];
-                    if (ok[u] && dR != 0.0f) {
-                        const float nw = old[u] + dR;
-                        if (mode != 0) Ra[off[u]] = nw;
-                        if (mode != 1) acc += ((double)nw - (double)old[u]) * ((double)nw + (double)old[u]);
+                    for (int u = 0; u < FIT_UNR; u++) {
+                        const float d0o = d0[u].x, d0n = d0[u].y;
+                        const float qo = fmaf(fmaf(ro.q[0], d0o, rlo), d0o, clo);
+                        const float qn = fmaf(fmaf(rn.q[0], d0n, rln), d0n, cln);
+                        const float vo = (qo >= kap) ? ro.alpha * __builtin_amdgcn_exp2f(qo) : 0.0f;
+                        const float vn = (qn >= kap) ? rn.alpha * __builtin_amdgcn_exp2f(qn) : 0.0f;
+                        const float dR = vn - vo;
+                        if (mode == 2 && ok[u]) Sa[off[u]] = old[u];
+                        if (ok[u] && dR != 0.0f) {
+                            const float nw = old[u] + dR;
+                            if (mode != 0) Ra[off[u]] = nw;
+                            if (mode != 1) acc += ((double)nw - (double)old[u]) * ((double)nw + (double)old[u]);
+                        }
                     }
                 }
             }
         }
+        wave_sync();  // the table is rewritten for the wave's next pair
         return 0.5 * acc;
     }
     Track4 tr;
@@ -648,7 +665,8 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                 for (int v = 0; v < MOMW; v++) acc[v] = 0.0f;
                 const gmem<float> *Ra = (gmem<float> *)S.resid + (size_t)a * slab;
                 if (K == 2)
-                    pair_moments_box<GT_MODE_RESID, FIT_UNR>(rec, L.p0, L.p1, Ra, n1, lane, acc);
+                    pair_moments_box<GT_MODE_RESID, FIT_UNR, FIT_TROWS>(rec, L.p0, L.p1, Ra, n1, lane, acc,
+                                                                        L.tab[wave]);
                 else
                     pair_moments4<K, GT_MODE_RESID>(rec, L.p0, L.p1, (const float *)Ra, n1, lane, acc);
                 float mine = 0.0f;
@@ -886,9 +904,10 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         for (int k = wave; k < nOwn; k += FIT_WAVES) {
             const int a = member + G * k;
             gmem<float> *Ra = (gmem<float> *)S.resid + (size_t)a * slab;
+            float2 *dt = reinterpret_cast<float2 *>(L.tab[wave]);
             double acc = stage ? pair_delta4<K, 2>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, Ra,
-                                                   stage + (size_t)a * slab, n1, lane)
-                               : pair_delta4<K, 0>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, Ra, nullptr, n1, lane);
+                                                   stage + (size_t)a * slab, n1, lane, dt)
+                               : pair_delta4<K, 0>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, Ra, nullptr, n1, lane, dt);
             acc = wave_sum_f64(acc);
             if (lane == 0) xput(dEa + a, acc);
         }
@@ -968,7 +987,8 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                 if (stage)
                     pair_restore<K>(L.rec_old[k], L.rec_new[k], Ra, stage + (size_t)a * slab, n1, lane);
                 else
-                    pair_delta4<K, 1>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, Ra, nullptr, n1, lane);
+                    pair_delta4<K, 1>(L.rec_old[k], L.rec_new[k], L.p0, L.p1, Ra, nullptr, n1, lane,
+                                      reinterpret_cast<float2 *>(L.tab[wave]));
             }
         }
         // ---- loop control (NewtonGaussian.py:50-98, :131-222) ---------------

@@ -429,64 +429,74 @@ __device__ inline void pair_moments_rows(const PairRec &rec, const float *p0, co
     wave_sync();  // tab may be rewritten by the caller's next use
 }
 
-// The same moments over the whole box, no row table and no spans: 64-column
-// chunks, each lane its rows ro, ro + R, ... of the box, x and its powers
-// evaluated per pixel (the expressions of row_table).  For a wave that walks
-// many small boxes whose residual rows sit in HBM (the fitter's one atom at a
-// time over every angle), the table and span set-up cost more than the
-// corners they skip.
-template <int MODE, int UNR>
+// The same moments over the whole box, without spans: 64-column chunks, each
+// lane its rows ro, ro + R, ... of the box, in windows of at most TROWS rows
+// whose x-powers the wave first writes to its row table (tab, the
+// expressions of row_table).  For a wave that walks many small boxes whose
+// residual rows sit in HBM (the fitter's one atom at a time over every
+// angle), the span set-up costs more than the corners it skips.
+template <int MODE, int UNR, int TROWS>
 __device__ inline void pair_moments_box(const PairRec &rec, const float *p0, const float *p1,
-                                        const gmem<float> *__restrict__ Ca, int n1, int lane, float *acc)
+                                        const gmem<float> *__restrict__ Ca, int n1, int lane, float *acc,
+                                        float4 *tab)
 {
     const int W = rec.hi[1] - rec.lo[1] + 1;
-    const int H = rec.hi[0] - rec.lo[0] + 1;
-    if (W <= 0 || H <= 0) return;
+    const int Hbox = rec.hi[0] - rec.lo[0] + 1;
+    if (W <= 0 || Hbox <= 0) return;
     const gmem<char> *cbase = reinterpret_cast<const gmem<char> *>(Ca);
     const unsigned rowb = 4u * (unsigned)n1;
-    for (int cb = 0; cb < W; cb += 64) {
-        ColChunk cc;
-        cc.init(lane, W, cb);
-        if (!cc.act) continue;
-        const int i1 = rec.lo[1] + cb + cc.c;
-        const float dp1 = (p1[i1] - rec.c_hi[1]) - rec.c_lo[1];
-        const float rowlin = fmaf(rec.q[1], dp1, rec.q[3]);
-        const float colt = fmaf(fmaf(rec.q[2], dp1, rec.q[4]), dp1, rec.q[5]);
-        float A[5] = {0, 0, 0, 0, 0}, B[5] = {0, 0, 0, 0, 0};
-        const unsigned cb0 = 4u * ((unsigned)rec.lo[0] * (unsigned)n1 + (unsigned)i1);
-        int r0 = cc.ro;
-        // full groups of UNR rows, all inside the box
-        for (; r0 + (UNR - 1) * cc.R < H; r0 += UNR * cc.R) {
-            float cv[UNR], x[UNR];
-#pragma unroll
-            for (int u = 0; u < UNR; u++) {
-                const unsigned r = (unsigned)(r0 + u * cc.R);
-                cv[u] = *reinterpret_cast<const gmem<float> *>(cbase + (cb0 + r * rowb));
-                x[u] = (p0[rec.lo[0] + (int)r] - rec.c_hi[0]) - rec.c_lo[0];
-            }
-#pragma unroll
-            for (int u = 0; u < UNR; u++) {
-                const float x2 = x[u] * x[u];
-                moment_pixel<MODE>(rec, cv[u], make_float4(x[u], x2, x2 * x[u], x2 * x2), rowlin, colt, A, B, acc);
-            }
+    for (int w0 = 0; w0 < Hbox; w0 += TROWS) {
+        const int H = min(TROWS, Hbox - w0);
+        const int row0 = rec.lo[0] + w0;
+        if (w0 > 0) wave_sync();  // the previous window's table is no longer read
+        for (int r = lane; r < H; r += 64) {
+            const float x = (p0[row0 + r] - rec.c_hi[0]) - rec.c_lo[0];
+            const float x2 = x * x;
+            tab[r] = make_float4(x, x2, x2 * x, x2 * x2);
         }
-        // remaining rows one at a time (same row order)
-        for (; r0 < H; r0 += cc.R) {
-            const float cv = *reinterpret_cast<const gmem<float> *>(cbase + (cb0 + (unsigned)r0 * rowb));
-            const float x = (p0[rec.lo[0] + r0] - rec.c_hi[0]) - rec.c_lo[0], x2 = x * x;
-            moment_pixel<MODE>(rec, cv, make_float4(x, x2, x2 * x, x2 * x2), rowlin, colt, A, B, acc);
-        }
-        const float y2 = dp1 * dp1;
-        const float yp[5] = {1.0f, dp1, y2, y2 * dp1, y2 * y2};
+        wave_sync();
+        for (int cb = 0; cb < W; cb += 64) {
+            ColChunk cc;
+            cc.init(lane, W, cb);
+            if (!cc.act) continue;
+            const int i1 = rec.lo[1] + cb + cc.c;
+            const float dp1 = (p1[i1] - rec.c_hi[1]) - rec.c_lo[1];
+            const float rowlin = fmaf(rec.q[1], dp1, rec.q[3]);
+            const float colt = fmaf(fmaf(rec.q[2], dp1, rec.q[4]), dp1, rec.q[5]);
+            float A[5] = {0, 0, 0, 0, 0}, B[5] = {0, 0, 0, 0, 0};
+            const unsigned cb0 = 4u * ((unsigned)row0 * (unsigned)n1 + (unsigned)i1);
+            int r0 = cc.ro;
+            // full groups of UNR rows, all inside the window
+            for (; r0 + (UNR - 1) * cc.R < H; r0 += UNR * cc.R) {
+                float cv[UNR];
+                float4 xp[UNR];
 #pragma unroll
-        for (int i = 0; i <= 4; i++)
+                for (int u = 0; u < UNR; u++) {
+                    const unsigned r = (unsigned)(r0 + u * cc.R);
+                    cv[u] = *reinterpret_cast<const gmem<float> *>(cbase + (cb0 + r * rowb));
+                    xp[u] = tab[r];
+                }
 #pragma unroll
-            for (int j = 0; i + j <= 4; j++) {
-                const int m = (i + j) * (i + j + 1) / 2 + j;
-                acc[m] = fmaf(A[i], yp[j], acc[m]);
-                if (MODE != GT_MODE_LINEAR) acc[15 + m] = fmaf(B[i], yp[j], acc[15 + m]);
+                for (int u = 0; u < UNR; u++) moment_pixel<MODE>(rec, cv[u], xp[u], rowlin, colt, A, B, acc);
             }
+            // remaining rows one at a time (same row order)
+            for (; r0 < H; r0 += cc.R) {
+                const float cv = *reinterpret_cast<const gmem<float> *>(cbase + (cb0 + (unsigned)r0 * rowb));
+                moment_pixel<MODE>(rec, cv, tab[r0], rowlin, colt, A, B, acc);
+            }
+            const float y2 = dp1 * dp1;
+            const float yp[5] = {1.0f, dp1, y2, y2 * dp1, y2 * y2};
+#pragma unroll
+            for (int i = 0; i <= 4; i++)
+#pragma unroll
+                for (int j = 0; i + j <= 4; j++) {
+                    const int m = (i + j) * (i + j + 1) / 2 + j;
+                    acc[m] = fmaf(A[i], yp[j], acc[m]);
+                    if (MODE != GT_MODE_LINEAR) acc[15 + m] = fmaf(B[i], yp[j], acc[15 + m]);
+                }
+        }
     }
+    wave_sync();  // tab may be rewritten by the caller's next use
 }
 
 __device__ inline float wave_sum_f32(float v)
