@@ -693,7 +693,12 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                 wave_sync();
                 if (lane < PrepCount<D>::N2) prep2<D>(W.g, W.a, lane, W.F);
                 wave_sync();
-                if (lane < PrepCount<D>::N3) prep3<D>(W.g, W.a, lane, W.F);
+                for (int i = lane; i < PrepCount<D>::N3 + PrepCount<D>::N4; i += 64) {
+                    if (i < PrepCount<D>::N3)
+                        prep3<D>(W.g, W.a, i, W.F);
+                    else
+                        prep4<D>(i - PrepCount<D>::N3, W.F);
+                }
                 wave_sync();
                 for (int e = lane; e < NV; e += 64) {
                     int i = e, l = e;

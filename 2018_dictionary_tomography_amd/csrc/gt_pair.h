@@ -493,6 +493,7 @@ template <int D> struct PairFin {
     static constexpr int P = Dims<D>::P, NG = Dims<D>::NG;
     double J[NG][P];
     double Kg[NG][NG];
+    double KJ[NG][P];  // Kg J
     double v[NG];
     double va, I, s, rs3, rs5, VKK;
     double t[3], T[3], z[3], qt[3], uq[3];
@@ -511,6 +512,7 @@ template <int D> struct PrepCount {
     static constexpr int N1 = NG + NG * NG + NG * P + 3;
     static constexpr int N2 = 10;
     static constexpr int N3 = 42;
+    static constexpr int N4 = NG * P;
 };
 
 // B-pair index of (b0, b1) in any order.
@@ -671,6 +673,17 @@ __host__ __device__ inline void prep3(const PairGeom<D> &g, const PairAlg<D> &a,
     }
 }
 
+// prep4: entry idx = l P + j of Kg J (needs prep1), the product every
+// Hessian entry of fin_hess contracts with a column of J.
+template <int D> __host__ __device__ inline void prep4(int idx, PairFin<D> &F)
+{
+    constexpr int NG = Dims<D>::NG, P = Dims<D>::P;
+    const int l = idx / P, j = idx - (idx / P) * P;
+    double kj = 0.0;
+    for (int m = 0; m < NG; m++) kj += F.Kg[l][m] * F.J[m][j];
+    F.KJ[l][j] = kj;
+}
+
 // `a` is caller-provided scratch (e.g. LDS) so that a thread does not have to
 // keep the whole PairAlg in registers.
 template <int D>
@@ -681,6 +694,7 @@ __host__ __device__ inline void pair_prepare(const PairGeom<D> &g, const double 
     for (int i = 0; i < PrepCount<D>::N1; i++) prep1<D>(g, a, S1, S2, gn, i, F);
     for (int i = 0; i < PrepCount<D>::N2; i++) prep2<D>(g, a, i, F);
     for (int i = 0; i < PrepCount<D>::N3; i++) prep3<D>(g, a, i, F);
+    for (int i = 0; i < PrepCount<D>::N4; i++) prep4<D>(i, F);
 }
 
 template <int D>
@@ -704,11 +718,7 @@ template <int D> __host__ __device__ inline double fin_hess(const PairFin<D> &F,
 {
     constexpr int NG = Dims<D>::NG;
     double acc = 0.0;
-    for (int l = 0; l < NG; l++) {
-        double kj = 0.0;
-        for (int m = 0; m < NG; m++) kj += F.Kg[l][m] * F.J[m][j];
-        acc += F.J[l][i] * kj;
-    }
+    for (int l = 0; l < NG; l++) acc += F.J[l][i] * F.KJ[l][j];
     if (i > j) {
         const int t = i;
         i = j;
