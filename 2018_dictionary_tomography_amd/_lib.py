@@ -18,6 +18,7 @@ GT_MODE_LINEAR, GT_MODE_L2, GT_MODE_RESID = 0, 1, 2
 GT_REG_NULL, GT_REG_JOUBERT, GT_REG_MASS, GT_REG_VOLUME, GT_REG_RADIUS, GT_REG_ISO = 0, 1, 2, 3, 4, 5
 GT_REG_NPARAM = 5
 GT_KL_CLAMP, GT_KL_LAGRANGE = 0, 1
+GT_PSF_EXACT, GT_PSF_SAMPLED = 0, 1
 
 
 class GTError(RuntimeError):
@@ -89,6 +90,9 @@ SIGNATURES = {
     "gt_psf_derivs": (_ci, [_P(gt_atoms), _P(gt_geom), _vp, _ci, _ci, _vp, _vp, _vp, _vp, _sz, _vp]),
     "gt_psf_volume_workspace": (_sz, [_P(gt_atoms), _P(gt_vol)]),
     "gt_psf_volume": (_ci, [_P(gt_atoms), _P(gt_vol), _vp, _vp, _sz, _vp]),
+    "gt_psf_forward_rule": (_ci, [_P(gt_atoms), _P(gt_geom), _ci, _vp, _vp, _sz, _vp]),
+    "gt_psf_derivs_rule": (_ci, [_P(gt_atoms), _P(gt_geom), _vp, _ci, _ci, _ci, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "gt_psf_volume_rule": (_ci, [_P(gt_atoms), _P(gt_vol), _ci, _vp, _vp, _sz, _vp]),
     "gt_sino_moments_workspace": (_sz, [_P(gt_geom)]),
     "gt_sino_moments": (_ci, [_P(gt_geom), _vp, _vp, _ci, ctypes.c_double, ctypes.c_double, _vp, _vp, _sz, _vp]),
     "gt_reg_eval": (_ci, [_P(gt_atoms), _ci, _P(ctypes.c_double), _vp, _vp, _vp, _vp]),

@@ -69,21 +69,28 @@ class GaussTomo:
 
 class SingleParticleGaussTomo(GaussTomo):
     """GaussTomo with the point-spread projector (atomFuncs.py:161-172):
-    detector values integrated over pixels, the volume over voxels."""
+    rule 'exact' integrates detector values over pixels and the volume over
+    voxels; rule 'reference' takes the reference's own samples of them
+    (psf module)."""
+
+    def __init__(self, Atoms, Volume, Sino, device="GPU", rule="exact"):
+        super().__init__(Atoms, Volume, Sino, device=device)
+        psf.check_rule(rule)
+        self.rule = rule
 
     def __call__(self, atoms, out=None):
         data = out.data if out is not None else None
-        R = psf.forward(atoms, self.range, out=data)
+        R = psf.forward(atoms, self.range, out=data, rule=self.rule)
         return out if out is not None else ProjElement(self.range, R)
 
     def discretise(self, atoms, out=None):
         data = out.data if out is not None else None
-        u = psf.volume(atoms, self.embedding, out=data)
+        u = psf.volume(atoms, self.embedding, out=data, rule=self.rule)
         return out if out is not None else VolElement(self.embedding, u)
 
     def _derivs(self, atoms, C, order, mode):
         C = C.data if hasattr(C, "data") and not isinstance(C, torch.Tensor) else C
-        f, g, H = psf.derivs(atoms, self.range, C, mode, order)
+        f, g, H = psf.derivs(atoms, self.range, C, mode, order, self.rule)
         f = f.cpu().numpy().reshape(-1, 1)
         g = g.cpu().numpy()
         if order == 1:
@@ -91,4 +98,4 @@ class SingleParticleGaussTomo(GaussTomo):
         return f, g, H.cpu().numpy()
 
     def derivs_device(self, atoms, C, order=2, mode="linear"):
-        return psf.derivs(atoms, self.range, C, mode, order)
+        return psf.derivs(atoms, self.range, C, mode, order, self.rule)

@@ -28,6 +28,19 @@
 // derivatives) are sums over pixels of weighted M_k; the Gauss-Newton term
 // sum_k dP_k dP_k^T, a product of two pixel integrals, enters through the
 // products sum_k M_k[m] M_k[m'] of the degree <= 2 monomials.
+//
+// Rule GT_PSF_SAMPLED reproduces the reference's own numbers instead
+// (__rf2 / __rf3 :413-527, __drf2 / __drf3 :924-1075, __if2 / __if3
+// :222-291): per (atom, cell) with n the distance of the (projected) centre
+// from the cell centre and R = 1 / min diag U, n + R < s gives the atom's
+// total (I (pi/30) / det U, or I sqrt(pi/30) / det U in 2D) with no
+// derivative, n - R > s gives 0, and otherwise the integrand (no cut-off) is
+// sampled at lo + k (hi - lo) / 10, k = 1..11, per axis on [lo, hi] = the
+// cell cut to [-R, R] about the centre, each sample weighted (hi - lo) / 11;
+// s is half the first detector axis' spacing for both detector axes and half
+// the second volume axis' spacing for every volume axis.  The derivatives are
+// those of the integrand at the samples (the same moment contraction, with the
+// samples for nodes).
 #include "gt_common.h"
 
 namespace gt {
@@ -78,6 +91,51 @@ struct alignas(16) PsfRec {
     float s[2];          // pixel half-widths
 };
 static_assert(sizeof(PsfRec) == 96, "PsfRec must stay 96 bytes");
+
+// R = 1 / min diag U (:25-30) and the atom's total I (pi/30) / det U, or
+// I sqrt(pi/30) / det U in 2D (:60-65): the constants of the sampled rule.
+template <int D> __device__ inline void psf_rule_consts(const PairGeom<D> &g, double &rad, double &total)
+{
+    double dmin = g.U[0][0], dprod = 1.0;
+    for (int i = 0; i < D; i++) {
+        dmin = fmin(dmin, g.U[i][i]);
+        dprod *= g.U[i][i];
+    }
+    rad = 1.0 / dmin;
+    const double c = 3.14159265358979323846 / 30.0;
+    total = g.I * (D == 3 ? c : sqrt(c)) / dprod;
+}
+
+// Record of the sampled rule: sl = (R, total), s = (s, s) with s half the
+// first detector axis' spacing; pixel box = the pixels within |R| + s of p*
+// per axis (every other pixel has n - R > s, or n >= |R| + s > s - R, so the
+// rule gives it 0).
+template <int D>
+__device__ inline void psf_setup_sampled(const PairGeom<D> &g, const float *const *det, const int *nd, float hs0,
+                                         PsfRec &R)
+{
+    constexpr int K = Dims<D>::K;
+    pair_setup<D>(g, det, nd, 0, R.r);
+    double rad, total;
+    psf_rule_consts<D>(g, rad, total);
+    const double pstar[2] = {(double)R.r.c_hi[0] + (double)R.r.c_lo[0], (double)R.r.c_hi[1] + (double)R.r.c_lo[1]};
+    R.sl[0] = (float)rad;
+    R.sl[1] = (float)total;
+    for (int k = 0; k < 2; k++) {
+        R.sh[k] = 0.0f;
+        R.sig[k] = 0.0f;
+        R.s[k] = hs0;
+        if (k >= K) {
+            R.r.lo[k] = 0;
+            R.r.hi[k] = 0;
+            continue;
+        }
+        const double reach = fabs(rad) + (double)hs0;
+        const double pad = 1e-5 * reach + 1e-7;
+        R.r.lo[k] = lower_idx(det[k], nd[k], pstar[k] - reach - pad);
+        R.r.hi[k] = upper_idx(det[k], nd[k], pstar[k] + reach + pad);
+    }
+}
 
 template <int D>
 __device__ inline void psf_setup(const PairGeom<D> &g, const float *const *det, const int *nd, const float *hs,
@@ -144,7 +202,8 @@ __device__ inline void psf_setup(const PairGeom<D> &g, const float *const *det, 
 // spacing (the reference's assumption of a uniform grid, :533, 563)
 __device__ inline float psf_cell_half(const float *v) { return 0.5f * fabsf(v[1] - v[0]); }
 
-template <int D> __global__ __launch_bounds__(256) void psf_setup_kernel(gt_atoms A, gt_geom G, PsfRec *recs)
+template <int D>
+__global__ __launch_bounds__(256) void psf_setup_kernel(gt_atoms A, gt_geom G, PsfRec *recs, int rule)
 {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long)A.n * G.nA) return;
@@ -156,7 +215,10 @@ template <int D> __global__ __launch_bounds__(256) void psf_setup_kernel(gt_atom
     const int nd[2] = {G.n0, G.n1};
     const float hs[2] = {psf_cell_half(G.p0), D == 3 ? psf_cell_half(G.p1) : 0.0f};
     PsfRec R;
-    psf_setup<D>(g, det, nd, hs, R);
+    if (rule == GT_PSF_SAMPLED)
+        psf_setup_sampled<D>(g, det, nd, hs[0], R);
+    else
+        psf_setup<D>(g, det, nd, hs, R);
     recs[idx] = R;
 }
 
@@ -388,6 +450,78 @@ __device__ inline void psf_pixel(const PsfRec &R, float pc0, float pc1, float *M
     }
 }
 
+// The sampled rule for one (atom, angle) and the pixel centred at (pc0, pc1):
+// returns the pixel's value; M[mono] = w sum over the samples of mono(d)
+// 2^q(d) (zero for total and zero pixels, whose value has no derivative).
+// Offsets d = p - p*: the pixel centre sits at c = pc - p* (the reference's
+// <x - y, W> is -c), the samples of an axis at L + k h, k = 1..11, with L =
+// max(-R, c - s), h = (min(R, c + s) - L) / 10, each of weight 10 h / 11.
+template <int K, int NM>
+__device__ inline float psf_pixel_sampled(const PsfRec &R, float pc0, float pc1, float *M)
+{
+    constexpr int NJ = (K == 1) ? NM : PsfDeg<NM>::NJ;
+#pragma unroll
+    for (int m = 0; m < NM; m++) M[m] = 0.0f;
+    const float c0 = (pc0 - R.r.c_hi[0]) - R.r.c_lo[0];
+    const float c1 = (K == 2) ? (pc1 - R.r.c_hi[1]) - R.r.c_lo[1] : 0.0f;
+    const float rad = R.sl[0], s = R.s[0];
+    const float n = (K == 2) ? sqrtf(fmaf(c0, c0, c1 * c1)) : fabsf(c0);
+    if (n + rad < s) return R.sl[1];
+    if (n - rad > s) return 0.0f;
+    const float *q = R.r.q;
+    const float L0 = fmaxf(-rad, c0 - s), h0 = 0.1f * (fminf(rad, c0 + s) - L0);
+    float w = (10.0f / 11.0f) * h0;
+    if (K == 1) {
+#pragma unroll
+        for (int k = 1; k <= 11; k++) {
+            const float d = fmaf((float)k, h0, L0);
+            float e = __builtin_amdgcn_exp2f(fmaf(fmaf(q[0], d, q[3]), d, q[5]));
+#pragma unroll
+            for (int m = 0; m < NM; m++) {
+                M[m] += e;
+                e *= d;
+            }
+        }
+    } else {
+        const float L1 = fmaxf(-rad, c1 - s), h1 = 0.1f * (fminf(rad, c1 + s) - L1);
+        w *= (10.0f / 11.0f) * h1;
+#pragma unroll 1
+        for (int k = 1; k <= 11; k++) {
+            const float d0 = fmaf((float)k, h0, L0);
+            // exponent along d1 for this d0: q2 d1^2 + (q1 d0 + q4) d1 + (q0 d0^2 + q3 d0 + q5)
+            const float b = fmaf(q[1], d0, q[4]), c = fmaf(fmaf(q[0], d0, q[3]), d0, q[5]);
+            float B[NJ];
+#pragma unroll
+            for (int j = 0; j < NJ; j++) B[j] = 0.0f;
+#pragma unroll
+            for (int l = 1; l <= 11; l++) {
+                const float d1 = fmaf((float)l, h1, L1);
+                float e = __builtin_amdgcn_exp2f(fmaf(fmaf(q[2], d1, b), d1, c));
+#pragma unroll
+                for (int j = 0; j < NJ; j++) {
+                    B[j] += e;
+                    e *= d1;
+                }
+            }
+            if (NM == 1) {
+                M[0] += B[0];
+                continue;
+            }
+            const float P0[5] = {1.0f, d0, d0 * d0, d0 * d0 * d0, d0 * d0 * d0 * d0};
+#pragma unroll
+            for (int deg = 0; deg <= 4; deg++)
+#pragma unroll
+                for (int j = 0; j <= deg; j++) {
+                    const int m = deg * (deg + 1) / 2 + j;
+                    if (m < NM) M[m] = fmaf(P0[deg - j], B[j < NJ ? j : 0], M[m]);
+                }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < NM; m++) M[m] *= w;
+    return R.r.alpha * M[0];
+}
+
 __device__ inline void psf_kahan(float &s, float &c, float v)
 {
     const float y = v - c;
@@ -427,7 +561,7 @@ __device__ inline int psf_stage(const PsfRec *__restrict__ ra, int base, int N, 
 // tiles first (block b: angle b % nA, tile of Morton rank b / nA over the
 // centre-out tile ranks, 2^e0 x 2^e1), or (angle = blockIdx.y, 256 pixels) for
 // 1D ones.
-template <int K>
+template <int K, int RULE>
 __global__ __launch_bounds__(256) void psf_forward_kernel(const PsfRec *__restrict__ recs, int N, int nA, int n0,
                                                           int n1, const float *__restrict__ p0,
                                                           const float *__restrict__ p1, float *__restrict__ R, int e0,
@@ -468,9 +602,14 @@ __global__ __launch_bounds__(256) void psf_forward_kernel(const PsfRec *__restri
         for (int e = 0; e < total; e++) {
             const PsfRec &r = list[e];
             if (!valid || r.r.lo[0] > i0 || r.r.hi[0] < i0 || r.r.lo[1] > i1 || r.r.hi[1] < i1) continue;
-            float M[1];
-            psf_pixel<K, 1>(r, pc0, pc1, M);
-            psf_kahan(s, c, r.r.alpha * M[0]);
+            float M[1], v;
+            if (RULE == GT_PSF_SAMPLED) {
+                v = psf_pixel_sampled<K, 1>(r, pc0, pc1, M);
+            } else {
+                psf_pixel<K, 1>(r, pc0, pc1, M);
+                v = r.r.alpha * M[0];
+            }
+            psf_kahan(s, c, v);
         }
         __syncthreads();
     }
@@ -478,7 +617,7 @@ __global__ __launch_bounds__(256) void psf_forward_kernel(const PsfRec *__restri
 }
 
 // Moments: one wave per (atom, angle) pair over the pixels of its box.
-template <int K, int MODE>
+template <int K, int MODE, int RULE>
 __global__ __launch_bounds__(256) void psf_moments_kernel(const PsfRec *__restrict__ recs, int N, int nA, int n0,
                                                           int n1, const float *__restrict__ p0,
                                                           const float *__restrict__ p1, const float *__restrict__ C,
@@ -500,9 +639,13 @@ __global__ __launch_bounds__(256) void psf_moments_kernel(const PsfRec *__restri
     const float *Ca = C + (size_t)a * n0 * n1;
     for (long t = lane; t < count; t += 64) {
         const int i0 = R.r.lo[0] + (int)(t / cols), i1 = R.r.lo[1] + (int)(t % cols);
-        float M[NM];
-        psf_pixel<K, NM>(R, p0[i0], (K == 2) ? p1[i1] : 0.0f, M);
-        const float P = R.r.alpha * M[0];
+        float M[NM], P;
+        if (RULE == GT_PSF_SAMPLED) {
+            P = psf_pixel_sampled<K, NM>(R, p0[i0], (K == 2) ? p1[i1] : 0.0f, M);
+        } else {
+            psf_pixel<K, NM>(R, p0[i0], (K == 2) ? p1[i1] : 0.0f, M);
+            P = R.r.alpha * M[0];
+        }
         const float cv = Ca[(size_t)i0 * n1 + i1];
         if (MODE == GT_MODE_L2) {
             const float res = P - cv;
@@ -680,7 +823,10 @@ static_assert(sizeof(PsfVolRec) == 96, "PsfVolRec must stay 96 bytes");
 
 constexpr double PSF_SQRT_NEG_KAPPA = 6.578818071009935;  // sqrt(30 log2 e)
 
-template <int D> __global__ __launch_bounds__(256) void psf_vol_setup_kernel(gt_atoms A, gt_vol V, PsfVolRec *recs)
+// Rule GT_PSF_SAMPLED: hw = (R, total, s) with s half the second axis'
+// spacing, voxel box = the voxels within |R| + s of x per axis.
+template <int D>
+__global__ __launch_bounds__(256) void psf_vol_setup_kernel(gt_atoms A, gt_vol V, PsfVolRec *recs, int rule)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= A.n) return;
@@ -699,6 +845,27 @@ template <int D> __global__ __launch_bounds__(256) void psf_vol_setup_kernel(gt_
     R.pad[0] = R.pad[1] = 0.0f;
     const float *y[3] = {V.y0, V.y1, V.y2};
     const int m[3] = {V.m0, V.m1, V.m2};
+    if (rule == GT_PSF_SAMPLED) {
+        double rad, total;
+        psf_rule_consts<D>(g, rad, total);
+        const float sv = psf_cell_half(V.y1);
+        R.hw[0] = (float)rad;
+        R.hw[1] = (float)total;
+        R.hw[2] = sv;
+        for (int c = 0; c < 3; c++) {
+            if (c >= D) {
+                R.lo[c] = 0;
+                R.hi[c] = 0;
+                continue;
+            }
+            const double reach = fabs(rad) + (double)sv;
+            const double pad = 1e-5 * reach + 1e-7;
+            R.lo[c] = lower_idx(y[c], m[c], g.x[c] - reach - pad);
+            R.hi[c] = upper_idx(y[c], m[c], g.x[c] + reach + pad);
+        }
+        recs[j] = R;
+        return;
+    }
     const float hv[3] = {psf_cell_half(V.y0), psf_cell_half(V.y1), D == 3 ? psf_cell_half(V.y2) : 0.0f};
     double Ui[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     bool ok = true;
@@ -835,6 +1002,53 @@ template <int D> __device__ inline float psf_voxel(const PsfVolRec &R, const flo
     return R.amp * acc;
 }
 
+// The sampled rule for one atom and the voxel centred at yc: n = |yc - x|,
+// n + R < s gives the atom's total, n - R > s gives 0, otherwise amp times
+// the samples' sum of exp(-30 |U d|^2) at d = L + k h per axis (k = 1..11, L =
+// max(-R, c - s), h = (min(R, c + s) - L) / 10, c = yc - x), each of weight
+// prod 10 h / 11.
+template <int D> __device__ inline float psf_voxel_sampled(const PsfVolRec &R, const float *yc)
+{
+    float c[3] = {0.0f, 0.0f, 0.0f}, L[3] = {0.0f, 0.0f, 0.0f}, h[3] = {0.0f, 0.0f, 0.0f};
+    float n2 = 0.0f;
+    for (int i = 0; i < D; i++) {
+        c[i] = (yc[i] - R.xh[i]) - R.xl[i];
+        n2 = fmaf(c[i], c[i], n2);
+    }
+    const float rad = R.hw[0], s = R.hw[2], n = sqrtf(n2);
+    if (n + rad < s) return R.hw[1];
+    if (n - rad > s) return 0.0f;
+    float w = R.amp;
+    for (int i = 0; i < D; i++) {
+        L[i] = fmaxf(-rad, c[i] - s);
+        h[i] = 0.1f * (fminf(rad, c[i] + s) - L[i]);
+        w *= (10.0f / 11.0f) * h[i];
+    }
+    float acc = 0.0f, dy[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll 1
+    for (int a = 1; a <= 11; a++) {
+        dy[0] = fmaf((float)a, h[0], L[0]);
+        if (D == 2) {
+#pragma unroll
+            for (int b = 1; b <= 11; b++) {
+                dy[1] = fmaf((float)b, h[1], L[1]);
+                acc += __builtin_amdgcn_exp2f(psf_vexp<D>(R, dy));
+            }
+            continue;
+        }
+#pragma unroll 1
+        for (int b = 1; b <= 11; b++) {
+            dy[1] = fmaf((float)b, h[1], L[1]);
+#pragma unroll
+            for (int k = 1; k <= 11; k++) {
+                dy[2] = fmaf((float)k, h[2], L[2]);
+                acc += __builtin_amdgcn_exp2f(psf_vexp<D>(R, dy));
+            }
+        }
+    }
+    return w * acc;
+}
+
 // Block = brick of 4 x 8 x 8 voxels (3D) or 16 x 16 (2D), thread = voxel,
 // PV_GROUPS groups of 256 threads on the same brick: group g takes every
 // PV_GROUPS-th staged record, and the groups' compensated sums are added in
@@ -844,7 +1058,7 @@ template <int D> __device__ inline float psf_voxel(const PsfVolRec &R, const flo
 // at once the light outer bricks fill the tail.
 constexpr int PV_GROUPS = 4, PV_THREADS = PV_GROUPS * 256, PV_CHUNK = 256;
 
-template <int D>
+template <int D, int RULE>
 __global__ __launch_bounds__(PV_THREADS) void psf_volume_kernel(const PsfVolRec *__restrict__ recs, int N, gt_vol V,
                                                                 float *__restrict__ u, int e0, int e1, int e2)
 {
@@ -891,7 +1105,7 @@ __global__ __launch_bounds__(PV_THREADS) void psf_volume_kernel(const PsfVolRec 
             if (!valid || r.lo[0] > i0 || r.hi[0] < i0 || r.lo[1] > i1 || r.hi[1] < i1 || r.lo[2] > i2 ||
                 r.hi[2] < i2)
                 continue;
-            psf_kahan(s, c, psf_voxel<D>(r, yc, hv));
+            psf_kahan(s, c, RULE == GT_PSF_SAMPLED ? psf_voxel_sampled<D>(r, yc) : psf_voxel<D>(r, yc, hv));
         }
         __syncthreads();
     }
@@ -925,11 +1139,14 @@ extern "C" size_t gt_psf_forward_workspace(const gt_atoms *atoms, const gt_geom 
     return psf_rec_bytes(atoms, geom);
 }
 
-static int psf_prepare(const gt_atoms *atoms, const gt_geom *geom, void *ws, size_t ws_bytes, size_t need,
+static int psf_prepare(const gt_atoms *atoms, const gt_geom *geom, int rule, void *ws, size_t ws_bytes, size_t need,
                        hipStream_t st, const char *who)
 {
     int rc;
     if ((rc = check_atoms(atoms)) || (rc = check_geom(geom, atoms->dim))) return rc;
+    GT_CHECK_ARG(rule == GT_PSF_EXACT || rule == GT_PSF_SAMPLED,
+                 "%s: rule must be %d (pixel integral) or %d (the reference's samples), got %d", who, GT_PSF_EXACT,
+                 GT_PSF_SAMPLED, rule);
     GT_CHECK_ARG(geom->sorted == 1, "%s: the detector coordinates must ascend (uniform pixels)", who);
     if (need > ws_bytes || (need > 0 && ws == nullptr)) {
         set_error("%s: workspace of %zu bytes given, %zu needed", who, ws_bytes, need);
@@ -943,15 +1160,15 @@ static int psf_prepare(const gt_atoms *atoms, const gt_geom *geom, void *ws, siz
     const long pairs = (long)atoms->n * geom->nA;
     const dim3 grid((unsigned)((pairs + 255) / 256));
     if (atoms->dim == 3)
-        hipLaunchKernelGGL(psf_setup_kernel<3>, grid, dim3(256), 0, st, *atoms, *geom, recs);
+        hipLaunchKernelGGL(psf_setup_kernel<3>, grid, dim3(256), 0, st, *atoms, *geom, recs, rule);
     else
-        hipLaunchKernelGGL(psf_setup_kernel<2>, grid, dim3(256), 0, st, *atoms, *geom, recs);
+        hipLaunchKernelGGL(psf_setup_kernel<2>, grid, dim3(256), 0, st, *atoms, *geom, recs, rule);
     GT_HIP(hipGetLastError());
     return 0;
 }
 
-extern "C" int gt_psf_forward(const gt_atoms *atoms, const gt_geom *geom, float *R, void *ws, size_t ws_bytes,
-                              void *stream)
+extern "C" int gt_psf_forward_rule(const gt_atoms *atoms, const gt_geom *geom, int rule, float *R, void *ws,
+                                   size_t ws_bytes, void *stream)
 {
     int rc;
     hipStream_t st = (hipStream_t)stream;
@@ -959,7 +1176,8 @@ extern "C" int gt_psf_forward(const gt_atoms *atoms, const gt_geom *geom, float 
         set_error("gt_psf_forward: atoms or geometry is NULL");
         return GT_EINVAL;
     }
-    if ((rc = psf_prepare(atoms, geom, ws, ws_bytes, gt_psf_forward_workspace(atoms, geom), st, "gt_psf_forward")))
+    if ((rc = psf_prepare(atoms, geom, rule, ws, ws_bytes, gt_psf_forward_workspace(atoms, geom), st,
+                          "gt_psf_forward")))
         return rc;
     GT_CHECK_ARG(R != nullptr, "gt_psf_forward: R is NULL");
     const int n1 = atoms->dim == 3 ? geom->n1 : 1;
@@ -976,16 +1194,30 @@ extern "C" int gt_psf_forward(const gt_atoms *atoms, const gt_geom *geom, float 
         const int e0 = ceil_log2((geom->n0 + 15) / 16), e1 = ceil_log2((n1 + 15) / 16);
         GT_CHECK_ARG(((unsigned long long)geom->nA << (e0 + e1)) < (1ull << 31), "gt_psf_forward: too many tiles");
         const dim3 grid((unsigned)((unsigned long long)geom->nA << (e0 + e1)));
-        hipLaunchKernelGGL(psf_forward_kernel<2>, grid, dim3(256), 0, st, recs, atoms->n, geom->nA, geom->n0, n1,
-                           geom->p0, geom->p1, R, e0, e1);
+        if (rule == GT_PSF_SAMPLED)
+            hipLaunchKernelGGL((psf_forward_kernel<2, GT_PSF_SAMPLED>), grid, dim3(256), 0, st, recs, atoms->n,
+                               geom->nA, geom->n0, n1, geom->p0, geom->p1, R, e0, e1);
+        else
+            hipLaunchKernelGGL((psf_forward_kernel<2, GT_PSF_EXACT>), grid, dim3(256), 0, st, recs, atoms->n, geom->nA,
+                               geom->n0, n1, geom->p0, geom->p1, R, e0, e1);
     } else {
         const dim3 grid((unsigned)((geom->n0 + 255) / 256), (unsigned)geom->nA);
-        hipLaunchKernelGGL(psf_forward_kernel<1>, grid, dim3(256), 0, st, recs, atoms->n, geom->nA, geom->n0, 1,
-                           geom->p0, geom->p0, R, 0, 0);
+        if (rule == GT_PSF_SAMPLED)
+            hipLaunchKernelGGL((psf_forward_kernel<1, GT_PSF_SAMPLED>), grid, dim3(256), 0, st, recs, atoms->n,
+                               geom->nA, geom->n0, 1, geom->p0, geom->p0, R, 0, 0);
+        else
+            hipLaunchKernelGGL((psf_forward_kernel<1, GT_PSF_EXACT>), grid, dim3(256), 0, st, recs, atoms->n, geom->nA,
+                               geom->n0, 1, geom->p0, geom->p0, R, 0, 0);
     }
     prof_end(st, GT_K_FORWARD);
     GT_HIP(hipGetLastError());
     return 0;
+}
+
+extern "C" int gt_psf_forward(const gt_atoms *atoms, const gt_geom *geom, float *R, void *ws, size_t ws_bytes,
+                              void *stream)
+{
+    return gt_psf_forward_rule(atoms, geom, GT_PSF_EXACT, R, ws, ws_bytes, stream);
 }
 
 extern "C" size_t gt_psf_derivs_workspace(const gt_atoms *atoms, const gt_geom *geom)
@@ -995,8 +1227,20 @@ extern "C" size_t gt_psf_derivs_workspace(const gt_atoms *atoms, const gt_geom *
            align_up((size_t)geom->nA * sizeof(double));
 }
 
-extern "C" int gt_psf_derivs(const gt_atoms *atoms, const gt_geom *geom, const float *C, int mode, int order,
-                             double *f, double *g, double *H, void *ws, size_t ws_bytes, void *stream)
+template <int K, int MODE>
+static void psf_launch_moments(int rule, dim3 mg, hipStream_t st, const PsfRec *recs, int N, int nA, int n0, int n1,
+                               const float *p0, const float *p1, const float *C, int order, float *mom)
+{
+    if (rule == GT_PSF_SAMPLED)
+        hipLaunchKernelGGL((psf_moments_kernel<K, MODE, GT_PSF_SAMPLED>), mg, dim3(256), 0, st, recs, N, nA, n0, n1, p0,
+                           p1, C, order, mom);
+    else
+        hipLaunchKernelGGL((psf_moments_kernel<K, MODE, GT_PSF_EXACT>), mg, dim3(256), 0, st, recs, N, nA, n0, n1, p0,
+                           p1, C, order, mom);
+}
+
+extern "C" int gt_psf_derivs_rule(const gt_atoms *atoms, const gt_geom *geom, const float *C, int mode, int order,
+                                  int rule, double *f, double *g, double *H, void *ws, size_t ws_bytes, void *stream)
 {
     int rc;
     hipStream_t st = (hipStream_t)stream;
@@ -1009,7 +1253,8 @@ extern "C" int gt_psf_derivs(const gt_atoms *atoms, const gt_geom *geom, const f
     GT_CHECK_ARG(order == 1 || order == 2, "gt_psf_derivs: order must be 1 or 2, got %d", order);
     GT_CHECK_ARG(C != nullptr || atoms->n == 0, "gt_psf_derivs: C is NULL");
     GT_CHECK_ARG(g != nullptr || atoms->n == 0, "gt_psf_derivs: g is NULL");
-    if ((rc = psf_prepare(atoms, geom, ws, ws_bytes, gt_psf_derivs_workspace(atoms, geom), st, "gt_psf_derivs")))
+    if ((rc = psf_prepare(atoms, geom, rule, ws, ws_bytes, gt_psf_derivs_workspace(atoms, geom), st,
+                          "gt_psf_derivs")))
         return rc;
     if (atoms->n == 0) return 0;
     char *w = (char *)ws;
@@ -1025,18 +1270,18 @@ extern "C" int gt_psf_derivs(const gt_atoms *atoms, const gt_geom *geom, const f
     prof_begin(st, GT_K_MOMENTS);
     if (atoms->dim == 3) {
         if (mode == GT_MODE_L2)
-            hipLaunchKernelGGL((psf_moments_kernel<2, GT_MODE_L2>), mg, dim3(256), 0, st, recs, atoms->n, geom->nA,
-                               geom->n0, n1, geom->p0, geom->p1, C, order, mom);
+            psf_launch_moments<2, GT_MODE_L2>(rule, mg, st, recs, atoms->n, geom->nA, geom->n0, n1, geom->p0, geom->p1,
+                                              C, order, mom);
         else
-            hipLaunchKernelGGL((psf_moments_kernel<2, GT_MODE_LINEAR>), mg, dim3(256), 0, st, recs, atoms->n,
-                               geom->nA, geom->n0, n1, geom->p0, geom->p1, C, order, mom);
+            psf_launch_moments<2, GT_MODE_LINEAR>(rule, mg, st, recs, atoms->n, geom->nA, geom->n0, n1, geom->p0,
+                                                  geom->p1, C, order, mom);
     } else {
         if (mode == GT_MODE_L2)
-            hipLaunchKernelGGL((psf_moments_kernel<1, GT_MODE_L2>), mg, dim3(256), 0, st, recs, atoms->n, geom->nA,
-                               geom->n0, 1, geom->p0, geom->p0, C, order, mom);
+            psf_launch_moments<1, GT_MODE_L2>(rule, mg, st, recs, atoms->n, geom->nA, geom->n0, 1, geom->p0, geom->p0,
+                                              C, order, mom);
         else
-            hipLaunchKernelGGL((psf_moments_kernel<1, GT_MODE_LINEAR>), mg, dim3(256), 0, st, recs, atoms->n,
-                               geom->nA, geom->n0, 1, geom->p0, geom->p0, C, order, mom);
+            psf_launch_moments<1, GT_MODE_LINEAR>(rule, mg, st, recs, atoms->n, geom->nA, geom->n0, 1, geom->p0,
+                                                  geom->p0, C, order, mom);
     }
     prof_end(st, GT_K_MOMENTS);
     prof_begin(st, GT_K_FINISH);
@@ -1061,6 +1306,12 @@ extern "C" int gt_psf_derivs(const gt_atoms *atoms, const gt_geom *geom, const f
     return 0;
 }
 
+extern "C" int gt_psf_derivs(const gt_atoms *atoms, const gt_geom *geom, const float *C, int mode, int order,
+                             double *f, double *g, double *H, void *ws, size_t ws_bytes, void *stream)
+{
+    return gt_psf_derivs_rule(atoms, geom, C, mode, order, GT_PSF_EXACT, f, g, H, ws, ws_bytes, stream);
+}
+
 extern "C" size_t gt_psf_volume_workspace(const gt_atoms *atoms, const gt_vol *vol)
 {
     (void)vol;
@@ -1068,11 +1319,14 @@ extern "C" size_t gt_psf_volume_workspace(const gt_atoms *atoms, const gt_vol *v
     return align_up((size_t)atoms->n * sizeof(PsfVolRec));
 }
 
-extern "C" int gt_psf_volume(const gt_atoms *atoms, const gt_vol *vol, float *u, void *ws, size_t ws_bytes,
-                             void *stream)
+extern "C" int gt_psf_volume_rule(const gt_atoms *atoms, const gt_vol *vol, int rule, float *u, void *ws,
+                                  size_t ws_bytes, void *stream)
 {
     int rc;
     if ((rc = check_atoms(atoms))) return rc;
+    GT_CHECK_ARG(rule == GT_PSF_EXACT || rule == GT_PSF_SAMPLED,
+                 "gt_psf_volume: rule must be %d (voxel integral) or %d (the reference's samples), got %d",
+                 GT_PSF_EXACT, GT_PSF_SAMPLED, rule);
     GT_CHECK_ARG(vol != nullptr && vol->dim == atoms->dim, "gt_psf_volume: volume dimension must match the atoms");
     GT_CHECK_ARG(vol->m0 >= 1 && vol->m1 >= 1 && vol->m2 >= 1, "gt_psf_volume: grid sizes must be >= 1");
     GT_CHECK_ARG(vol->dim == 3 || vol->m2 == 1, "gt_psf_volume: 2D volumes have m2 == 1");
@@ -1100,23 +1354,38 @@ extern "C" int gt_psf_volume(const gt_atoms *atoms, const gt_vol *vol, float *u,
     // bricks numbered by Morton index over 2^e0 x 2^e1 x 2^e2 centre-out ranks
     int nb[3];
     if (vol->dim == 3) {
-        hipLaunchKernelGGL(psf_vol_setup_kernel<3>, sg, dim3(256), 0, st, *atoms, *vol, recs);
+        hipLaunchKernelGGL(psf_vol_setup_kernel<3>, sg, dim3(256), 0, st, *atoms, *vol, recs, rule);
         nb[0] = (vol->m0 + 3) / 4, nb[1] = (vol->m1 + 7) / 8, nb[2] = (vol->m2 + 7) / 8;
     } else {
-        hipLaunchKernelGGL(psf_vol_setup_kernel<2>, sg, dim3(256), 0, st, *atoms, *vol, recs);
+        hipLaunchKernelGGL(psf_vol_setup_kernel<2>, sg, dim3(256), 0, st, *atoms, *vol, recs, rule);
         nb[0] = (vol->m0 + 15) / 16, nb[1] = (vol->m1 + 15) / 16, nb[2] = 1;
     }
     const int e[3] = {ceil_log2(nb[0]), ceil_log2(nb[1]), ceil_log2(nb[2])};
     GT_CHECK_ARG(e[0] + e[1] + e[2] <= 30, "gt_psf_volume: grid too large");
     const unsigned blocks = 1u << (e[0] + e[1] + e[2]);
     prof_begin(st, GT_K_VOLUME);
-    if (vol->dim == 3)
-        hipLaunchKernelGGL(psf_volume_kernel<3>, dim3(blocks), dim3(PV_THREADS), 0, st, recs, atoms->n, *vol, u, e[0],
-                           e[1], e[2]);
-    else
-        hipLaunchKernelGGL(psf_volume_kernel<2>, dim3(blocks), dim3(PV_THREADS), 0, st, recs, atoms->n, *vol, u, e[0],
-                           e[1], e[2]);
+    if (vol->dim == 3) {
+        if (rule == GT_PSF_SAMPLED)
+            hipLaunchKernelGGL((psf_volume_kernel<3, GT_PSF_SAMPLED>), dim3(blocks), dim3(PV_THREADS), 0, st, recs,
+                               atoms->n, *vol, u, e[0], e[1], e[2]);
+        else
+            hipLaunchKernelGGL((psf_volume_kernel<3, GT_PSF_EXACT>), dim3(blocks), dim3(PV_THREADS), 0, st, recs,
+                               atoms->n, *vol, u, e[0], e[1], e[2]);
+    } else {
+        if (rule == GT_PSF_SAMPLED)
+            hipLaunchKernelGGL((psf_volume_kernel<2, GT_PSF_SAMPLED>), dim3(blocks), dim3(PV_THREADS), 0, st, recs,
+                               atoms->n, *vol, u, e[0], e[1], e[2]);
+        else
+            hipLaunchKernelGGL((psf_volume_kernel<2, GT_PSF_EXACT>), dim3(blocks), dim3(PV_THREADS), 0, st, recs,
+                               atoms->n, *vol, u, e[0], e[1], e[2]);
+    }
     prof_end(st, GT_K_VOLUME);
     GT_HIP(hipGetLastError());
     return 0;
+}
+
+extern "C" int gt_psf_volume(const gt_atoms *atoms, const gt_vol *vol, float *u, void *ws, size_t ws_bytes,
+                             void *stream)
+{
+    return gt_psf_volume_rule(atoms, vol, GT_PSF_EXACT, u, ws, ws_bytes, stream);
 }

@@ -273,8 +273,14 @@ class Kernels:
 
 POINT = Kernels(ops.forward, lambda a, s, C, order: ops.derivs(a, s, C, "L2", order),
                 lambda a, s, res: ops.derivs(a, s, res, "resid", order=1, want_f=False)[1])
-PIXEL = Kernels(psf.forward, lambda a, s, C, order: psf.derivs(a, s, C, "L2", order),
-                lambda a, s, res: psf.derivs(a, s, res, "linear", 1)[1])
+def pixel_kernels(rule="exact"):
+    """The point-spread kernels with cell rule `rule` (psf module)."""
+    return Kernels(lambda a, s: psf.forward(a, s, rule=rule),
+                   lambda a, s, C, order: psf.derivs(a, s, C, "L2", order, rule),
+                   lambda a, s, res: psf.derivs(a, s, res, "linear", 1, rule)[1])
+
+
+PIXEL = pixel_kernels("exact")
 
 
 def fourier_kernels(vol):
@@ -294,13 +300,16 @@ def fourier_kernels(vol):
 
 
 def kernels_of(Radon):
-    """POINT for GaussTomo (and a bare projection space), PIXEL for
-    SingleParticleGaussTomo, the Fourier-plane kernels for GaussFTVolume."""
+    """POINT for GaussTomo (and a bare projection space), the point-spread
+    kernels of its rule for SingleParticleGaussTomo, the Fourier-plane kernels
+    for GaussFTVolume."""
     from .atomFuncs import SingleParticleGaussTomo
     from .fourier import GaussFTVolume
     if isinstance(Radon, GaussFTVolume):
         return fourier_kernels(Radon)
-    return PIXEL if isinstance(Radon, SingleParticleGaussTomo) else POINT
+    if isinstance(Radon, SingleParticleGaussTomo):
+        return PIXEL if Radon.rule == "exact" else pixel_kernels(Radon.rule)
+    return POINT
 
 
 def _sumsq(R, data):

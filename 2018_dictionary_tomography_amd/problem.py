@@ -22,6 +22,8 @@ ground truth the data are zero, as in the reference.
 """
 from __future__ import annotations
 
+import functools
+
 import numpy as np
 import torch
 
@@ -228,10 +230,10 @@ def standardGaussTomo(gt=None, data=None, noise=None, dim=3, device="GPU", isotr
 def standardSingleParticleGaussTomo(gt=None, data=None, noise=None, dim=3, device="GPU", isotropic=False,
                                     vol_box=None, vol_size=None, angles=None, angle_range=None, angle_num=None,
                                     axis=(0, 0, 1), det_box=None, det_size=None, fidelity="l2_squared", reg=None,
-                                    solver="Newton"):
+                                    solver="Newton", rule="exact"):
     """GaussDictCode/__init__.py:72-122: standardGaussTomo with the
-    point-spread projector (SingleParticleGaussTomo).  Data simulated from
-    atoms as ground truth use that projector."""
+    point-spread projector (SingleParticleGaussTomo with cell rule `rule`).
+    Data simulated from atoms as ground truth use that projector."""
     return standardGaussTomo(gt, data, noise, dim, device, isotropic, vol_box, vol_size, angles, angle_range,
                              angle_num, axis, det_box, det_size, fidelity, reg, solver,
-                             _operator=SingleParticleGaussTomo)
+                             _operator=functools.partial(SingleParticleGaussTomo, rule=rule))

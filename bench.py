@@ -362,6 +362,10 @@ def extras_leg(torch, gtp, dev, reps):
     vol = gtp.VolSpace(box=(-1, 1), shape=64, dim=3)
     u = psf.volume(atoms, vol)
     ms_v = _timed(torch, lambda: psf.volume(atoms, vol, out=u), reps)
+    # the reference's own sampled cell rule on the same workload
+    ms_fs = _timed(torch, lambda: psf.forward(atoms, space, out=P, rule="reference"), reps)
+    ms_ds = _timed(torch, lambda: psf.derivs(atoms, space, C, "L2", 2, rule="reference"), reps)
+    ms_vs = _timed(torch, lambda: psf.volume(atoms, vol, out=u, rule="reference"), reps)
     vol128 = gtp.VolSpace(box=(-1, 1), shape=128, dim=3)
     u128 = gtp.ops.volume(atoms, vol128)
     ms_pv = _timed(torch, lambda: gtp.ops.volume(atoms, vol128, out=u128), reps)
@@ -382,7 +386,10 @@ def extras_leg(torch, gtp, dev, reps):
     out["psf"] = {"workload": "1000 atoms, 61 x 128^2, pixel-integrated; voxel-integrated render on 64^3",
                   "forward_ms": ms_f, "derivs_L2_ms": ms_d, "volume_ms": ms_v,
                   "forward_atom_pixel_per_s": 1000 * rays / (ms_f * 1e-3),
-                  "derivs_atom_pixel_per_s": 1000 * rays / (ms_d * 1e-3)}
+                  "derivs_atom_pixel_per_s": 1000 * rays / (ms_d * 1e-3),
+                  "reference_rule": {"forward_ms": ms_fs, "derivs_L2_ms": ms_ds, "volume_ms": ms_vs,
+                                     "note": "11 (x 11 (x 11)) samples per cell, total / zero cells as "
+                                             "point_spread_functions.py"}}
     I, x, r, _ = W.draw_aniso3(2000, rng)
     nI, nx, nr = Fo.analytic_gauss_fourier(I, x, r * np.sqrt(60.0))
     ft = Fo.FTAtoms(nI, nx, nr, dev)

@@ -183,6 +183,38 @@ size_t gt_psf_volume_workspace(const gt_atoms *atoms, const gt_vol *vol);
 int gt_psf_volume(const gt_atoms *atoms, const gt_vol *vol, float *u, void *ws, size_t ws_bytes, void *stream);
 
 /*
+ * The same three operators with a choice of cell rule (workspaces as above):
+ *   GT_PSF_EXACT    the pixel / voxel integral (what gt_psf_forward,
+ *                   gt_psf_derivs and gt_psf_volume compute);
+ *   GT_PSF_SAMPLED  the reference's own numbers: __rf2 / __rf3
+ *                   (point_spread_functions.py:413-527) for RadProj,
+ *                   __drf2 / __drf3 (:924-1075) for the derivatives and
+ *                   __if2 / __if3 (:222-291) for VolProj.  Per (atom, cell),
+ *                   with n the distance of the (projected) atom centre from
+ *                   the cell centre and R = 1 / min diag U: n + R < s gives
+ *                   the atom's total I (pi/30) / det U (2D: I sqrt(pi/30) /
+ *                   det U) and no derivative, n - R > s gives 0, otherwise
+ *                   the integrand is sampled at lo + k (hi - lo) / 10,
+ *                   k = 1..11, per axis of [lo, hi] = the cell cut to
+ *                   [-R, R] about the centre, each sample weighted
+ *                   (hi - lo) / 11; s is half the first detector axis'
+ *                   spacing on both detector axes, half the second volume
+ *                   axis' spacing on every volume axis.  Detector frames are
+ *                   taken as orthonormal (n from the detector offsets).
+ *                   Isotropic derivatives use the exact chain rule (the
+ *                   reference's collapse, :1101-1113, drops terms).
+ * Any other rule value returns GT_EINVAL.
+ */
+#define GT_PSF_EXACT 0
+#define GT_PSF_SAMPLED 1
+int gt_psf_forward_rule(const gt_atoms *atoms, const gt_geom *geom, int rule, float *R, void *ws, size_t ws_bytes,
+                        void *stream);
+int gt_psf_derivs_rule(const gt_atoms *atoms, const gt_geom *geom, const float *C, int mode, int order, int rule,
+                       double *f, double *g, double *H, void *ws, size_t ws_bytes, void *stream);
+int gt_psf_volume_rule(const gt_atoms *atoms, const gt_vol *vol, int rule, float *u, void *ws, size_t ws_bytes,
+                       void *stream);
+
+/*
  * Sinogram moments for the KL moment initialiser (KL_GaussRadon.py:57-190):
  * replaces MOM0 / MOM1 / MOM2 (:221-392) and the f updates of
  * doKL_ProjGDStep_iso (:128) and doKL_LagrangeStep_iso (Newton_sino_2D,

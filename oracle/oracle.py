@@ -20,6 +20,10 @@ reference in NumPy:
                                          GD_lib.py:492-536 (``_step``)
 * ``newton_fit`` (also block variant)    NewtonGaussian.py:18-237, any regulariser
 * ``gd_linesearch_block``                GD_lib.py:82-178
+* ``psf_radproj`` / ``psf_derivs`` / ``psf_volproj``  pixel and voxel
+  integrals of the point projector (point_spread_functions.py:168-1350);
+  ``psf_radproj_sampled`` / ``psf_derivs_sampled`` / ``psf_volproj_sampled``
+  the reference's own sampled rule (:25-88, 201-291, 389-527, 924-1075)
 * ``Reg`` (regulariser terms)            GaussDictCode/regularisation.py:22-442;
   values restated from the ``__call__`` methods, gradients and Hessians by
   FP64 automatic differentiation of those values (torch.autograd on the CPU)
@@ -460,7 +464,8 @@ def newton_linesearch(I, x, r, geom, data, max_iter, tol=1e-4, min_iter=1, seed=
 def _projector(kind, n_sub):
     """(radproj, derivs) of the point projector (GaussTomo), of the
     pixel-integrated one (SingleParticleGaussTomo, n_sub nodes per pixel
-    and axis), or of the Fourier-slice planes (GaussFTVolume: 'geometry' is
+    and axis), of the reference's sampled point-spread rule
+    ('psf_reference'), or of the Fourier-slice planes (GaussFTVolume: 'geometry' is
     the sample tuple (k0, k1, w0, w1), values cut at |L k|^2 < 30 and L2
     derivatives at 60, FT-space atoms).  A pair of callables (radproj(I, x,
     r, geom), derivs(I, x, r, geom, C, mode, order)) is used as given: the
@@ -473,6 +478,8 @@ def _projector(kind, n_sub):
     if kind == "psf":
         return (lambda I, x, r, g: psf_radproj(I, x, r, g, n_sub),
                 lambda I, x, r, g, C, mode, order: psf_derivs(I, x, r, g, C, mode, order, n_sub))
+    if kind == "psf_reference":
+        return psf_radproj_sampled, psf_derivs_sampled
     if kind == "ft":
         def fwd(I, x, r, g):
             return ft_hyperplane(I, x, r, g[0], g[1], g[2], g[3], iso=np.asarray(r).shape[-1] == 1)
@@ -1062,6 +1069,228 @@ def psf_volproj(I, x, r, coords, n_sub=12):
     if len(coords) == 2:
         return np.einsum("iajb,a,b->ij", uf, cw[0], cw[1])
     return np.einsum("iajbkc,a,b,c->ijk", uf, cw[0], cw[1], cw[2])
+
+
+# ---------------------------------------------------------------------------
+# Point-spread projector, the reference's own sampled rule
+# (GaussDictCode/bin/point_spread_functions.py:25-88, 201-291, 389-527,
+# 924-1075), FP64.  Values and derivatives come from one torch function of
+# the atom's parameters so that the derivatives are those of the sampled sum
+# itself (automatic differentiation with the samples held fixed).
+# ---------------------------------------------------------------------------
+def _psf_U_torch(rs, D):
+    """U from the reference's slot order (:201-216, 389-404): 2D
+    [[r0, r2], [0, r1]], 3D [[r0, r3, r5], [0, r1, r4], [0, 0, r2]]."""
+    import torch
+    z = torch.zeros((), dtype=rs.dtype)
+    if D == 2:
+        return torch.stack([torch.stack([rs[0], rs[2]]), torch.stack([z, rs[1]])])
+    return torch.stack([torch.stack([rs[0], rs[3], rs[5]]), torch.stack([z, rs[1], rs[4]]),
+                        torch.stack([z, z, rs[2]])])
+
+
+def _psf_frames(geom):
+    """Per angle (T, W0, W1) as the reference's kernels see them (:530-566,
+    1096-1097, 1255-1263): 2D two-vectors and no W1; 2.5D T and W0 with a zero
+    third entry and W1 = e_z; 3D as given."""
+    D, T, lenT, w0, w1, p0, p1 = _geom_args(geom)
+    out = []
+    for a in range(T.shape[0]):
+        if D == 2:
+            out.append((T[a], w0[a], None))
+        elif lenT == 2:
+            out.append((np.array([T[a, 0], T[a, 1], 0.0]), np.array([w0[a, 0], w0[a, 1], 0.0]),
+                        np.array([0.0, 0.0, 1.0])))
+        else:
+            out.append((T[a], w0[a], w1[a]))
+    return out
+
+
+def _psf_rule_axis(c, R, s):
+    """Sample offsets from the cell centre and the axis weight of the
+    reference's rule along one axis (:234-253, 440-458, 495-526): c = the
+    atom's offset <x - y, W> from the cell centre, limits lo = max(c - R, -s)
+    and hi = min(c + R, s), samples lo + k (hi - lo) / 10 for k = 1..11 (the
+    first one step inside lo, the last one step beyond hi), each of weight
+    (hi - lo) / 11 (the mean of the samples times hi - lo)."""
+    lo = np.maximum(c - R, -s)
+    hi = np.minimum(c + R, s)
+    k = np.arange(1, 12, dtype=np.float64)
+    return lo[..., None] + (0.1 * (hi - lo))[..., None] * k, (hi - lo) / 11.0
+
+
+def _psf_atom_consts(tv, D):
+    """R = 1 / min diag U (:25-30) and the atom's total, I (pi/30) / det U in
+    3D and I sqrt(pi/30) / det U in 2D (:60-65), from the parameter values."""
+    d = np.array([tv[1 + D + i] for i in range(D)])
+    c = np.pi / 30 if D == 3 else np.sqrt(np.pi / 30)
+    return 1.0 / d.min(), tv[0] * c / np.prod(d)
+
+
+def _psf_sampled_atom(th, geom):
+    """Sampled pixel values P[a, k0(, k1)] of one atom th = [I, x, U slots]
+    (torch FP64 vector), the reference's __rf2 / __rf3 (:413-527) per pixel:
+    n = distance of the projected centre from the pixel centre (__projnorm,
+    :77-88); n + R < s gives the atom's total, n - R > s gives 0, otherwise
+    the line integral I exp(-30 m) / |U T| (__Rf, :389-410, no cut-off) at
+    the 11 (2D) or 11 x 11 samples of _psf_rule_axis, with s half the first
+    detector axis' spacing for both axes.  Cases, samples and weights come
+    from th's values and are held fixed, so derivatives are those of the
+    integrand at the samples (__drf2 / __drf3, :924-1075; total pixels have
+    none)."""
+    import torch
+    D, T, lenT, w0, w1, p0, p1 = _geom_args(geom)
+    tv = th.detach().numpy()
+    x = th[1:1 + D]
+    U = _psf_U_torch(th[1 + D:], D)
+    xv = tv[1:1 + D]
+    R, total = _psf_atom_consts(tv, D)
+    s = abs(p0[1] - p0[0]) / 2
+    frames = _psf_frames(geom)
+    nA = len(frames)
+    npix = p0.size * (p1.size if D == 3 else 1)
+    rows = []
+    for a in range(nA):
+        Ta, W0, W1 = frames[a]
+        if D == 2:
+            Y = p0[:, None] * W0[None, :]
+        else:
+            Y = (p0[:, None, None] * W0 + p1[None, :, None] * W1).reshape(-1, 3)
+        X = xv[None, :] - Y
+        t2 = (X * X).sum(1) - (X @ Ta) ** 2
+        n = np.sqrt(np.where(t2 > 0, t2, 0.0))
+        tot = n + R < s
+        zero = ~tot & (n - R > s)
+        samp = np.nonzero(~(tot | zero))[0]
+        P = torch.zeros(npix, dtype=torch.float64)
+        P = P.index_put((torch.from_numpy(np.nonzero(tot)[0]),), torch.full((int(tot.sum()),), total,
+                                                                           dtype=torch.float64))
+        if samp.size:
+            Xs = X[samp]
+            t0, wt = _psf_rule_axis(Xs @ W0, R, s)
+            if D == 2:
+                pts = Y[samp][:, None, :] + t0[:, :, None] * W0
+            else:
+                t1, wt1 = _psf_rule_axis(Xs @ W1, R, s)
+                pts = (Y[samp][:, None, None, :] + t0[:, :, None, None] * W0 + t1[:, None, :, None] * W1)
+                wt = wt * wt1
+            Yt = torch.from_numpy(pts) - x
+            rY = Yt @ U.T
+            rT = U @ torch.from_numpy(Ta)
+            inv = 1.0 / torch.sqrt((rT * rT).sum())
+            rTh = rT * inv
+            MY = rY - (rY @ rTh)[..., None] * rTh
+            m = (MY * MY).sum(-1)
+            v = th[0] * torch.exp(-30.0 * m) * inv
+            vals = v.reshape(samp.size, -1).sum(1) * torch.from_numpy(wt)
+            P = P.index_put((torch.from_numpy(samp),), vals)
+        rows.append(P)
+    shape = [nA, p0.size] + ([p1.size] if D == 3 else [])
+    return torch.stack(rows).reshape(shape)
+
+
+def _psf_theta(I, x, r, D):
+    I, x = _c64(I).reshape(-1), _c64(x).reshape(-1, D)
+    r = np.asarray(r, dtype=np.float64).reshape(I.size, -1)
+    iso = r.shape[1] == 1
+    return np.concatenate([I[:, None], x, _c64(to_aniso(r, D))], 1), iso
+
+
+def psf_radproj_sampled(I, x, r, geom):
+    """RadProj (point_spread_functions.py:371-385) with the reference's own
+    sampled pixel rule (_psf_sampled_atom), summed over atoms, FP64."""
+    import torch
+    th, _ = _psf_theta(I, x, r, geom.dim)
+    out = np.zeros(geom.shape)
+    with torch.no_grad():
+        for j in range(th.shape[0]):
+            out += _psf_sampled_atom(torch.from_numpy(th[j]), geom).numpy()
+    return out
+
+
+def psf_derivs_sampled(I, x, r, geom, C, mode="L2", order=2):
+    """Per atom (f, Df, DDf) of f = 1/2 sum (P - C)^2 (mode 'L2',
+    L2derivs_RadProj :1077-1115) or f = sum P C ('linear', derivs_RadProj
+    :1314-1350) with P the sampled rule: FP64 automatic differentiation of
+    _psf_sampled_atom (samples held fixed).  Isotropic atoms are
+    differentiated as U = rho Id and collapsed by the chain rule (the
+    reference's own collapse, :1101-1113, drops terms)."""
+    import torch
+    D = geom.dim
+    th, iso = _psf_theta(I, x, r, D)
+    Ct = torch.from_numpy(_c64(C).reshape(geom.shape))
+    N, P = th.shape
+    f = np.zeros(N)
+    Df = np.zeros((N, P))
+    DDf = np.zeros((N, P, P))
+
+    def fun(t):
+        Pj = _psf_sampled_atom(t, geom)
+        if mode == "L2":
+            d = Pj - Ct
+            return 0.5 * (d * d).sum()
+        return (Pj * Ct).sum()
+
+    for j in range(N):
+        t = torch.from_numpy(th[j].copy()).requires_grad_(True)
+        val = fun(t)
+        f[j] = val.item()
+        if not val.requires_grad:
+            continue          # no sampled pixel: total and zero pixels have no derivative
+        (g,) = torch.autograd.grad(val, t)
+        Df[j] = g.numpy()
+        if order > 1:
+            DDf[j] = torch.autograd.functional.hessian(fun, torch.from_numpy(th[j].copy())).numpy()
+    if iso:
+        E = iso_collapse(D)
+        Df = Df @ E
+        DDf = np.einsum("ai,nab,bj->nij", E, DDf, E)
+    return f, Df, DDf
+
+
+def psf_volproj_sampled(I, x, r, coords):
+    """VolProj (point_spread_functions.py:168-197) with the reference's own
+    sampled voxel rule (__if2 / __if3, :222-291), FP64: per (atom, voxel) n =
+    |x - y|; n + R < s gives the atom's total, n - R > s gives 0, otherwise
+    I sqrt(30/pi) exp(-30 |U (y' - x)|^2) (__f, :201-219) at the 11^D samples
+    y' of _psf_rule_axis per axis, with s half the second axis' spacing for
+    every axis (:302, 324, 347, 362)."""
+    D = len(coords)
+    y = [_c64(c).reshape(-1) for c in coords]
+    th, _ = _psf_theta(I, x, r, D)
+    s = abs(y[1][1] - y[1][0]) / 2
+    grids = np.meshgrid(*y, indexing="ij")
+    Yv = np.stack([g.reshape(-1) for g in grids], 1)
+    out = np.zeros(Yv.shape[0])
+    for j in range(th.shape[0]):
+        tv = th[j]
+        xv = tv[1:1 + D]
+        R, total = _psf_atom_consts(tv, D)
+        Um = np.zeros((D, D))
+        rs = tv[1 + D:]
+        if D == 2:
+            Um[0, 0], Um[1, 1], Um[0, 1] = rs[0], rs[1], rs[2]
+        else:
+            Um[0, 0], Um[1, 1], Um[2, 2], Um[0, 1], Um[1, 2], Um[0, 2] = rs
+        X = xv[None, :] - Yv
+        n = np.sqrt((X * X).sum(1))
+        tot = n + R < s
+        zero = ~tot & (n - R > s)
+        samp = np.nonzero(~(tot | zero))[0]
+        out[tot] += total
+        if samp.size:
+            offs, wts = zip(*[_psf_rule_axis(X[samp, i], R, s) for i in range(D)])
+            if D == 2:
+                pts = np.stack(np.broadcast_arrays(Yv[samp, 0, None, None] + offs[0][:, :, None],
+                                                   Yv[samp, 1, None, None] + offs[1][:, None, :]), -1)
+            else:
+                pts = np.stack(np.broadcast_arrays(Yv[samp, 0, None, None, None] + offs[0][:, :, None, None],
+                                                   Yv[samp, 1, None, None, None] + offs[1][:, None, :, None],
+                                                   Yv[samp, 2, None, None, None] + offs[2][:, None, None, :]), -1)
+            d = (pts - xv) @ Um.T
+            v = tv[0] * np.sqrt(30 / np.pi) * np.exp(-30 * (d * d).sum(-1))
+            out[samp] += v.reshape(samp.size, -1).sum(1) * np.prod(np.stack(wts, 1), 1)
+    return out.reshape(tuple(len(c) for c in y))
 
 
 # ---------------------------------------------------------------------------

@@ -57,6 +57,17 @@ def test_argument_validation_errors(gt):
     assert lib.gt_derivs_workspace(ctypes.byref(big), ctypes.byref(gbig), 2) > 128 << 20
     assert lib.gt_forward_derivs_noalloc(ctypes.byref(big), ctypes.byref(gbig), 1, 2, 1, None, 1, None, None) == -3
     assert b"scratch" in lib.gt_last_error()
+    # an unknown point-spread cell rule is refused
+    assert lib.gt_psf_forward_rule(ctypes.byref(a), ctypes.byref(g), 5, 1, 1, 1 << 20, None) == -1
+    assert b"rule" in lib.gt_last_error()
+    assert lib.gt_psf_derivs_rule(ctypes.byref(a), ctypes.byref(g), 1, 1, 2, 9, None, 1, None, 1, 1 << 20, None) == -1
+    assert b"rule" in lib.gt_last_error()
+    assert lib.gt_psf_volume_rule(ctypes.byref(a), None, -1, 1, 1, 1 << 20, None) == -1
+    assert b"rule" in lib.gt_last_error()
+    psf = importlib.import_module("2018_dictionary_tomography_amd.psf")
+    assert psf.check_rule("reference") == psf.check_rule("sampled") == L.GT_PSF_SAMPLED
+    with pytest.raises(ValueError):
+        psf.check_rule("midpoint")
     with pytest.raises(L.GTError):
         L.check(-1, "probe")
 
