@@ -87,9 +87,6 @@ inline int ceil_log2(long v)
     return e;
 }
 
-// sin(2 pi t) and cos(2 pi t) of a phase t in turns, |t| <= 256: the
-// hardware sine and cosine, whose argument is in turns (quarter-rate, like
-// exp2); measured maximum absolute error 1.25e-7 for |t| <= 1.5 on gfx950.
 // T in global memory.  Loads and stores through a gmem<T> * are global_*
 // instructions (scalar base plus 32-bit offset, counted by the vector-memory
 // counter only).  Through a plain pointer the compiler cannot trace to a
@@ -97,6 +94,19 @@ inline int ceil_log2(long v)
 // ones: 64-bit address arithmetic per access, and counted by the LDS counter
 // too, so that every wait for an LDS read also waits for them.
 template <typename T> using gmem = __attribute__((address_space(1))) T;
+
+// Raw buffer resource over a read-only array (gfx9 word 3; no bounds beyond
+// 32-bit offsets): buffer loads add a per-lane offset, a wave-uniform scalar
+// offset and the base in the load itself, so a group of loads whose
+// addresses differ by uniform amounts needs one vector offset.
+__device__ inline __amdgpu_buffer_rsrc_t buffer_rsrc(const void *base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, -1, 0x00020000);
+}
+__device__ inline float buffer_load_f32(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
 
 // LDS written by some lanes of a wave, then read by others: order the wave's
 // LDS traffic (the lanes of a wave execute together).
@@ -107,6 +117,9 @@ __device__ inline void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// sin(2 pi t) and cos(2 pi t) of a phase t in turns, |t| <= 256: the
+// hardware sine and cosine, whose argument is in turns (quarter-rate, like
+// exp2); measured maximum absolute error 1.25e-7 for |t| <= 1.5 on gfx950.
 __device__ inline void ft_sincos_turns(float t, float &sn, float &cs)
 {
     sn = __builtin_amdgcn_sinf(t);

@@ -2,7 +2,8 @@
 //
 // A wave walks the pixels of a PairRec box in row-major order, 64 pixels per
 // step, keeping (row, col) incrementally so no integer division is done per
-// pixel.  Pixels with m > 1 contribute nothing (NewtonBases.py:49-56, 375-380).
+// pixel.  Pixels with m > 1 contribute nothing (NewtonBases.py:49-56, 375-380);
+// the moment walks take the tail just outside as it is (moment_pixel).
 #pragma once
 
 #include "gt_common.h"
@@ -308,18 +309,24 @@ __host__ __device__ inline bool support_meets_rect(const float4 &f1, const float
     return best >= (float)KAPPA - SPAN_SLACK;
 }
 
-// One pixel of pair_moments_rows: branch-free cutoff (outside the support
-// E = 0 and every accumulation adds an exact zero).  xp = (x, x^2, x^3, x^4).
+// One pixel of the moments walks, branch-free.  xp = (x, x^2, x^3, x^4).
+// The walks visit a few pixels just outside the support {m <= 1} (the rows of
+// a column chunk's span hull, or of the whole box): there E = 2^q is below
+// 2^kappa = e^-30 of the profile's peak and falls off as a Gaussian, so the
+// moment sums take it as it is instead of testing every pixel against the
+// cut-off; what that adds is below 1e-13 of the sums, under their FP32
+// resolution.  The L2 energy terms, which do not vanish outside the support,
+// are taken for pixels inside it only.
 template <int MODE>
 __device__ inline void moment_pixel(const PairRec &rec, float cv, float4 xp, float rowlin, float colt, float *A,
                                     float *B, float *acc)
 {
     const float x = xp.x;
     const float q = fmaf(fmaf(rec.q[0], x, rowlin), x, colt);
-    const bool in = q >= (float)KAPPA;
-    const float E = in ? __builtin_amdgcn_exp2f(q) : 0.0f;
+    const float E = __builtin_amdgcn_exp2f(q);
     float w1;
     if (MODE == GT_MODE_L2) {
+        const bool in = q >= (float)KAPPA;
         const float res = fmaf(rec.alpha, E, -cv);
         w1 = res * E;
         acc[30] = fmaf(in ? res : 0.0f, res, acc[30]);
@@ -339,6 +346,42 @@ __device__ inline void moment_pixel(const PairRec &rec, float cv, float4 xp, flo
         B[2] = fmaf(w2, xp.y, B[2]);
         B[3] = fmaf(w2, xp.z, B[3]);
         B[4] = fmaf(w2, xp.w, B[4]);
+    }
+}
+
+// Rows r0, r0 + step, ... <= he of one lane's column in the moments walk
+// (step = RS when RS > 0, a compile-time row-table stride).  The lane's byte
+// offset of its row and its row-table entry advance once per group of UNR
+// rows; the rows of a group sit at wave-uniform distances (u step rows) from
+// them, which the residual loads take as their scalar offsets.
+template <int MODE, int UNR, int RS>
+__device__ inline void moment_rows(const PairRec &rec, __amdgpu_buffer_rsrc_t crs, unsigned off, unsigned rowb,
+                                   const float4 *trow, int step_, int r0, int he, float rowlin, float colt, float *A,
+                                   float *B, float *acc)
+{
+    const int step = RS > 0 ? RS : step_;
+    const unsigned gstep = (unsigned)step * rowb;
+    // full groups of UNR rows, all inside the hull
+    const int last = he - (UNR - 1) * step;
+    for (; r0 <= last; r0 += UNR * step) {
+        float cv[UNR];
+        float4 xp[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            cv[u] = buffer_load_f32(crs, off, (unsigned)u * gstep);
+            xp[u] = trow[u * step];
+        }
+        off += UNR * gstep;
+        trow += UNR * step;
+#pragma unroll
+        for (int u = 0; u < UNR; u++) moment_pixel<MODE>(rec, cv[u], xp[u], rowlin, colt, A, B, acc);
+    }
+    // remaining rows one at a time (same row order)
+    for (; r0 <= he; r0 += step) {
+        const float cv = buffer_load_f32(crs, off, 0u);
+        moment_pixel<MODE>(rec, cv, *trow, rowlin, colt, A, B, acc);
+        off += gstep;
+        trow += step;
     }
 }
 
@@ -367,9 +410,10 @@ __device__ inline void pair_moments_rows(const PairRec &rec, const float *p0, co
     const int W = rec.hi[1] - rec.lo[1] + 1;
     const int Hbox = rec.hi[0] - rec.lo[0] + 1;
     if (W <= 0 || Hbox <= 0) return;
-    // 32-bit byte offsets from the wave-uniform base (scalar-base loads, one
-    // multiply-add per pixel); detector slabs are far below 4 GiB
-    const char *cbase = reinterpret_cast<const char *>(Ca);
+    // residual rows through a raw buffer resource: 32-bit byte offsets (the
+    // detector slabs are far below 4 GiB) split into a per-lane part and a
+    // wave-uniform part that the load instruction adds itself
+    const __amdgpu_buffer_rsrc_t crs = buffer_rsrc(Ca);
     const unsigned rowb = 4u * (unsigned)n1;
     for (int w0 = 0; w0 < Hbox; w0 += TROWS) {
         const int H = min(TROWS, Hbox - w0);
@@ -392,28 +436,14 @@ __device__ inline void pair_moments_rows(const PairRec &rec, const float *p0, co
             const int hs = wave_min_i32(any ? rs : H), he = wave_max_i32(any ? re : -1);
             if (he < hs || !cc.act) continue;
             float A[5] = {0, 0, 0, 0, 0}, B[5] = {0, 0, 0, 0, 0};
-            const unsigned cb0 = 4u * ((unsigned)row0 * (unsigned)n1 + (unsigned)i1);
-            const unsigned step = (unsigned)cc.R;
-            int r0 = hs + cc.ro;
-            // full groups of UNR rows, all inside the hull
-            for (; r0 + (UNR - 1) * cc.R <= he; r0 += UNR * cc.R) {
-                float cv[UNR];
-                float4 xp[UNR];
-#pragma unroll
-                for (int u = 0; u < UNR; u++) {
-                    const unsigned ru = (unsigned)r0 + (unsigned)u * step;
-                    cv[u] = *reinterpret_cast<const float *>(cbase + (cb0 + ru * rowb));
-                    xp[u] = tab[ru];
-                }
-#pragma unroll
-                for (int u = 0; u < UNR; u++) moment_pixel<MODE>(rec, cv[u], xp[u], rowlin, colt, A, B, acc);
-            }
-            // remaining rows one at a time (same row order)
-            for (; r0 <= he; r0 += cc.R) {
-                const unsigned r = (unsigned)r0;
-                const float cv = *reinterpret_cast<const float *>(cbase + (cb0 + r * rowb));
-                moment_pixel<MODE>(rec, cv, tab[r], rowlin, colt, A, B, acc);
-            }
+            const int r0 = hs + cc.ro;
+            const unsigned off = 4u * ((unsigned)(row0 + r0) * (unsigned)n1 + (unsigned)i1);
+            // two row phases (chunks of 32 columns, the common case): the
+            // row-table reads of a group take immediate offsets
+            if (cc.R == 2)
+                moment_rows<MODE, UNR, 2>(rec, crs, off, rowb, tab + r0, 2, r0, he, rowlin, colt, A, B, acc);
+            else
+                moment_rows<MODE, UNR, 0>(rec, crs, off, rowb, tab + r0, cc.R, r0, he, rowlin, colt, A, B, acc);
             const float y2 = dp1 * dp1;
             const float yp[5] = {1.0f, dp1, y2, y2 * dp1, y2 * y2};
 #pragma unroll
