@@ -58,6 +58,7 @@ constexpr int DETMAX = GT_FIT_MAX_DETECTOR;
 constexpr int NVS = 65;                          // stride of a per-angle gradient/Hessian record
 constexpr int FIT_UNR = 8;                       // detector rows in flight per lane in the pixel passes
 constexpr int FIT_TROWS = 256;                   // rows of a wave's row table (moments, energy and commit passes)
+constexpr int FIT_BOXW = 64;                     // columns per chunk of the moments walk
 constexpr double FIT_EPS = 1e-4;                 // NewtonGaussian.py:21
 constexpr double GD_EPS = 1e-8;                  // GD_lib.py:96
 constexpr long long FIT_WAIT_TICKS = 200000000;  // barrier wait limit: 2 s of the 100 MHz clock
@@ -665,8 +666,8 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                 for (int v = 0; v < MOMW; v++) acc[v] = 0.0f;
                 const gmem<float> *Ra = (gmem<float> *)S.resid + (size_t)a * slab;
                 if (K == 2)
-                    pair_moments_box<GT_MODE_RESID, FIT_UNR, FIT_TROWS>(rec, L.p0, L.p1, Ra, n1, lane, acc,
-                                                                        L.tab[wave]);
+                    pair_moments_box<GT_MODE_RESID, FIT_UNR, FIT_TROWS, FIT_BOXW>(rec, L.p0, L.p1, Ra, n1, lane,
+                                                                                  acc, L.tab[wave]);
                 else
                     pair_moments4<K, GT_MODE_RESID>(rec, L.p0, L.p1, (const float *)Ra, n1, lane, acc);
                 float mine = 0.0f;

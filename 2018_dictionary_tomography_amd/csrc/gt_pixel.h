@@ -459,13 +459,13 @@ __device__ inline void pair_moments_rows(const PairRec &rec, const float *p0, co
     wave_sync();  // tab may be rewritten by the caller's next use
 }
 
-// The same moments over the whole box, without spans: 64-column chunks, each
-// lane its rows ro, ro + R, ... of the box, in windows of at most TROWS rows
-// whose x-powers the wave first writes to its row table (tab, the
+// The same moments over the whole box, without spans: BOXW-column chunks,
+// each lane its rows ro, ro + R, ... of the box, in windows of at most TROWS
+// rows whose x-powers the wave first writes to its row table (tab, the
 // expressions of row_table).  For a wave that walks many small boxes whose
 // residual rows sit in HBM (the fitter's one atom at a time over every
 // angle), the span set-up costs more than the corners it skips.
-template <int MODE, int UNR, int TROWS>
+template <int MODE, int UNR, int TROWS, int BOXW = 64>
 __device__ inline void pair_moments_box(const PairRec &rec, const float *p0, const float *p1,
                                         const gmem<float> *__restrict__ Ca, int n1, int lane, float *acc,
                                         float4 *tab)
@@ -473,7 +473,7 @@ __device__ inline void pair_moments_box(const PairRec &rec, const float *p0, con
     const int W = rec.hi[1] - rec.lo[1] + 1;
     const int Hbox = rec.hi[0] - rec.lo[0] + 1;
     if (W <= 0 || Hbox <= 0) return;
-    const gmem<char> *cbase = reinterpret_cast<const gmem<char> *>(Ca);
+    const __amdgpu_buffer_rsrc_t crs = buffer_rsrc((const float *)Ca);
     const unsigned rowb = 4u * (unsigned)n1;
     for (int w0 = 0; w0 < Hbox; w0 += TROWS) {
         const int H = min(TROWS, Hbox - w0);
@@ -485,35 +485,21 @@ __device__ inline void pair_moments_box(const PairRec &rec, const float *p0, con
             tab[r] = make_float4(x, x2, x2 * x, x2 * x2);
         }
         wave_sync();
-        for (int cb = 0; cb < W; cb += 64) {
+        for (int cb = 0; cb < W; cb += BOXW) {
             ColChunk cc;
-            cc.init(lane, W, cb);
+            cc.init(lane, W, cb, BOXW);
             if (!cc.act) continue;
             const int i1 = rec.lo[1] + cb + cc.c;
             const float dp1 = (p1[i1] - rec.c_hi[1]) - rec.c_lo[1];
             const float rowlin = fmaf(rec.q[1], dp1, rec.q[3]);
             const float colt = fmaf(fmaf(rec.q[2], dp1, rec.q[4]), dp1, rec.q[5]);
             float A[5] = {0, 0, 0, 0, 0}, B[5] = {0, 0, 0, 0, 0};
-            const unsigned cb0 = 4u * ((unsigned)row0 * (unsigned)n1 + (unsigned)i1);
-            int r0 = cc.ro;
-            // full groups of UNR rows, all inside the window
-            for (; r0 + (UNR - 1) * cc.R < H; r0 += UNR * cc.R) {
-                float cv[UNR];
-                float4 xp[UNR];
-#pragma unroll
-                for (int u = 0; u < UNR; u++) {
-                    const unsigned r = (unsigned)(r0 + u * cc.R);
-                    cv[u] = *reinterpret_cast<const gmem<float> *>(cbase + (cb0 + r * rowb));
-                    xp[u] = tab[r];
-                }
-#pragma unroll
-                for (int u = 0; u < UNR; u++) moment_pixel<MODE>(rec, cv[u], xp[u], rowlin, colt, A, B, acc);
-            }
-            // remaining rows one at a time (same row order)
-            for (; r0 < H; r0 += cc.R) {
-                const float cv = *reinterpret_cast<const gmem<float> *>(cbase + (cb0 + (unsigned)r0 * rowb));
-                moment_pixel<MODE>(rec, cv, tab[r0], rowlin, colt, A, B, acc);
-            }
+            const int r0 = cc.ro;
+            const unsigned off = 4u * ((unsigned)(row0 + r0) * (unsigned)n1 + (unsigned)i1);
+            if (cc.R == 2)
+                moment_rows<MODE, UNR, 2>(rec, crs, off, rowb, tab + r0, 2, r0, H - 1, rowlin, colt, A, B, acc);
+            else
+                moment_rows<MODE, UNR, 0>(rec, crs, off, rowb, tab + r0, cc.R, r0, H - 1, rowlin, colt, A, B, acc);
             const float y2 = dp1 * dp1;
             const float yp[5] = {1.0f, dp1, y2, y2 * dp1, y2 * y2};
 #pragma unroll
