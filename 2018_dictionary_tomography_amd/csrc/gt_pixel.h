@@ -188,9 +188,17 @@ struct ColChunk {
     __device__ inline void init(int lane, int W, int cb, int maxw = 64)
     {
         Wc = min(maxw, W - cb);
-        R = 64 / Wc;
-        ro = lane / Wc;
-        c = lane - ro * Wc;
+        if (Wc == 32 || Wc == 64) {
+            // full chunks: shifts instead of the per-lane integer division
+            const int sh = Wc == 32 ? 5 : 6;
+            R = 64 >> sh;
+            ro = lane >> sh;
+            c = lane & (Wc - 1);
+        } else {
+            R = 64 / Wc;
+            ro = lane / Wc;
+            c = lane - ro * Wc;
+        }
         act = ro < R;
     }
 };
@@ -265,17 +273,33 @@ __host__ __device__ inline void column_span(const PairRec &rec, float rowlin, fl
 __host__ __device__ inline float span_inv2a(const PairRec &rec) { return -rec.q[0] > 0.0f ? 0.5f / -rec.q[0] : 0.0f; }
 __host__ __device__ inline float span_inv_dx(float dx) { return dx > 0.0f ? 1.0f / dx : 0.0f; }
 
+// Wave minimum / maximum through DPP (no LDS round trips): a running
+// min / max over the previous 1, 2, 4, 8 lanes of each 16-lane row (lanes
+// before the row start contribute the identity), then lane 15 of each row
+// broadcast into the next row (rows 1 and 3) and lane 31 into rows 2 and 3:
+// lane 63 ends with the value of the whole wave, which readlane makes
+// uniform.  min / max are idempotent, so the overlapping windows are exact.
 __device__ inline int wave_min_i32(int v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
-    return v;
+    constexpr int ID = 0x7fffffff;
+    v = min(v, __builtin_amdgcn_update_dpp(ID, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+    v = min(v, __builtin_amdgcn_update_dpp(ID, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+    v = min(v, __builtin_amdgcn_update_dpp(ID, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+    v = min(v, __builtin_amdgcn_update_dpp(ID, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+    v = min(v, __builtin_amdgcn_update_dpp(ID, v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    v = min(v, __builtin_amdgcn_update_dpp(ID, v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return __builtin_amdgcn_readlane(v, 63);
 }
 __device__ inline int wave_max_i32(int v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
-    return v;
+    constexpr int ID = -0x7fffffff - 1;
+    v = max(v, __builtin_amdgcn_update_dpp(ID, v, 0x111, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(ID, v, 0x112, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(ID, v, 0x114, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(ID, v, 0x118, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(ID, v, 0x142, 0xa, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(ID, v, 0x143, 0xc, 0xf, false));
+    return __builtin_amdgcn_readlane(v, 63);
 }
 
 // Can the support {q >= kappa} of a record meet the rectangle [xa, xb] x
@@ -433,7 +457,8 @@ __device__ inline void pair_moments_rows(const PairRec &rec, const float *p0, co
             int rs, re;
             column_span(rec, rowlin, colt, x0, dx, inv_dx, inv2a, H, rs, re);
             const bool any = cc.act && rs <= re;
-            const int hs = wave_min_i32(any ? rs : H), he = wave_max_i32(any ? re : -1);
+            // (clamped to the window: every lane's span lies inside it)
+            const int hs = max(wave_min_i32(any ? rs : H), 0), he = min(wave_max_i32(any ? re : -1), H - 1);
             if (he < hs || !cc.act) continue;
             float A[5] = {0, 0, 0, 0, 0}, B[5] = {0, 0, 0, 0, 0};
             const int r0 = hs + cc.ro;
