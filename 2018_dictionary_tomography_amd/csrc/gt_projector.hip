@@ -25,6 +25,7 @@ constexpr int TILE_ROWS = 8, TILE_COLS = 64;
 constexpr int CHUNK = 256;
 constexpr int MOM_STRIDE = 32;  // floats per (atom, angle) moment record
 constexpr int MOM_TROWS = 512;  // rows of a moments wave's row table
+constexpr int MOM_WAVES = 1;    // waves (pairs) per moments block
 static_assert(MOM_TROWS * 16 <= 64 * (MOM_STRIDE + 1) * 4, "row table must fit the reduction slab");
 constexpr int MODE_LINEAR = GT_MODE_LINEAR, MODE_L2 = GT_MODE_L2, MODE_RESID = GT_MODE_RESID;
 
@@ -336,17 +337,18 @@ __global__ __launch_bounds__(256) void forward2_kernel(const PairRec *__restrict
 // Layout of one moment record (32 floats): [0, NM) sum w1 mono, [15, 15+NM)
 // sum E^2 mono, [30] sum_fp (alpha E - C)^2, [31] sum_fp C^2 (L2 mode).
 template <int K, int MODE>
-__global__ __launch_bounds__(256) void moments_kernel(const PairRec *__restrict__ recs, int N, int nA, int n0,
+__global__ __launch_bounds__(64 * MOM_WAVES) void moments_kernel(const PairRec *__restrict__ recs, int N, int nA,
+                                                                 int n0,
                                                       int n1, const float *__restrict__ p0,
                                                       const float *__restrict__ p1, const float *__restrict__ C,
                                                       float *__restrict__ mom)
 {
     // the reduction slab of a wave doubles as its row table while it walks
     // its pair (MOM_TROWS float4 = 8 KiB of the 8.25 KiB slab)
-    __shared__ alignas(16) float red[4][64][MOM_STRIDE + 1];
+    __shared__ alignas(16) float red[MOM_WAVES][64][MOM_STRIDE + 1];
     // the wave index is uniform: scalar pair index, record and slab base
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const long pair = (long)blockIdx.x * 4 + wave;
+    const long pair = (long)blockIdx.x * MOM_WAVES + wave;
     const bool valid = pair < (long)N * nA;
     float acc[MOM_STRIDE];
 #pragma unroll
@@ -360,9 +362,11 @@ __global__ __launch_bounds__(256) void moments_kernel(const PairRec *__restrict_
         else
             pair_moments<K, MODE>(rec, p0, p1, C + (size_t)a * n0 * n1, n1, lane, acc);
     }
+    // each wave reduces its own slab: no block barrier, so a wave whose pair
+    // is done does not wait for the others of its block
 #pragma unroll
     for (int v = 0; v < MOM_STRIDE; v++) red[wave][lane][v] = acc[v];
-    __syncthreads();
+    wave_sync();
     if (valid) {
         const int col = lane & 31, half = lane >> 5;
         float s = 0.0f;
@@ -592,7 +596,8 @@ static void launch_deriv_kernels(const gt_atoms *A, const gt_geom *G, const Pair
 {
     constexpr int K = D - 1;
     const long pairs = (long)A->n * G->nA;
-    launch_timed(GT_K_MOMENTS, moments_kernel<K, MODE>, dim3((unsigned)((pairs + 3) / 4)), dim3(256), 0, st, recs,
+    launch_timed(GT_K_MOMENTS, moments_kernel<K, MODE>, dim3((unsigned)((pairs + MOM_WAVES - 1) / MOM_WAVES)),
+                 dim3(64 * MOM_WAVES), 0, st, recs,
                  A->n, G->nA, G->n0, G->n1, G->p0, G->p1, C, mom);
     launch_timed(GT_K_FINISH, finish_kernel<D, MODE>, dim3(A->n), dim3(64), 0, st, *A, *G, recs, mom, sumC2, want_hess,
                  f, g, H);
