@@ -176,6 +176,12 @@ def serving_leg(torch, args, dev):
     allX = _cost_features([sp.nA for sp in specs], [sp.S for sp in specs])
     per_fit = np.array([updates[sp.rid] for sp in specs]) * (allX @ beta)
     plain_fps = len(specs) / float(per_fit.sum())
+    # the plain server's latency on the same arrival stream: one request at a
+    # time, first come first served, with the modelled service times
+    free, lat = 0.0, []
+    for sp, t in sorted(zip(specs, per_fit), key=lambda z: z[0].arrival):
+        free = max(free, sp.arrival) + float(t)
+        lat.append(free - sp.arrival)
     for r, x in zip(rows, X):
         r["model_seconds_full"] = float(r["updates_total"] * (x @ beta))
     return {
@@ -185,6 +191,8 @@ def serving_leg(torch, args, dev):
         "plain": {
             "fits_per_s": plain_fps,
             "p50_service_s": float(np.percentile(per_fit, 50)), "p99_service_s": float(np.percentile(per_fit, 99)),
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "latency_note": "first come first served on the same arrivals with the modelled service times",
             "sample": rows,
             "cost_model": {"form": "us/update = b0 + b1 nA + b2 S^2 + b3 nA S^2", "coef_s": beta.tolist(),
                            "rms_rel_residual": float(np.sqrt(np.mean(resid ** 2))),
@@ -192,6 +200,7 @@ def serving_leg(torch, args, dev):
             "note": "same device fitter, one request at a time with the whole device at its fastest group size; "
                     "service times = served update counts x modelled time per update"},
         "speedup_fits_per_s": served["fits_per_s"] / plain_fps,
+        "p99_latency_ratio_plain_over_served": float(np.percentile(lat, 99)) / served["p99_latency_s"],
         "alone_equals_mixed": bool(same),
         "alone_equals_mixed_checked": int(sum(r["full_fit"] for r in rows)),
     }
