@@ -7,9 +7,9 @@
 //            4 waves owns an 8x16 rectangle (2 pixels per lane) and walks
 //            the records whose support meets it; FP64 pixel sums of FP32
 //            values taken four records at a time
-//   moments  one wave per (atom, angle): FP32 separable pixel moments over
-//            the rows of each column chunk that the support can reach,
-//            LDS-transposed wave reduction, no atomics
+//   moments  one wave, and one block, per (atom, angle): FP32 separable
+//            pixel moments over the rows of each column chunk that the
+//            support can reach, LDS-transposed wave reduction, no atomics
 //   finish   one wave per atom, one lane per angle: FP64 moment -> (f, g, H)
 //            contraction (gt_pair.h), FP64 wave reduction over angles
 //
@@ -338,10 +338,9 @@ __global__ __launch_bounds__(256) void forward2_kernel(const PairRec *__restrict
 // sum E^2 mono, [30] sum_fp (alpha E - C)^2, [31] sum_fp C^2 (L2 mode).
 template <int K, int MODE>
 __global__ __launch_bounds__(64 * MOM_WAVES) void moments_kernel(const PairRec *__restrict__ recs, int N, int nA,
-                                                                 int n0,
-                                                      int n1, const float *__restrict__ p0,
-                                                      const float *__restrict__ p1, const float *__restrict__ C,
-                                                      float *__restrict__ mom)
+                                                                 int n0, int n1, const float *__restrict__ p0,
+                                                                 const float *__restrict__ p1,
+                                                                 const float *__restrict__ C, float *__restrict__ mom)
 {
     // the reduction slab of a wave doubles as its row table while it walks
     // its pair (MOM_TROWS float4 = 8 KiB of the 8.25 KiB slab)
