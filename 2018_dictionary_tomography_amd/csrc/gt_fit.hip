@@ -100,7 +100,21 @@ template <int D> struct FitLDS {
     FitCtl c;
     unsigned gen;
     int accept, j, stop, abort_, skip, moved;
+    int next[3];  // next own-angle number of the moments, energy and commit passes
 };
+
+// The wave's next own angle of a pixel pass: waves take angles as they
+// finish the previous one (the angles' boxes differ in size), so a pass ends
+// when the work runs out rather than when the wave with the largest boxes
+// does.  Every angle's result goes to a slot of its own and the sums over
+// angles run in angle order, so the results do not depend on which wave
+// took which angle.
+__device__ inline int next_angle(int *ctr, int lane)
+{
+    int k = 0;
+    if (lane == 0) k = atomicAdd(ctr, 1);
+    return __builtin_amdgcn_readfirstlane(k);
+}
 
 // theta (FP64 values of the FP32 atom) -> PairGeom atom part
 template <int D> __device__ inline void atom_from_theta(const double *th, int iso, PairGeom<D> &g)
@@ -635,6 +649,7 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                     c.done = 2;
                 }
                 L.j = j;
+                L.next[0] = L.next[1] = L.next[2] = 0;
                 if (!L.stop) {
                     L.th_old[0] = S.I[j];
                     for (int i = 0; i < D; i++) L.th_old[1 + i] = S.x[(size_t)j * D + i];
@@ -658,7 +673,7 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         //      gradient/Hessian record, both by the wave that owns the angle --
         {
             WaveC<D> &W = L.wc[wave];
-            for (int k = wave; k < nOwn; k += FIT_WAVES) {
+            for (int k = next_angle(&L.next[0], lane); k < nOwn; k = next_angle(&L.next[0], lane)) {
                 const int a = member + G * k;
                 const PairRec &rec = L.rec_old[k];
                 float acc[MOMW];
@@ -907,7 +922,7 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         __syncthreads();
         GT_FIT_TICK(4);
         // ---- phase F: per-angle FP64 energy changes -----------------------
-        for (int k = wave; k < nOwn; k += FIT_WAVES) {
+        for (int k = next_angle(&L.next[1], lane); k < nOwn; k = next_angle(&L.next[1], lane)) {
             const int a = member + G * k;
             gmem<float> *Ra = (gmem<float> *)S.resid + (size_t)a * slab;
             float2 *dt = reinterpret_cast<float2 *>(L.tab[wave]);
@@ -987,7 +1002,7 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         // with a staging slab the energy pass already committed: undo on
         // reject; without one, commit on accept
         if (stage ? !L.accept : L.accept) {
-            for (int k = wave; k < nOwn; k += FIT_WAVES) {
+            for (int k = next_angle(&L.next[2], lane); k < nOwn; k = next_angle(&L.next[2], lane)) {
                 const int a = member + G * k;
                 gmem<float> *Ra = (gmem<float> *)S.resid + (size_t)a * slab;
                 if (stage)
