@@ -670,7 +670,7 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         __syncthreads();
         GT_FIT_TICK(0);
         // ---- phase B+C: per-angle pixel moments, then that angle's FP64
-        //      gradient/Hessian record, both by the wave that owns the angle --
+        //      gradient/Hessian record, both by the wave that took the angle --
         {
             WaveC<D> &W = L.wc[wave];
             for (int k = next_angle(&L.next[0], lane); k < nOwn; k = next_angle(&L.next[0], lane)) {
