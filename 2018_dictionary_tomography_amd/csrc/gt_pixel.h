@@ -3,7 +3,8 @@
 // A wave walks the pixels of a PairRec box in row-major order, 64 pixels per
 // step, keeping (row, col) incrementally so no integer division is done per
 // pixel.  Pixels with m > 1 contribute nothing (NewtonBases.py:49-56, 375-380);
-// the moment walks take the tail just outside as it is (moment_pixel).
+// the moment walks (moment_pixel) and the 2D-detector forward take the tail
+// just outside as it is (below e^-30 of the profile's peak).
 #pragma once
 
 #include "gt_common.h"

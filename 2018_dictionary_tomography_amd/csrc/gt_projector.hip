@@ -195,12 +195,12 @@ __device__ inline int stage_rects(const PairRec *__restrict__ ra, int base, int 
 
 // One record's values at a lane's two pixels.  The record arrives as its
 // first 48 bytes (alpha, c_hi, c_lo, q): f0 = (alpha, c_hi0, c_hi1, c_lo0),
-// f1 = (c_lo1, q0, q1, q2), f2 = (q3, q4, q5, pad).  Branch-free cutoff:
-// every record in a wave's list reaches its rectangle.
+// f1 = (c_lo1, q0, q1, q2), f2 = (q3, q4, q5, pad).  No cut-off test: every
+// record in a wave's list reaches its rectangle, and the values at pixels of
+// the rectangle outside the support are below e^-30 of the record's peak.
 __device__ inline void f3_values(const float4 &f0, const float4 &f1, const float4 &f2, float pc, float prA,
                                  float prB, float &vA, float &vB)
 {
-    const float kap = (float)KAPPA;
     const float dp1 = (pc - f0.z) - f1.x;
     const float rowlin = fmaf(f1.z, dp1, f2.x);
     const float col = fmaf(fmaf(f1.w, dp1, f2.y), dp1, f2.z);
@@ -209,8 +209,8 @@ __device__ inline void f3_values(const float4 &f0, const float4 &f1, const float
     const float qA = fmaf(fmaf(f1.y, dA, rowlin), dA, col);
     const float qB = fmaf(fmaf(f1.y, dB, rowlin), dB, col);
     const float eA = __builtin_amdgcn_exp2f(qA), eB = __builtin_amdgcn_exp2f(qB);
-    vA = qA >= kap ? f0.x * eA : 0.0f;
-    vB = qB >= kap ? f0.x * eB : 0.0f;
+    vA = f0.x * eA;
+    vB = f0.x * eB;
 }
 
 __global__ __launch_bounds__(F3_WAVES * 64) void forward3_kernel(const PairRec *__restrict__ recs, int N, int nA,
