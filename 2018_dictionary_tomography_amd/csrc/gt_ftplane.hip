@@ -53,27 +53,6 @@ template <typename R> struct alignas(16) KpRec {
     int lo[2], hi[2];
 };
 
-template <typename T> __host__ __device__ inline int lower_idx_t(const T *p, int n, double v)
-{
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if ((double)p[mid] < v) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-template <typename T> __host__ __device__ inline int upper_idx_t(const T *p, int n, double v)
-{
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if ((double)p[mid] <= v) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo - 1;
-}
-
 // Sample frame of angle a: k = G p (columns G[.][0], G[.][1]); D = 3 or 2.
 template <typename R> __device__ inline void kp_frame(const KpDev<R> &P, int a, double G[3][2])
 {
@@ -183,8 +162,8 @@ __device__ inline void kp_pair(int D, int iso, double I, const AR *x, const AR *
             rec.hi[c] = m[c] - 1;
         } else {
             const double pad = 1e-5 * hw + 1e-7;
-            rec.lo[c] = lower_idx_t(kv[c], m[c], -hw - pad);
-            rec.hi[c] = upper_idx_t(kv[c], m[c], hw + pad);
+            rec.lo[c] = lower_idx(kv[c], m[c], -hw - pad);
+            rec.hi[c] = upper_idx(kv[c], m[c], hw + pad);
         }
     }
 }

@@ -166,10 +166,24 @@ struct alignas(16) PairRec {
 };
 static_assert(sizeof(PairRec) == 64, "PairRec must stay 64 bytes");
 
-// First index i with p[i] >= v (p ascending, length n).
-__host__ __device__ inline int lower_idx(const float *p, int n, double v)
+// Index searches on an ascending coordinate vector p of length n.  The
+// first guess is read off the end points (exact on a uniform grid) and kept
+// when its neighbours confirm it, so a uniform detector costs two dependent
+// loads instead of a bisection's log2(n); a guess that fails the check falls
+// back to bisection, so the result is the same for every ascending vector.
+
+// First index i with p[i] >= v, n if none (0 for v NaN).
+template <typename T> __host__ __device__ inline int lower_idx(const T *p, int n, double v)
 {
-    int lo = 0, hi = n;
+    if (n <= 0) return 0;
+    const double a = p[0], b = p[n - 1];
+    if (!(v > a)) return 0;
+    if (v > b) return n;
+    // here a < v <= b, so n >= 2 and b > a
+    int g = (int)ceil((v - a) / (b - a) * (double)(n - 1));
+    g = g < 1 ? 1 : (g > n - 1 ? n - 1 : g);
+    if ((double)p[g] >= v && (double)p[g - 1] < v) return g;
+    int lo = 1, hi = n - 1;
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
         if ((double)p[mid] < v) lo = mid + 1;
@@ -177,10 +191,18 @@ __host__ __device__ inline int lower_idx(const float *p, int n, double v)
     }
     return lo;
 }
-// Last index i with p[i] <= v (p ascending), -1 if none.
-__host__ __device__ inline int upper_idx(const float *p, int n, double v)
+// Last index i with p[i] <= v, -1 if none (-1 for v NaN).
+template <typename T> __host__ __device__ inline int upper_idx(const T *p, int n, double v)
 {
-    int lo = 0, hi = n;
+    if (n <= 0) return -1;
+    const double a = p[0], b = p[n - 1];
+    if (!(v >= a)) return -1;
+    if (v >= b) return n - 1;
+    // here a <= v < b, so n >= 2 and b > a
+    int g = (int)floor((v - a) / (b - a) * (double)(n - 1));
+    g = g < 0 ? 0 : (g > n - 2 ? n - 2 : g);
+    if ((double)p[g] <= v && (double)p[g + 1] > v) return g;
+    int lo = 0, hi = n - 1;
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
         if ((double)p[mid] <= v) lo = mid + 1;

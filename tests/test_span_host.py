@@ -71,3 +71,22 @@ def test_fourier_patch_test_keeps_every_support_sample(ft_patch_check, iso):
     patches, kept, with_support = map(int, res.stdout.split())
     assert 0 < with_support <= kept < patches
     assert kept <= 1.05 * with_support + 16
+
+
+def test_index_searches_match_bisection(tmp_path):
+    """The footprint boxes' index searches (gt_pair.h, lower_idx / upper_idx:
+    a guess from the end points, checked against its neighbours, bisection
+    when the check fails) return what std::lower_bound / std::upper_bound do
+    on uniform, non-uniform, clustered and repeated ascending vectors of 1 to
+    1000 entries, for queries on, between and beyond the points, +-inf and
+    NaN."""
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not available")
+    exe = str(tmp_path / "idx_check")
+    src = os.path.join(ROOT, "tests", "native", "idx_check.cpp")
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "2018_dictionary_tomography_amd", "csrc")]
+    res = subprocess.run([HIPCC, "-O2", "-std=c++17", *inc, src, "-o", exe], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    res = subprocess.run([exe], capture_output=True, text=True)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert int(res.stdout.split()[1]) > 50000
