@@ -314,15 +314,29 @@ __host__ __device__ inline float pair_q(const float *q, float dp0, float dp1)
 //   The pixel weight is w1 = (lambda alpha E - sigma C) E; the finishing
 //   only needs to know that v = sum res dR/dgamma uses w1 with "res" the
 //   residual (L2, residual modes) or C itself (linear mode).
-// Accumulates into grad[P] and hess[P][P] (full, symmetric).
+// Accumulates into grad[P] and, through the sink hs, the Hessian: hs.add(i,
+// j, v) with i <= j adds v to entry (i, j) of the symmetric P x P matrix.
 // Qgn (optional, replaces S2): the products sum_k M_k[m] M_k[m'] over the
 // low monomials m, m' of degree <= 2 (NL = 6 for 2D detectors, 3 for 1D),
 // for values that are pixel integrals M_k[m] = int_k mono E rather than point
 // samples (the Gauss-Newton term is then not a single moment).
-template <int D>
-__host__ __device__ inline void pair_finish(const PairGeom<D> &g, const double *pstar, const double *S1,
-                                            const double *S2, int gn, int want_hess, double *grad, double *hess,
-                                            const double *Qgn = nullptr)
+template <int P> struct HessFull {  // full P x P array, both triangles
+    double *h;
+    __host__ __device__ void add(int i, int j, double v) const
+    {
+        h[i * P + j] += v;
+        if (i != j) h[j * P + i] += v;
+    }
+};
+template <int P> struct HessUpper {  // upper triangle by rows: (0,0..P-1), (1,1..P-1), ...
+    double *h;
+    __host__ __device__ void add(int i, int j, double v) const { h[i * P - i * (i - 1) / 2 + (j - i)] += v; }
+};
+
+template <int D, class Sink>
+__host__ __device__ inline void pair_finish_to(const PairGeom<D> &g, const double *pstar, const double *S1,
+                                               const double *S2, int gn, int want_hess, double *grad, const Sink &hs,
+                                               const double *Qgn = nullptr)
 {
     using Dm = Dims<D>;
     constexpr int K = Dm::K, NB = Dm::NB, NG = Dm::NG, P = Dm::P, NR = Dm::NR;
@@ -417,8 +431,7 @@ __host__ __device__ inline void pair_finish(const PairGeom<D> &g, const double *
         for (int j = i; j < P; j++) {
             double acc = 0.0;
             for (int l = 0; l < NG; l++) acc += J[l][i] * KJ[l][j];
-            hess[i * P + j] += acc;
-            if (j != i) hess[j * P + i] += acc;
+            hs.add(i, j, acc);
         }
 
     // hess += Hessian of Phi(theta) = v_alpha alpha + sum_ab V_ab B_ab  (v fixed)
@@ -450,8 +463,7 @@ __host__ __device__ inline void pair_finish(const PairGeom<D> &g, const double *
     for (int k = 0; k < NR; k++) {
         int i = slot_row(D, k), j = slot_col(D, k);
         double val = -va * rs3 * a.t[i] * g.T[j];
-        hess[0 * P + 1 + D + k] += val;
-        hess[(1 + D + k) * P + 0] += val;
+        hs.add(0, 1 + D + k, val);
     }
     // x - x : 2 V_KK Q_cd, Q_cd = (U e_c) . M (U e_d)
     double z[3];  // U^T t
@@ -464,10 +476,10 @@ __host__ __device__ inline void pair_finish(const PairGeom<D> &g, const double *
     for (int c = 0; c < D; c++)
         for (int i = 0; i < D; i++) MUc[c][i] = (i <= c ? g.U[i][c] : 0.0) - z[c] / s * a.t[i];
     for (int c = 0; c < D; c++)
-        for (int d = 0; d < D; d++) {
+        for (int d = c; d < D; d++) {  // Q is symmetric: one triangle
             double Qcd = 0.0;
             for (int i = 0; i <= c; i++) Qcd += g.U[i][c] * MUc[d][i];
-            hess[(1 + c) * P + 1 + d] += 2.0 * V[K][K] * Qcd;
+            hs.add(1 + c, 1 + d, 2.0 * V[K][K] * Qcd);
         }
     // x - r
     double qt[3], uq[3];
@@ -485,8 +497,7 @@ __host__ __device__ inline void pair_finish(const PairGeom<D> &g, const double *
             int i = slot_row(D, k), j = slot_col(D, k);
             double ec_j = (j == c ? 1.0 : 0.0) - z[c] / s * g.T[j];
             double val = -2.0 * (MUc[c][i] * qt[j] + uq[i] * ec_j);
-            hess[(1 + c) * P + 1 + D + k] += val;
-            hess[(1 + D + k) * P + 1 + c] += val;
+            hs.add(1 + c, 1 + D + k, val);
         }
     // r - r
     for (int k = 0; k < NR; k++) {
@@ -499,10 +510,18 @@ __host__ __device__ inline void pair_finish(const PairGeom<D> &g, const double *
                                  g.T[j1] * a.t[i2] * Uh[i1][j2] + g.T[j1] * g.T[j2] * Uu[i1][i2]) / s);
             val += va * g.I *
                    (3.0 * rs5 * a.t[i1] * g.T[j1] * a.t[i2] * g.T[j2] - rs3 * dik * g.T[j1] * g.T[j2]);
-            hess[(1 + D + k) * P + 1 + D + l] += val;
-            if (l != k) hess[(1 + D + l) * P + 1 + D + k] += val;
+            hs.add(1 + D + k, 1 + D + l, val);
         }
     }
+}
+
+// pair_finish_to with the Hessian in a full P x P array (both triangles).
+template <int D>
+__host__ __device__ inline void pair_finish(const PairGeom<D> &g, const double *pstar, const double *S1,
+                                            const double *S2, int gn, int want_hess, double *grad, double *hess,
+                                            const double *Qgn = nullptr)
+{
+    pair_finish_to<D>(g, pstar, S1, S2, gn, want_hess, grad, HessFull<Dims<D>::P>{hess}, Qgn);
 }
 
 // The same contraction split in two, for callers that spread one pair's

@@ -413,12 +413,16 @@ __global__ __launch_bounds__(64) void finish_kernel(gt_atoms A, gt_geom G, const
 {
     using Dm = Dims<D>;
     constexpr int P = Dm::P, NM = Dm::NM, NH = P * (P + 1) / 2, NV = 1 + P + NH;
+    // row of lane l: [F, grad (P), upper Hessian by rows (NH)]; the lane
+    // accumulates its pairs' Hessian entries there directly (a P x P array
+    // per lane in registers would spill)
     __shared__ double red[64][NV + 1];
     const int j = blockIdx.x, lane = threadIdx.x;
     const int N = A.n;
-    double grad[P], hess[P * P], F = 0.0;
+    double grad[P], F = 0.0;
     for (int i = 0; i < P; i++) grad[i] = 0.0;
-    for (int i = 0; i < P * P; i++) hess[i] = 0.0;
+    double *row = red[lane];
+    for (int k = 0; k < NV; k++) row[k] = 0.0;
     PairGeom<D> g;
     load_atom<D>(A, j, g);
     for (int a = lane; a < G.nA; a += 64) {
@@ -442,15 +446,10 @@ __global__ __launch_bounds__(64) void finish_kernel(gt_atoms A, gt_geom G, const
         }
         const double pstar[2] = {(double)rec.c_hi[0] + (double)rec.c_lo[0],
                                  (double)rec.c_hi[1] + (double)rec.c_lo[1]};
-        pair_finish<D>(g, pstar, S1, S2, MODE != MODE_LINEAR, want_hess, grad, hess);
+        pair_finish_to<D>(g, pstar, S1, S2, MODE != MODE_LINEAR, want_hess, grad, HessUpper<P>{row + 1 + P});
     }
-    red[lane][0] = F;
-    for (int i = 0; i < P; i++) red[lane][1 + i] = grad[i];
-    {
-        int k = 1 + P;
-        for (int i = 0; i < P; i++)
-            for (int l = i; l < P; l++) red[lane][k++] = hess[i * P + l];
-    }
+    row[0] = F;
+    for (int i = 0; i < P; i++) row[1 + i] = grad[i];
     __syncthreads();
     for (int col = lane; col < NV; col += 64) {
         double s = 0.0;

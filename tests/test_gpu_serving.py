@@ -6,6 +6,8 @@ import numpy as np
 import pytest
 import torch
 
+from conftest import f32
+
 pytestmark = pytest.mark.gpu
 
 
@@ -80,3 +82,36 @@ def test_server_memory_admission(gt, cuda):
     srv2 = Sv.FitServer(cuda, max_slots=4, hbm_budget=big // 1000, work_per_launch=4.0e4, use_graph=False)
     st2 = srv2.run(specs[:2])
     assert st2.refused == 2 and st2.completed == 0
+
+
+def test_served_smallest_workload_shapes_match_oracle(gt, oracle, cuda):
+    """Requests at the smallest shapes of the served workload (50-70 atoms
+    drawn as the 3D workload, 30-40 angles over +-60 deg, 64^2 and 96^2
+    detectors, data with 1% noise, centres perturbed) served together for 2
+    Newton sweeps: each result equals the FP64 oracle's Newton fit
+    (NewtonGaussian.py:18-109) on the same inputs -- final energy within 1e-5
+    of the start energy, centres within 1e-4, the same number of updates."""
+    Sv = importlib.import_module("2018_dictionary_tomography_amd.serving")
+    specs = Sv.c5_stream(3, rate=500.0, seed=31, n_range=(50, 70), angle_range=(30, 40), sides=(64, 96),
+                         max_iter=2)
+    pay = Sv.prepare_payloads(specs, cuda)
+    srv = Sv.FitServer(cuda, keep_results=True)
+    st = srv.run(specs, payloads=pay)
+    assert st.completed == len(specs) and st.failed == 0 and st.singular == 0
+    worst = 0.0
+    for spec in specs:
+        start, space, data = pay[spec.rid]
+        Im, xm, rm, Em, donem, updm = st.results[spec.rid]
+        assert donem == 1 and updm == 2 * spec.N
+        og = oracle.Geometry(f32(space.orientations), tuple(f32(o) for o in space.ortho),
+                             [f32(p) for p in space.detector])
+        Io, xo, ro, Eo = oracle.newton_linesearch(f32(start.I.cpu().numpy()), f32(start.x.cpu().numpy()),
+                                                  f32(start.r.cpu().numpy()), og, data.double().cpu().numpy(),
+                                                  [spec.max_iter, 1], tol=1e-4)
+        assert len(Eo) == updm + 1
+        err = abs(Em - Eo[-1]) / Eo[0]
+        worst = max(worst, err)
+        assert err < 1e-5, (spec, Em, Eo[-1], Eo[0])
+        assert np.abs(xm.cpu().numpy() - xo).max() < 1e-4
+        assert np.abs(Im.cpu().numpy() - Io).max() < 1e-4 * max(1.0, np.abs(Io).max())
+    print(f"served vs oracle, final energy error / E0: {worst:.2e}")
