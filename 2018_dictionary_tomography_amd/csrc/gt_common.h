@@ -139,6 +139,8 @@ template <int D> __device__ __host__ inline void load_atom(const gt_atoms &A, in
         double rho = A.r[j];
         for (int i = 0; i < D; i++) g.U[i][i] = rho;
     } else {
+        // unrolled: constant indices keep U in registers
+#pragma unroll
         for (int k = 0; k < NR; k++) g.U[slot_row(D, k)][slot_col(D, k)] = A.r[(size_t)j * NR + k];
     }
     for (int i = 0; i < 3; i++) g.x[i] = (i < D) ? (double)A.x[(size_t)j * D + i] : 0.0;
@@ -154,9 +156,14 @@ template <int D> __device__ __host__ inline void load_angle(const gt_geom &G, in
         g.G[1][i] = 0.0;
     }
     const int L = G.lenT;
-    for (int i = 0; i < L; i++) {
-        g.T[i] = G.T[(size_t)a * L + i];
-        g.G[0][i] = G.w0[(size_t)a * L + i];
+    // constant indices (a loop to the run-time L would index T and G[0]
+    // dynamically and put them in scratch memory)
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        if (i < L) {
+            g.T[i] = G.T[(size_t)a * L + i];
+            g.G[0][i] = G.w0[(size_t)a * L + i];
+        }
     }
     if (D == 3) {
         if (L == 3) {

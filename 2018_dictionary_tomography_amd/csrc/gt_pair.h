@@ -347,6 +347,7 @@ __host__ __device__ inline void pair_finish_to(const PairGeom<D> &g, const doubl
     // psi_p = (2 - delta_ab) d_a d_b: factor and exponents
     double fac[NB];
     int pe0[NB], pe1[NB];
+#pragma unroll
     for (int p = 0; p < NB; p++) {
         int x0, x1, ea0, ea1, eb0, eb1;
         bpair<K>(p, x0, x1);
@@ -360,8 +361,10 @@ __host__ __device__ inline void pair_finish_to(const PairGeom<D> &g, const doubl
     double v[NG], Kg[NG][NG];
     constexpr int NL = (K == 2) ? 6 : 3;
     v[0] = S1[0];
+#pragma unroll
     for (int p = 0; p < NB; p++) v[1 + p] = -30.0 * al * fac[p] * S1[mono_index<K>(pe0[p], pe1[p])];
     Kg[0][0] = gn ? (Qgn ? Qgn[0] : S2[0]) : 0.0;
+#pragma unroll
     for (int p = 0; p < NB; p++) {
         int m = mono_index<K>(pe0[p], pe1[p]);
         const double gnv = gn ? (Qgn ? Qgn[m] : S2[m]) : 0.0;
@@ -369,7 +372,9 @@ __host__ __device__ inline void pair_finish_to(const PairGeom<D> &g, const doubl
         Kg[0][1 + p] = val;
         Kg[1 + p][0] = val;
     }
+#pragma unroll
     for (int p = 0; p < NB; p++)
+#pragma unroll
         for (int q = p; q < NB; q++) {
             int m = mono_index<K>(pe0[p] + pe0[q], pe1[p] + pe1[q]);
             const double gnv =

@@ -457,21 +457,22 @@ __global__ __launch_bounds__(64) void finish_kernel(gt_atoms A, gt_geom G, const
         red[0][col] = s;  // rows 1..63 of this column are no longer read
     }
     __syncthreads();
-    if (lane != 0) return;
     const double *v = red[0];
-    if (fo) fo[j] = v[0];
     if (!A.iso) {
-        for (int i = 0; i < P; i++) go[(size_t)j * P + i] = v[1 + i];
-        if (Ho && want_hess) {
-            int k = 1 + P;
-            for (int i = 0; i < P; i++)
-                for (int l = i; l < P; l++) {
-                    Ho[((size_t)j * P + i) * P + l] = v[k];
-                    Ho[((size_t)j * P + l) * P + i] = v[k];
-                    k++;
-                }
-        }
-    } else {
+        // the lanes write the gradient and the full Hessian, one entry each
+        if (lane == 0 && fo) fo[j] = v[0];
+        if (lane < P) go[(size_t)j * P + lane] = v[1 + lane];
+        if (Ho && want_hess)
+            for (int e = lane; e < P * P; e += 64) {
+                const int i = e / P, l = e - (e / P) * P;
+                const int lo = i < l ? i : l, hi = i < l ? l : i;
+                Ho[(size_t)j * P * P + e] = v[1 + P + lo * P - lo * (lo - 1) / 2 + (hi - lo)];
+            }
+        return;
+    }
+    if (lane != 0) return;
+    if (fo) fo[j] = v[0];
+    {
         // U = rho * Id: d r_k / d rho = 1 for the D diagonal slots.
         constexpr int PI = 2 + D;
         double Hf[P][P];

@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--cpu-atoms", type=int, default=1000)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--atoms", type=int, default=1000)
+    ap.add_argument("--kernel-sample-every", type=int, default=8,
+                    help="bracket every k-th moments launch of the timed steps with HIP events")
     ap.add_argument("--fit-iters", type=int, default=20, help="Newton outer iterations of the fit leg (0 skips it)")
     ap.add_argument("--serve-requests", type=int, default=2000, help="requests of the serving leg (0 skips it)")
     ap.add_argument("--serve-nmax", type=int, default=5000, help="upper end of the log-uniform atom count")
@@ -686,17 +688,23 @@ def main():
     for k in range(7):
         L.profile_read(k)
     # The timed steps bracket only the moments kernel (the roofline's) with HIP
-    # events on its stream: every event pair adds a few microseconds between
-    # launches.  The other kernels are timed in a second, untimed pass.
+    # events on its stream, and only every k-th launch of it: a bracketed
+    # launch leaves a gap of about 9 us before the next kernel, so bracketing
+    # every step would add that to every step.  The other kernels are timed in
+    # a second, untimed pass.
+    every = max(1, args.kernel_sample_every)
     L.profile_select([L.GT_K_MOMENTS])
-    L.profile_enable(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
+    for i in range(args.steps):
+        if i % every == 0:
+            L.profile_enable(True)
+            step()
+            L.profile_enable(False)
+        else:
+            step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    L.profile_enable(False)
     kt_mom = L.profile_read(L.GT_K_MOMENTS)
     L.profile_select(None)
     L.profile_enable(True)
@@ -745,8 +753,9 @@ def main():
         "forward_kernel": {"avg_ms": round(fwd_ms, 5), "achieved_tflops": round(P_in * FLOPS_FWD / (fwd_ms * 1e-3) / 1e12, 3),
                            "trans_frac": round(P_in / (fwd_ms * 1e-3) / PEAK_TRANS_PER_S, 4)},
         "kernel_ms": {k: round(v[0] / max(v[1], 1), 5) for k, v in kt.items()},
-        "kernel_ms_note": "moments: HIP events around it inside the timed steps (only that kernel bracketed); "
-                          "setup, forward, finish: a second, untimed pass with every kernel bracketed",
+        "kernel_ms_note": "moments: HIP events around every %d-th launch of it inside the timed steps (only that "
+                          "kernel bracketed, %d launches); setup, forward, finish: a second, untimed pass with every "
+                          "kernel bracketed" % (every, kt_mom[1]),
     }
     fit = fit_leg(torch, gtp, space, data, cur, args.fit_iters) if args.fit_iters > 0 else None
     serving = serving_leg(torch, args, dev) if args.serve_requests > 0 else None
