@@ -38,8 +38,9 @@ PEAK_FP32_TFLOPS = 157.3      # MI355X published FP32 (vector, packed / matrix)
 PEAK_FP32_NONPACKED = 78.65   # 256 CU x 4 SIMD x 16 lanes x 2 flop x 2.4 GHz
 PEAK_TRANS_PER_S = 256 * 4 * 16 * 2.4e9 / 4  # quarter-rate transcendental
 PEAK_HBM_GBS = 8000.0
-# moments kernel, 1000 atoms x 61 angles x 128^2: 38.0 MiB read + 7.4 MiB written per launch
-MOMENTS_HBM_BYTES = int((38.0 + 7.4) * 2**20)
+# moments kernel, 1000 atoms x 61 angles x 128^2, per launch: FETCH_SIZE 19481.5 KiB x 1.9998 (the
+# counter's reading of a 256 MiB copy) read + WRITE_SIZE 7625.0 KiB written (38.0 + 7.4 MiB)
+MOMENTS_HBM_BYTES = int((19481.5 * 1.9998 + 7625.0) * 1024)
 
 
 def parse():
