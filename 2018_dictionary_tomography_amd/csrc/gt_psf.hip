@@ -714,12 +714,16 @@ __global__ __launch_bounds__(64) void psf_finish_kernel(gt_atoms A, gt_geom G, c
 {
     using Dm = Dims<D>;
     constexpr int K = Dm::K, P = Dm::P, NM = Dm::NM, NL = (K == 2) ? 6 : 3, NH = P * (P + 1) / 2, NV = 1 + P + NH;
+    // row of lane l: [F, grad (P), upper Hessian by rows (NH)]; the lane
+    // accumulates its pairs' Hessian entries there directly (a P x P array
+    // per lane in registers would spill)
     __shared__ double red[64][NV + 1];
     const int j = blockIdx.x, lane = threadIdx.x;
     const int N = A.n;
-    double grad[P], hess[P * P], F = 0.0;
+    double grad[P], F = 0.0;
     for (int i = 0; i < P; i++) grad[i] = 0.0;
-    for (int i = 0; i < P * P; i++) hess[i] = 0.0;
+    double *row = red[lane];
+    for (int k = 0; k < NV; k++) row[k] = 0.0;
     PairGeom<D> g;
     load_atom<D>(A, j, g);
     for (int a = lane; a < G.nA; a += 64) {
@@ -741,16 +745,11 @@ __global__ __launch_bounds__(64) void psf_finish_kernel(gt_atoms A, gt_geom G, c
                 }
         }
         const double pstar[2] = {(double)R.r.c_hi[0] + (double)R.r.c_lo[0], (double)R.r.c_hi[1] + (double)R.r.c_lo[1]};
-        pair_finish<D>(g, pstar, S1, nullptr, MODE == GT_MODE_L2, want_hess, grad, hess,
-                       MODE == GT_MODE_L2 ? Q : nullptr);
+        pair_finish_to<D>(g, pstar, S1, nullptr, MODE == GT_MODE_L2, want_hess, grad, HessUpper<P>{row + 1 + P},
+                          MODE == GT_MODE_L2 ? Q : nullptr);
     }
-    red[lane][0] = F;
-    for (int i = 0; i < P; i++) red[lane][1 + i] = grad[i];
-    {
-        int k = 1 + P;
-        for (int i = 0; i < P; i++)
-            for (int l = i; l < P; l++) red[lane][k++] = hess[i * P + l];
-    }
+    row[0] = F;
+    for (int i = 0; i < P; i++) row[1 + i] = grad[i];
     __syncthreads();
     for (int col = lane; col < NV; col += 64) {
         double s = 0.0;
@@ -758,8 +757,20 @@ __global__ __launch_bounds__(64) void psf_finish_kernel(gt_atoms A, gt_geom G, c
         red[0][col] = s;
     }
     __syncthreads();
-    if (lane != 0) return;
     const double *v = red[0];
+    if (!A.iso) {
+        // the lanes write the gradient and the full Hessian, one entry each
+        if (lane == 0 && fo) fo[j] = v[0];
+        if (lane < P) go[(size_t)j * P + lane] = v[1 + lane];
+        if (Ho && want_hess)
+            for (int e = lane; e < P * P; e += 64) {
+                const int i = e / P, l = e - (e / P) * P;
+                const int lo = i < l ? i : l, hi = i < l ? l : i;
+                Ho[(size_t)j * P * P + e] = v[1 + P + lo * P - lo * (lo - 1) / 2 + (hi - lo)];
+            }
+        return;
+    }
+    if (lane != 0) return;
     if (fo) fo[j] = v[0];
     double Hf[P][P];
     {
@@ -769,13 +780,6 @@ __global__ __launch_bounds__(64) void psf_finish_kernel(gt_atoms A, gt_geom G, c
                 Hf[i][l] = Hf[l][i] = v[k];
                 k++;
             }
-    }
-    if (!A.iso) {
-        for (int i = 0; i < P; i++) go[(size_t)j * P + i] = v[1 + i];
-        if (Ho && want_hess)
-            for (int i = 0; i < P; i++)
-                for (int l = 0; l < P; l++) Ho[((size_t)j * P + i) * P + l] = Hf[i][l];
-        return;
     }
     // U = rho Id: the diagonal slots move together
     constexpr int PI = 2 + D;
