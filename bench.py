@@ -655,6 +655,13 @@ def main():
 
     L = import_module("2018_dictionary_tomography_amd._lib")
     dev = torch.device("cuda", 0)
+    if args.calib_copy_mib > 0:
+        src = torch.ones(args.calib_copy_mib << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if args.serve_only:
         print(json.dumps({"serving": serving_leg(torch, args, dev)}))
         return 0
@@ -676,13 +683,6 @@ def main():
         # forward, residual and every atom's gradient / Hessian block in one call
         return gtp.ops.forward_derivs(cur, space, data, 2, out=R, want_f=False)
 
-    if args.calib_copy_mib > 0:
-        src = torch.ones(args.calib_copy_mib << 18, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
