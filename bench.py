@@ -69,6 +69,8 @@ def parse():
     ap.add_argument("--gd-cpu-iters", type=int, default=5,
                     help="outer iterations of the 2D block-GD leg timed on the CPU oracle")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg and print its JSON")
+    ap.add_argument("--steady-fits", type=int, default=0,
+                    help="run only the fit kernel's steady state with this many fits at once and print its JSON")
     ap.add_argument("--skip-checks", action="store_true",
                     help="skip the FP64 parity spot check and the in-support pair count (counter runs)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -118,6 +120,56 @@ def stratified_sample(specs, n_edges, s_bins, seed):
 def _cost_features(nA, S):
     nA, S = np.asarray(nA, float), np.asarray(S, float)
     return np.stack([np.ones_like(nA), nA, S * S, nA * S * S], -1)
+
+
+def steady_leg(torch, args, dev, launches=4, budget=100):
+    """Steady state of the served fitter: args.steady_fits requests of the
+    serving stream (a seed-5 pick of its first 2000), one slot and one
+    workgroup each, all advanced together by `launches` grouped launches of
+    `budget` updates per fit after one warm-up launch of 10.  Reports atom
+    updates/s and, for the first fit of each (detector side, more than 75
+    angles) class, the fit kernel's wall-clock ticks (100 MHz) per phase and
+    update: setup, moments + per-angle contraction, angle sums, solve,
+    proposal set-up, energy pass, decision, commit."""
+    Sv = importlib.import_module("2018_dictionary_tomography_amd.serving")
+    F = importlib.import_module("2018_dictionary_tomography_amd.fit")
+    specs = Sv.c5_stream(2000, rate=200.0, seed=2024, n_range=(50, 5000))
+    rng = np.random.default_rng(5)
+    sub = [specs[i] for i in sorted(rng.choice(len(specs), args.steady_fits, replace=False))]
+    first = {}
+    for k, spec in enumerate(sub):
+        first.setdefault((spec.S, spec.nA > 75), k)
+    watched = sorted(first.values())
+    ticks = torch.zeros((len(watched), 16), dtype=torch.int64, device=dev)
+    reqs = []
+    for k, spec in enumerate(sub):
+        start, space, data = Sv.build_request(spec, dev)
+        req = F.FitRequest(start, space, data, [spec.max_iter, 1], record=False)
+        if k in watched:
+            req.slot.phase_ticks = ticks[watched.index(k)].data_ptr()
+        reqs.append(req)
+    torch.cuda.synchronize()
+    raw = b"".join(r.slot_bytes() for r in reqs)
+    table = torch.as_tensor(np.frombuffer(raw, dtype=np.uint8).copy(), device=dev)
+    n = len(reqs)
+    F.run_slots(table, n, 3, 10)
+    torch.cuda.synchronize()
+    u0 = sum(sl.updates for sl in F.read_slots(table, n))
+    ticks.zero_()
+    t0 = time.perf_counter()
+    for _ in range(launches):
+        F.run_slots(table, n, 3, budget)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    slots = F.read_slots(table, n)
+    u1 = sum(sl.updates for sl in slots)
+    per = ticks.cpu().numpy() / 100.0 / (launches * budget)
+    names = ["setup", "moments_contraction", "angle_sums", "solve", "proposal", "energy", "decision", "commit"]
+    return {"fits": n, "updates": int(u1 - u0), "seconds": dt, "updates_per_s": (u1 - u0) / dt,
+            "accepted_frac": sum(sl.accepted for sl in slots) / max(u1, 1),
+            "us_per_update": [{"N": sub[k].N, "angles": sub[k].nA, "side": sub[k].S,
+                               **{nm: round(float(per[i][j]), 1) for j, nm in enumerate(names)}}
+                              for i, k in enumerate(watched)]}
 
 
 def serving_leg(torch, args, dev):
@@ -664,6 +716,9 @@ def main():
         del src, dst
     if args.serve_only:
         print(json.dumps({"serving": serving_leg(torch, args, dev)}))
+        return 0
+    if args.steady_fits > 0:
+        print(json.dumps({"steady": steady_leg(torch, args, dev)}))
         return 0
     rng = np.random.default_rng(args.seed)
     N = args.atoms
