@@ -696,8 +696,8 @@ def gd2d_leg(torch, gtp, dev, iters, cpu_iters):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
-              "multi-GPU sharding is described in DESIGN.md but not run", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (--gpus {args.gpus} requested); the path's sharding over "
+              "several GPUs is a design, not run here", file=sys.stderr)
         return 2
     import torch
 

@@ -35,7 +35,7 @@ class _Stage:
     def __call__(self, sub, mi):
         n = len(sub)
         self.calls.append((n, list(mi)))
-        sub.I.mul_(0.5)  # the stage moves the atoms; the driver must keep them
+        sub.I.mul_(0.5)  # the stage moves the atoms; the greedy loop must keep them
         e0 = float(10 * len(self.calls))
         return np.array([e0, e0 - 1, e0 - 2]), np.array([e0, e0 - 0.5, e0 - 1.5])
 
