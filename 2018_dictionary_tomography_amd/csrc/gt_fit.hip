@@ -716,7 +716,11 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                         prep4<D>(i - PrepCount<D>::N3, W.F);
                 }
                 wave_sync();
-                for (int e = lane; e < NV; e += 64) {
+                // entries 1 .. NV on lanes 0 .. NV - 1 and entry 0 (the
+                // intensity gradient, the cheapest) as lane 0's second item
+                // when NV = 65 > 64
+                for (int q = lane + 1; q <= NV; q += 64) {
+                    const int e = q == NV ? 0 : q;
                     int i = e, l = e;
                     if (e >= P) upper_ij(e - P, P, i, l);
                     xput(prec + (size_t)a * NVS + e, (e < P) ? fin_grad<D>(W.F, e) : fin_hess<D>(W.F, i, l));
@@ -727,7 +731,7 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         GT_FIT_TICK(1);
         GT_FIT_MEET();
         // sum the per-angle records over all angles in one fixed order (angles
-        // a, a + 64, a + 128, a + 192 on lane a, then the wave butterfly); all
+        // a, a + 64, a + 128, a + 192 on lane a, then the wave's DPP sum); all
         // loads first
         {
             constexpr int NE = (NV + FIT_WAVES - 1) / FIT_WAVES;

@@ -541,18 +541,46 @@ __device__ inline void pair_moments_box(const PairRec &rec, const float *p0, con
     wave_sync();  // tab may be rewritten by the caller's next use
 }
 
+// Wave64 sums through DPP row shifts and row broadcasts (VALU only, no LDS
+// permutes): an inclusive scan within each row of 16 lanes, then rows 0-1
+// and 2-3 joined, then the halves; lane 63 ends with the total, which every
+// lane reads back.  The order of the additions is fixed.  Lanes whose DPP
+// source lies outside the row (or whose row is masked) add 0.
+template <int CTRL, int ROWS> __device__ inline float dpp_f32(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROWS, 0xf, false));
+}
+template <int CTRL, int ROWS> __device__ inline double dpp_f64(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 __device__ inline float wave_sum_f32(float v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    v += dpp_f32<0x111, 0xf>(v);  // row_shr:1
+    v += dpp_f32<0x112, 0xf>(v);  // row_shr:2
+    v += dpp_f32<0x114, 0xf>(v);  // row_shr:4
+    v += dpp_f32<0x118, 0xf>(v);  // row_shr:8
+    v += dpp_f32<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_f32<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
 __device__ inline double wave_sum_f64(double v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    v += dpp_f64<0x111, 0xf>(v);
+    v += dpp_f64<0x112, 0xf>(v);
+    v += dpp_f64<0x114, 0xf>(v);
+    v += dpp_f64<0x118, 0xf>(v);
+    v += dpp_f64<0x142, 0xa>(v);
+    v += dpp_f64<0x143, 0xc>(v);
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffll), 63);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
 }  // namespace gt
