@@ -57,7 +57,7 @@ static_assert(NAMAX % 64 == 0, "per-angle sums take whole waves of angles");
 constexpr int DETMAX = GT_FIT_MAX_DETECTOR;
 constexpr int NVS = 65;                          // stride of a per-angle gradient/Hessian record
 constexpr int FIT_UNR = 8;                       // detector rows in flight per lane in the pixel passes
-constexpr int FIT_TROWS = 256;                   // rows of a wave's row table (moments, energy and commit passes)
+constexpr int FIT_TROWS = 192;                   // rows of a wave's row table (moments, energy and commit passes)
 constexpr int FIT_BOXW = 64;                     // columns per chunk of the moments walk
 constexpr double FIT_EPS = 1e-4;                 // NewtonGaussian.py:21
 constexpr double GD_EPS = 1e-8;                  // GD_lib.py:96
@@ -89,6 +89,7 @@ template <int D> struct FitLDS {
     PairRec rec_new[NAMAX];
     WaveC<D> wc[FIT_WAVES];
     float p0[DETMAX], p1[DETMAX];
+    float geo[NAMAX][9];               // own angles' T, w0, w1 (as load_angle reads them), once per launch
     float4 tab[FIT_WAVES][FIT_TROWS];  // per-wave row tables of the pixel passes
     double th_old[10], th_new[10];
     double g[10], H[100];
@@ -114,6 +115,37 @@ __device__ inline int next_angle(int *ctr, int lane)
     int k = 0;
     if (lane == 0) k = atomicAdd(ctr, 1);
     return __builtin_amdgcn_readfirstlane(k);
+}
+
+// The geometry of angle a (T, w0 and, for a 3D atom, w1) as load_angle reads
+// it, into 9 floats of LDS; and back into a PairGeom, with the same values.
+template <int D> __device__ inline void stash_angle(const gt_geom &G, int a, float *o)
+{
+    const int L = G.lenT;
+    for (int i = 0; i < 9; i++) o[i] = 0.0f;
+    for (int i = 0; i < L; i++) {
+        o[i] = G.T[(size_t)a * L + i];
+        o[3 + i] = G.w0[(size_t)a * L + i];
+    }
+    if (D == 3 && L == 3)
+        for (int i = 0; i < 3; i++) o[6 + i] = G.w1[(size_t)a * 3 + i];
+}
+template <int D> __device__ inline void angle_from_stash(const float *o, int L, PairGeom<D> &g)
+{
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        g.T[i] = o[i];
+        g.G[0][i] = o[3 + i];
+        g.G[1][i] = 0.0;
+    }
+    if (D == 3) {
+        if (L == 3) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) g.G[1][i] = o[6 + i];
+        } else {
+            g.G[1][2] = 1.0;  // 2.5D, as load_angle
+        }
+    }
 }
 
 // theta (FP64 values of the FP32 atom) -> PairGeom atom part
@@ -595,6 +627,7 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
     for (int i = tid; i < S.n0; i += FIT_THREADS) L.p0[i] = S.p0[i];
     if (K == 2)
         for (int i = tid; i < S.n1; i += FIT_THREADS) L.p1[i] = S.p1[i];
+    for (int k = tid; k < nOwn; k += FIT_THREADS) stash_angle<D>(Gm, member + G * k, L.geo[k]);
     if (tid == 0) {
         FitCtl &c = L.c;
         c.it = S.it;
@@ -664,7 +697,7 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         for (int k = tid; k < nOwn; k += FIT_THREADS) {
             PairGeom<D> g;
             atom_from_theta<D>(L.th_old, iso, g);
-            load_angle<D>(Gm, member + G * k, g);
+            angle_from_stash<D>(L.geo[k], Gm.lenT, g);
             pair_setup<D>(g, det, nd, S.sorted, L.rec_old[k]);
         }
         __syncthreads();
@@ -698,7 +731,7 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
                     W.S2[lane - 15] = empty ? 0.0 : (double)mine;
                 if (lane == 0) {
                     atom_from_theta<D>(L.th_old, iso, W.g);
-                    load_angle<D>(Gm, a, W.g);
+                    angle_from_stash<D>(L.geo[k], Gm.lenT, W.g);
                 }
                 wave_sync();
                 const double pstar[2] = {(double)rec.c_hi[0] + (double)rec.c_lo[0],
@@ -920,7 +953,7 @@ __global__ __launch_bounds__(FIT_THREADS) void fit_kernel(gt_fit_slot *__restric
         for (int k = tid; k < nOwn; k += FIT_THREADS) {
             PairGeom<D> g;
             atom_from_theta<D>(L.th_new, iso, g);
-            load_angle<D>(Gm, member + G * k, g);
+            angle_from_stash<D>(L.geo[k], Gm.lenT, g);
             pair_setup<D>(g, det, nd, S.sorted, L.rec_new[k]);
         }
         __syncthreads();
